@@ -1,0 +1,254 @@
+"""MI355X (gfx950) blocked Bunch-Kaufman LDL^T: Python bindings.
+
+Thin ctypes bindings over the C ABI in ``include/bbk_ldlt.h``
+(``libbbk_ldlt.so``, built in-tree by ``blockedbunchkaufmanldlt_amd.build``).
+Device buffers are torch tensors on the current HIP device; calls are
+queued on torch's current stream.  There is no CPU fallback: every entry
+point raises if the HIP library is missing or no GPU is present.
+
+Layout: an order-n matrix is a row-major ``(n, lda)`` tensor whose upper
+triangle (i <= j) holds A -- the byte layout of LAPACK's column-major
+lower triangle.  Results use the reference's compact encoding (see the C
+header): D on the diagonal, D(i,i+1) for 2x2 pivots, L(j,i) at (i,j)
+otherwise, plus ``perm`` and ``piv`` int32 vectors.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libbbk_ldlt.so")
+CXXLIB_PATH = os.path.join(PKG, "libnaive_block_ldlt.so")
+
+BBK_OK = 0
+BBK_ERR_HIP = 1
+BBK_ERR_ALLOC = 2
+BBK_ERR_UNSUPPORTED = 3
+BBK_ERR_NOT_CONVERGED = 4
+
+_lib = None
+
+
+class BbkError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bbk error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """Load libbbk_ldlt.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: run `python -m blockedbunchkaufmanldlt_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.bbk_create.argtypes = [ctypes.POINTER(vp), vp]
+    L.bbk_destroy.argtypes = [vp]
+    L.bbk_synchronize.argtypes = [vp]
+    L.bbk_last_error.argtypes = [vp]
+    L.bbk_last_error.restype = ctypes.c_char_p
+    for name in ("bbk_dsytrf", "bbk_ssytrf"):
+        getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp]
+    L.bbk_dsytrf_host.argtypes = [vp, i32, i32, vp, i64, vp, vp]
+    for name in ("bbk_dsytrs", "bbk_ssytrs"):
+        getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp, vp, i64]
+    L.bbk_dsyberr.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, vp]
+    L.bbk_dsysv_mixed.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, f64,
+                                  i32, ip, ctypes.POINTER(f64)]
+    L.bbk_dsysv.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64,
+                            ctypes.POINTER(f64)]
+    for name in ("bbk_dsytrf_batched", "bbk_ssytrf_batched"):
+        getattr(L, name).argtypes = [vp, i32, ip, vp, vp, vp, vp]
+    L.bbk_kernel_scale_1x1.argtypes = [vp, i32, f64, vp, i32, vp, i32]
+    L.bbk_kernel_scale_2x2.argtypes = [vp, i32, f64, f64, f64, vp, i32, vp, i32,
+                                       vp, i32, vp, i32]
+    L.bbk_kernel_update_trailing.argtypes = [vp, i32, i32, i32, vp, i32, vp, i32,
+                                             vp, i32]
+    L.bbk_dgen_synth.argtypes = [vp, i32, ctypes.c_uint64, f64, i32, vp, i64]
+    L.bbk_set_timing.argtypes = [vp, i32]
+    L.bbk_last_timing.argtypes = [vp, ctypes.POINTER(f64), ctypes.POINTER(f64),
+                                  ctypes.POINTER(f64), ip]
+    _lib = L
+    return L
+
+
+def _ptr(t) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+class Handle:
+    """A bbk handle bound to torch's current stream on the current device."""
+
+    def __init__(self, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("blockedbunchkaufmanldlt_amd needs a HIP GPU")
+        self._torch = torch
+        st = stream if stream is not None else torch.cuda.current_stream()
+        self.stream = st
+        h = ctypes.c_void_p()
+        rc = lib().bbk_create(ctypes.byref(h), ctypes.c_void_p(st.cuda_stream))
+        if rc != 0:
+            raise BbkError(rc, "bbk_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bbk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            msg = lib().bbk_last_error(self.h).decode()
+            raise BbkError(rc, f"{what}: {msg}")
+
+    def synchronize(self):
+        self.check(lib().bbk_synchronize(self.h), "synchronize")
+
+    def set_timing(self, on: bool):
+        lib().bbk_set_timing(self.h, int(on))
+
+    def last_timing(self):
+        p, u, s = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int()
+        lib().bbk_last_timing(self.h, ctypes.byref(p), ctypes.byref(u),
+                              ctypes.byref(s), ctypes.byref(n))
+        return {"panel_ms": p.value, "update_ms": u.value, "swap_ms": s.value,
+                "panels": n.value}
+
+    # ---- factorization ------------------------------------------------
+    def sytrf(self, A, nb: int = 0, perm=None, piv=None):
+        """Factor A in place (float64 -> bbk_dsytrf, float32 -> bbk_ssytrf).
+        Returns (perm, piv) int32 device tensors."""
+        torch = self._torch
+        n, lda = A.shape[0], A.stride(0)
+        assert A.is_cuda and A.stride(1) == 1 and A.shape[1] >= n
+        if perm is None:
+            perm = torch.empty(n, dtype=torch.int32, device=A.device)
+        if piv is None:
+            piv = torch.empty(n, dtype=torch.int32, device=A.device)
+        if A.dtype == torch.float64:
+            rc = lib().bbk_dsytrf(self.h, n, nb, _ptr(A), lda, _ptr(perm), _ptr(piv))
+        elif A.dtype == torch.float32:
+            rc = lib().bbk_ssytrf(self.h, n, nb, _ptr(A), lda, _ptr(perm), _ptr(piv))
+        else:
+            raise TypeError("A must be float64 or float32")
+        self.check(rc, "sytrf")
+        return perm, piv
+
+    def sytrs(self, F, perm, piv, B):
+        """Solve in place: B (n, nrhs) row-major -> X."""
+        torch = self._torch
+        n, lda = F.shape[0], F.stride(0)
+        nrhs = B.shape[1] if B.dim() == 2 else 1
+        ldb = B.stride(0) if B.dim() == 2 else 1
+        fn = lib().bbk_dsytrs if F.dtype == torch.float64 else lib().bbk_ssytrs
+        self.check(fn(self.h, n, nrhs, _ptr(F), lda, _ptr(perm), _ptr(piv), _ptr(B), ldb),
+                   "sytrs")
+        return B
+
+    def syberr(self, A, B, X):
+        torch = self._torch
+        n, lda = A.shape[0], A.stride(0)
+        nrhs = B.shape[1] if B.dim() == 2 else 1
+        ldb = B.stride(0) if B.dim() == 2 else 1
+        ldx = X.stride(0) if X.dim() == 2 else 1
+        eta = torch.empty(nrhs, dtype=torch.float64, device=A.device)
+        self.check(lib().bbk_dsyberr(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb,
+                                     _ptr(X), ldx, _ptr(eta)), "syberr")
+        return eta
+
+    def sysv_mixed(self, A, B, X, tol: float = 1e-12, max_iter: int = 30):
+        n, lda = A.shape[0], A.stride(0)
+        nrhs = B.shape[1] if B.dim() == 2 else 1
+        ldb = B.stride(0) if B.dim() == 2 else 1
+        ldx = X.stride(0) if X.dim() == 2 else 1
+        it = ctypes.c_int()
+        be = ctypes.c_double()
+        rc = lib().bbk_dsysv_mixed(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb, _ptr(X),
+                                   ldx, tol, max_iter, ctypes.byref(it), ctypes.byref(be))
+        if rc not in (0, BBK_ERR_NOT_CONVERGED):
+            self.check(rc, "sysv_mixed")
+        return {"iters": it.value, "berr": be.value, "converged": rc == 0}
+
+    def sysv(self, A, B, X):
+        n, lda = A.shape[0], A.stride(0)
+        nrhs = B.shape[1] if B.dim() == 2 else 1
+        ldb = B.stride(0) if B.dim() == 2 else 1
+        ldx = X.stride(0) if X.dim() == 2 else 1
+        be = ctypes.c_double()
+        self.check(lib().bbk_dsysv(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb, _ptr(X),
+                                   ldx, ctypes.byref(be)), "sysv")
+        return be.value
+
+    def sytrf_batched(self, mats: Sequence, perms: Sequence, pivs: Sequence):
+        """Factor a list of (n_i, lda_i) tensors (all float64 or all float32)."""
+        torch = self._torch
+        count = len(mats)
+        dev = mats[0].device
+        ns = (ctypes.c_int * count)(*[m.shape[0] for m in mats])
+        aptr = torch.tensor([m.data_ptr() for m in mats], dtype=torch.int64, device=dev)
+        lds = torch.tensor([m.stride(0) for m in mats], dtype=torch.int64, device=dev)
+        pp = torch.tensor([p.data_ptr() for p in perms], dtype=torch.int64, device=dev)
+        vp = torch.tensor([p.data_ptr() for p in pivs], dtype=torch.int64, device=dev)
+        fn = lib().bbk_dsytrf_batched if mats[0].dtype == torch.float64 else \
+            lib().bbk_ssytrf_batched
+        self.check(fn(self.h, count, ns, _ptr(aptr), _ptr(lds), _ptr(pp), _ptr(vp)),
+                   "sytrf_batched")
+        # keep the pointer arrays alive until the queued work has run
+        self._keep = (aptr, lds, pp, vp)
+
+    def gen_synth(self, n: int, seed: int, scale: float = 1.0, zero_diag_stride: int = 0,
+                  lda: Optional[int] = None, dtype=None):
+        torch = self._torch
+        lda = lda or n
+        A = torch.empty((n, lda), dtype=torch.float64, device="cuda")
+        self.check(lib().bbk_dgen_synth(self.h, n, seed, scale, zero_diag_stride,
+                                        _ptr(A), lda), "gen_synth")
+        return A
+
+    # ---- kernels.h counterparts -----------------------------------------
+    def kernel_scale_1x1(self, count, pivot, src, src_stride, dst, dst_stride):
+        self.check(lib().bbk_kernel_scale_1x1(self.h, count, pivot, _ptr(src), src_stride,
+                                              _ptr(dst), dst_stride), "scale_1x1")
+
+    def kernel_scale_2x2(self, count, a, b, c, s1, s1s, s2, s2s, d1, d1s, d2, d2s):
+        self.check(lib().bbk_kernel_scale_2x2(self.h, count, a, b, c, _ptr(s1), s1s,
+                                              _ptr(s2), s2s, _ptr(d1), d1s, _ptr(d2), d2s),
+                   "scale_2x2")
+
+    def kernel_update_trailing(self, rows, cols, inner, left, ls, work, ws, target, ts):
+        self.check(lib().bbk_kernel_update_trailing(self.h, rows, cols, inner, _ptr(left), ls,
+                                                    _ptr(work), ws, _ptr(target), ts),
+                   "update_trailing")
+
+
+_default: Optional[Handle] = None
+
+
+def default_handle() -> Handle:
+    global _default
+    if _default is None:
+        _default = Handle()
+    return _default
+
+
+def padded_ld(n: int) -> int:
+    """Leading dimension used for device copies: a multiple of 64 elements
+    that is not a multiple of 1024 (avoids one-channel column walks)."""
+    ld = (n + 63) // 64 * 64
+    if ld % 1024 == 0:
+        ld += 64
+    return ld

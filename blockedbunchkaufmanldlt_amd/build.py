@@ -1,0 +1,74 @@
+"""Build the gfx950 library in-tree.
+
+    python -m blockedbunchkaufmanldlt_amd.build
+
+produces
+
+* ``blockedbunchkaufmanldlt_amd/libbbk_ldlt.so`` -- HIP kernels + C ABI
+  (include/bbk_ldlt.h), compiled with hipcc for gfx950 only;
+* ``blockedbunchkaufmanldlt_amd/libnaive_block_ldlt.so`` -- the C++20
+  drop-in API (namespace naive_block_ldlt, include/naive_block_ldlt/*.h),
+  plain C++ over the C ABI.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import glob
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libbbk_ldlt.so")
+CXXLIB = os.path.join(PKG, "libnaive_block_ldlt.so")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 instructions stay off (hardware notes)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "recognized feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("command failed: " + " ".join(cmd))
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in srcs]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        futs = [ex.submit(_run, [HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+                for s, o in zip(srcs, objs)]
+        for f in futs:
+            out = f.result()
+            if verbose and out:
+                print(out)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+
+    cxx_srcs = sorted(glob.glob(os.path.join(CSRC, "naive_block_ldlt", "*.cc")))
+    if cxx_srcs:
+        tmp = CXXLIB + ".tmp"
+        _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared",
+              "-I" + os.path.join(ROOT, "include"),
+              "-I" + os.path.join(ROOT, "include", "naive_block_ldlt"),
+              "-o", tmp] + cxx_srcs + ["-L" + PKG, "-lbbk_ldlt", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, CXXLIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

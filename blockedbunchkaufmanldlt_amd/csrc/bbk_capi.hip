@@ -1,0 +1,327 @@
+// bbk_capi.hip -- the C ABI (include/bbk_ldlt.h): handle, workspace and
+// the host-side panel loop that drives the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bbk_ldlt.h"
+#include "bbk_handle.hpp"
+#include "bbk_kernels.hpp"
+
+using bbk::PanelArgs;
+using bbk::PanelExchange;
+using bbk::PanelState;
+
+namespace bbk {
+
+int Handle::fail(int code, const char* what) {
+  last_error = what;
+  return code;
+}
+
+int Handle::hip_fail(hipError_t e, const char* what) {
+  last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return BBK_ERR_HIP;
+}
+
+// Device scratch grows monotonically and is reused across calls.
+int Handle::reserve(size_t bytes) {
+  if (bytes <= ws_bytes) return BBK_OK;
+  if (ws) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    hipFree(ws);
+    ws = nullptr;
+    ws_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&ws, bytes);
+  if (e != hipSuccess) {
+    ws = nullptr;
+    return hip_fail(e, "hipMalloc(workspace)");
+  }
+  ws_bytes = bytes;
+  return BBK_OK;
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Workspace carve-up for one factorization of order n in type T.
+struct FactorScratch {
+  unsigned* bar;
+  double* x;
+  int* xi;
+  int* state;
+  int* swaps;
+  int* plan;
+  void* Wg;
+  int64_t ldw;
+};
+
+template <typename T>
+static size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return base ? base + o : nullptr;
+  };
+  const int64_t ldw = align_up(static_cast<size_t>(n), 64);
+  FactorScratch f;
+  f.bar = reinterpret_cast<unsigned*>(take(256));
+  f.x = reinterpret_cast<double*>(take(sizeof(double) * PanelExchange::kDoubles));
+  f.xi = reinterpret_cast<int*>(take(sizeof(int) * kMaxPanelGroups));
+  f.state = reinterpret_cast<int*>(take(sizeof(int) * PanelState::kInts));
+  f.swaps = reinterpret_cast<int*>(take(sizeof(int) * 2 * (kPanelMaxWidth + 1)));
+  f.plan = reinterpret_cast<int*>(take(sizeof(int) * 4 * (kPanelMaxWidth + 1)));
+  f.Wg = take(sizeof(T) * (kPanelMaxWidth + 1) * static_cast<size_t>(ldw));
+  f.ldw = ldw;
+  if (fs) *fs = f;
+  return off;
+}
+
+int default_panel_width(int n, int elem_bytes) {
+  if (n <= 256) return 32;
+  if (n <= 1024 || elem_bytes == 8) return 64;
+  return 128;
+}
+
+template <typename T>
+int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
+              int* piv) {
+  if (n <= 0) return -2;
+  if (nb < 0) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!perm) return -6;
+  if (!piv) return -7;
+  int b = nb == 0 ? default_panel_width(n, sizeof(T)) : nb;
+  b = std::min(b, PanelCfg<T>::kMaxWidth);
+  int max_groups = 0;
+  hipError_t e = panel_max_groups<T>(&max_groups);
+  if (e != hipSuccess) return h->hip_fail(e, "occupancy query");
+  const int groups0 = (n + kPanelThreads - 1) / kPanelThreads;
+  if (groups0 > std::min(max_groups, kMaxPanelGroups))
+    return h->fail(BBK_ERR_UNSUPPORTED,
+                   "matrix order exceeds the co-resident panel grid");
+  FactorScratch fs;
+  const size_t bytes = factor_scratch_bytes<T>(n, nullptr, nullptr);
+  int rc = h->reserve(bytes);
+  if (rc != BBK_OK) return rc;
+  factor_scratch_bytes<T>(n, &fs, reinterpret_cast<char*>(h->ws));
+
+  hipStream_t st = h->stream;
+  launch_init(n, perm, piv, fs.state, st);
+  h->timing_reset();
+  int panels = 0;
+  for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
+    const int groups = (n - s_nom + kPanelThreads - 1) / kPanelThreads;
+    PanelArgs pa;
+    pa.n = n;
+    pa.b = b;
+    pa.lda = lda;
+    pa.state = fs.state;
+    pa.swaps = fs.swaps;
+    pa.perm = perm;
+    pa.piv = piv;
+    pa.bar = fs.bar;
+    pa.x = fs.x;
+    pa.xi = fs.xi;
+    pa.Wg = fs.Wg;
+    pa.ldw = fs.ldw;
+    h->timing_mark(Handle::kPanel, true);
+    launch_zero_u32(fs.bar, 1, st);
+    e = launch_panel<T>(A, pa, groups, st);
+    if (e != hipSuccess) return h->hip_fail(e, "panel launch");
+    h->timing_mark(Handle::kPanel, false);
+    // Rows above this panel: at most s_nom + panels (each earlier panel
+    // can widen by one column).
+    const int rows_bound = std::min(n, s_nom + panels);
+    if (s_nom > 0) {
+      h->timing_mark(Handle::kSwap, true);
+      launch_swap_rows<T>(A, lda, fs.state, fs.swaps, fs.plan, rows_bound, st);
+      h->timing_mark(Handle::kSwap, false);
+    }
+    h->timing_mark(Handle::kUpdate, true);
+    launch_trailing_update<T>(A, lda, reinterpret_cast<const T*>(fs.Wg), fs.ldw,
+                              fs.state, n, n - s_nom - b, st);
+    h->timing_mark(Handle::kUpdate, false);
+  }
+  h->panels = panels;
+  e = hipGetLastError();
+  if (e != hipSuccess) return h->hip_fail(e, "kernel launch");
+  return BBK_OK;
+}
+
+template int factorize<double>(Handle*, int, int, double*, int64_t, int*, int*);
+template int factorize<float>(Handle*, int, int, float*, int64_t, int*, int*);
+
+// ---- optional event timing of the kernel groups -------------------------
+void Handle::timing_reset() {
+  for (auto& v : marks) {
+    for (hipEvent_t ev : v) hipEventDestroy(ev);
+    v.clear();
+  }
+}
+
+void Handle::timing_mark(int group, bool begin) {
+  if (!timing) return;
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  hipEventRecord(ev, stream);
+  marks[group].push_back(ev);
+  (void)begin;
+}
+
+double Handle::timing_total(int group) {
+  double total = 0.0;
+  std::vector<hipEvent_t>& v = marks[group];
+  for (size_t i = 0; i + 1 < v.size(); i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess) total += ms;
+  }
+  return total;
+}
+
+Handle::~Handle() {
+  for (auto& v : marks)
+    for (hipEvent_t ev : v) hipEventDestroy(ev);
+  if (ws) hipFree(ws);
+  if (own_stream && stream) hipStreamDestroy(stream);
+}
+
+}  // namespace bbk
+
+using bbk::Handle;
+
+extern "C" {
+
+int bbk_create(bbk_handle_t* handle, void* stream) {
+  if (!handle) return -1;
+  Handle* h = new Handle();
+  hipError_t e = hipGetDevice(&h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return BBK_ERR_HIP;
+  }
+  if (stream) {
+    h->stream = reinterpret_cast<hipStream_t>(stream);
+    h->own_stream = false;
+  } else {
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete h;
+      return BBK_ERR_HIP;
+    }
+    h->own_stream = true;
+  }
+  *handle = reinterpret_cast<bbk_handle_t>(h);
+  return BBK_OK;
+}
+
+int bbk_destroy(bbk_handle_t handle) {
+  if (!handle) return -1;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  hipStreamSynchronize(h->stream);
+  delete h;
+  return BBK_OK;
+}
+
+int bbk_synchronize(bbk_handle_t handle) {
+  if (!handle) return -1;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+  return BBK_OK;
+}
+
+const char* bbk_last_error(bbk_handle_t handle) {
+  if (!handle) return "null handle";
+  return reinterpret_cast<Handle*>(handle)->last_error.c_str();
+}
+
+int bbk_dsytrf(bbk_handle_t handle, int n, int nb, double* d_A, int64_t lda,
+               int* d_perm, int* d_piv) {
+  if (!handle) return -1;
+  return bbk::factorize<double>(reinterpret_cast<Handle*>(handle), n, nb, d_A,
+                                lda, d_perm, d_piv);
+}
+
+int bbk_ssytrf(bbk_handle_t handle, int n, int nb, float* d_A, int64_t lda,
+               int* d_perm, int* d_piv) {
+  if (!handle) return -1;
+  return bbk::factorize<float>(reinterpret_cast<Handle*>(handle), n, nb, d_A,
+                               lda, d_perm, d_piv);
+}
+
+int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
+                    int64_t lda, int* perm, int* piv) {
+  if (!handle) return -1;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  if (n <= 0) return -2;
+  if (nb < 0) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!perm) return -6;
+  if (!piv) return -7;
+  // Device copy with a padded leading dimension (a power-of-two row pitch
+  // would put a column walk on one HBM channel).
+  int64_t ldd = (n + 63) / 64 * 64;
+  if (ldd % 1024 == 0) ldd += 64;
+  double* dA = nullptr;
+  int* dperm = nullptr;
+  int* dpiv = nullptr;
+  hipError_t e = hipMalloc(&dA, sizeof(double) * static_cast<size_t>(ldd) * n);
+  if (e == hipSuccess) e = hipMalloc(&dperm, sizeof(int) * n);
+  if (e == hipSuccess) e = hipMalloc(&dpiv, sizeof(int) * n);
+  int rc = BBK_OK;
+  if (e != hipSuccess) {
+    rc = h->hip_fail(e, "hipMalloc");
+  } else {
+    e = hipMemcpy2DAsync(dA, sizeof(double) * ldd, A, sizeof(double) * lda,
+                         sizeof(double) * n, n, hipMemcpyHostToDevice,
+                         h->stream);
+    if (e != hipSuccess) rc = h->hip_fail(e, "hipMemcpy2D H2D");
+    if (rc == BBK_OK) rc = bbk::factorize<double>(h, n, nb, dA, ldd, dperm, dpiv);
+    if (rc == BBK_OK) {
+      e = hipMemcpy2DAsync(A, sizeof(double) * lda, dA, sizeof(double) * ldd,
+                           sizeof(double) * n, n, hipMemcpyDeviceToHost,
+                           h->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(perm, dperm, sizeof(int) * n, hipMemcpyDeviceToHost,
+                           h->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(piv, dpiv, sizeof(int) * n, hipMemcpyDeviceToHost,
+                           h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) rc = h->hip_fail(e, "D2H copy");
+    }
+  }
+  hipStreamSynchronize(h->stream);
+  if (dA) hipFree(dA);
+  if (dperm) hipFree(dperm);
+  if (dpiv) hipFree(dpiv);
+  return rc;
+}
+
+int bbk_set_timing(bbk_handle_t handle, int enabled) {
+  if (!handle) return -1;
+  reinterpret_cast<Handle*>(handle)->timing = enabled != 0;
+  return BBK_OK;
+}
+
+int bbk_last_timing(bbk_handle_t handle, double* panel_ms, double* update_ms,
+                    double* swap_ms, int* panels) {
+  if (!handle) return -1;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  if (panel_ms) *panel_ms = h->timing_total(Handle::kPanel);
+  if (update_ms) *update_ms = h->timing_total(Handle::kUpdate);
+  if (swap_ms) *swap_ms = h->timing_total(Handle::kSwap);
+  if (panels) *panels = h->panels;
+  return BBK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// bbk_kernels.hpp -- kernel launch interface between the C-ABI layer
+// (bbk_capi.hip) and the kernel translation units.
+#pragma once
+
+#include "bbk_internal.hpp"
+
+namespace bbk {
+
+struct PanelArgs {
+  int n;
+  int b;            // target panel width (<= kPanelMaxWidth)
+  int64_t lda;
+  int* state;       // PanelState ints
+  int* swaps;       // 2 * (kPanelMaxWidth + 1) ints: interchanges (a, r)
+  int* perm;
+  int* piv;
+  unsigned* bar;    // grid barrier counter, zero at launch
+  double* x;        // PanelExchange::kDoubles
+  int* xi;          // kMaxPanelGroups partial argmax indices
+  void* Wg;         // (kPanelMaxWidth + 1) x ldw elements of T
+  int64_t ldw;
+};
+
+template <typename T>
+hipError_t launch_panel(T* A, const PanelArgs& args, int groups,
+                        hipStream_t stream);
+template <typename T>
+hipError_t panel_max_groups(int* groups);
+template <typename T>
+void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
+                            const int* state, int n, int m_bound,
+                            hipStream_t stream);
+template <typename T>
+void launch_swap_rows(T* A, int64_t lda, const int* state, const int* swaps,
+                      int* plan, int rows_bound, hipStream_t stream);
+void launch_init(int n, int* perm, int* piv, int* state, hipStream_t stream);
+void launch_zero_u32(unsigned* p, int count, hipStream_t stream);
+
+}  // namespace bbk

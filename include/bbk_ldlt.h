@@ -1,0 +1,166 @@
+/*
+ * bbk_ldlt.h -- C ABI of the MI355X (gfx950) blocked Bunch-Kaufman LDL^T
+ * library (libbbk_ldlt.so).
+ *
+ * Matrix layout.  Every matrix argument is row-major with leading
+ * dimension lda >= n, and only its UPPER triangle (i <= j) is read and
+ * written.  Row-major upper is byte-identical to LAPACK's column-major
+ * lower ('L'), so a LAPACK caller passes its 'L' array unchanged.
+ *
+ * Output encoding (identical to the reference's compact factor storage,
+ * include/block_ldlt_decomposition.h:25-30 and
+ * src/block_ldlt_decomposition.cc:633-704,746-775):
+ *   perm[i]  : (L D L^T)(i, j) = A(perm[i], perm[j])   -- Permutation()
+ *   piv[i]   : 1 for a 1x1 pivot; 2, -2 at i, i+1 for a 2x2 -- PivotTypes()
+ *   A(i, i)  : D(i, i);  A(i, i+1) = D(i, i+1) when piv[i] == 2
+ *   A(i, j)  : L(j, i) for every other i < j (L unit lower, L(i+1,i) = 0
+ *              inside a 2x2).
+ * Pivot tests, 2x2 inversion and |d| <= DBL_EPSILON singular handling
+ * follow src/block_ldlt_decomposition.cc:21-26,369-390 in binary64 for
+ * every precision.
+ *
+ * Errors.  Functions return BBK_OK (0), a negative value -i when argument
+ * i (1-based, handle = 1) is invalid (LAPACK convention), or a positive
+ * BBK_ERR_* code.  The C++ layer (naive_block_ldlt) maps these to the
+ * reference's exception types.
+ *
+ * Threading.  A handle owns one HIP stream and its workspace; calls on one
+ * handle must not overlap.  All device-pointer functions are asynchronous
+ * on the handle's stream unless stated otherwise.
+ */
+#ifndef BBK_LDLT_H_
+#define BBK_LDLT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBK_OK 0
+#define BBK_ERR_HIP 1          /* a HIP runtime call failed */
+#define BBK_ERR_ALLOC 2        /* device/host allocation failed */
+#define BBK_ERR_UNSUPPORTED 3  /* size beyond what the kernels handle */
+#define BBK_ERR_NOT_CONVERGED 4 /* refinement stopped above tolerance */
+
+typedef struct bbk_handle* bbk_handle_t;
+
+/* Create a handle on the current HIP device.  stream may be NULL (the
+ * handle then creates its own default-priority stream). */
+int bbk_create(bbk_handle_t* handle, void* stream);
+int bbk_destroy(bbk_handle_t handle);
+/* Block until all work queued on the handle's stream has finished. */
+int bbk_synchronize(bbk_handle_t handle);
+/* Text of the last error seen by this handle (never NULL). */
+const char* bbk_last_error(bbk_handle_t handle);
+
+/* ---- plain factorization (device pointers, asynchronous) -------------
+ * Replaces BlockLdltDecomposition::Factorize()
+ * (include/block_ldlt_decomposition.h:22, src/block_ldlt_decomposition.cc:50-59)
+ * for kCompactOnly storage.  nb is the panel width (0 = tuned default);
+ * results are the same Bunch-Kaufman decisions for any nb (a 2x2 that
+ * straddles a panel edge widens that panel by one column instead of
+ * being deferred).  d_A: n x lda; d_perm, d_piv: n ints. */
+int bbk_dsytrf(bbk_handle_t handle, int n, int nb, double* d_A, int64_t lda,
+               int* d_perm, int* d_piv);
+/* FP32 form: same algorithm on binary32 data with binary32 MFMA
+ * accumulation; pivot tests and 2x2 inverses in binary64. */
+int bbk_ssytrf(bbk_handle_t handle, int n, int nb, float* d_A, int64_t lda,
+               int* d_perm, int* d_piv);
+
+/* Host-buffer convenience (synchronous): copies A (row-major, lda, upper
+ * triangle) to the device, factors it, and copies the compact factors,
+ * perm and piv back into the same host arrays.  This is the entry the C++
+ * drop-in layer uses. */
+int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
+                    int64_t lda, int* perm, int* piv);
+
+/* ---- solves (new; no reference counterpart) -------------------------
+ * Solve A X = B with compact factors from bbk_?sytrf.  B: n x nrhs,
+ * row-major with leading dimension ldb >= nrhs; overwritten by X. */
+int bbk_dsytrs(bbk_handle_t handle, int n, int nrhs, const double* d_F,
+               int64_t lda, const int* d_perm, const int* d_piv, double* d_B,
+               int64_t ldb);
+int bbk_ssytrs(bbk_handle_t handle, int n, int nrhs, const float* d_F,
+               int64_t lda, const int* d_perm, const int* d_piv, float* d_B,
+               int64_t ldb);
+
+/* Normwise backward error per right-hand side, computed in binary64:
+ * eta_q = ||b_q - A x_q||_inf / (||A||_inf ||x_q||_inf + ||b_q||_inf).
+ * A: symmetric, upper triangle of d_A.  Writes nrhs doubles to d_eta. */
+int bbk_dsyberr(bbk_handle_t handle, int n, int nrhs, const double* d_A,
+                int64_t lda, const double* d_B, int64_t ldb,
+                const double* d_X, int64_t ldx, double* d_eta);
+
+/* Mixed-precision solve: factor a binary32 copy of A, solve in binary32,
+ * refine in binary64 (residual b - A x and the correction accumulation)
+ * until max_q eta_q <= tol or max_iter corrections.  d_A is not modified.
+ * Synchronous.  iters/berr (host pointers, may be NULL) receive the number
+ * of corrections applied and the final max backward error.  Returns
+ * BBK_ERR_NOT_CONVERGED if tol was not reached (X still holds the best
+ * iterate). */
+int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A,
+                    int64_t lda, const double* d_B, int64_t ldb, double* d_X,
+                    int64_t ldx, double tol, int max_iter, int* iters,
+                    double* berr);
+/* Plain binary64 solve for comparison: factor a copy, solve, report the
+ * backward error.  Synchronous. */
+int bbk_dsysv(bbk_handle_t handle, int n, int nrhs, const double* d_A,
+              int64_t lda, const double* d_B, int64_t ldb, double* d_X,
+              int64_t ldx, double* berr);
+
+/* ---- batched factorization (new) -------------------------------------
+ * Factor count independent matrices of uneven sizes in one persistent
+ * launch.  h_n: host array of sizes; d_A: device array of device
+ * pointers; d_lda: device array of leading dimensions; d_perm/d_piv:
+ * device arrays of device pointers (n ints each).  Matrices are scheduled
+ * largest first over a device work queue; the panel width per size class
+ * comes from the autotuned table (bbk_tune_batched). */
+int bbk_dsytrf_batched(bbk_handle_t handle, int count, const int* h_n,
+                       double* const* d_A, const int64_t* d_lda,
+                       int* const* d_perm, int* const* d_piv);
+int bbk_ssytrf_batched(bbk_handle_t handle, int count, const int* h_n,
+                       float* const* d_A, const int64_t* d_lda,
+                       int* const* d_perm, int* const* d_piv);
+
+/* ---- kernels.h counterparts (device pointers, asynchronous) ----------
+ * Element strides, no return value semantics of the reference become
+ * status codes.  Same arithmetic as src/kernels.cc:11-91 (non-MKL path). */
+/* kernels::ScaleOneByOnePivot, include/kernels.h:6-8 */
+int bbk_kernel_scale_1x1(bbk_handle_t handle, int count, double pivot,
+                         const double* d_src, int src_stride, double* d_dst,
+                         int dst_stride);
+/* kernels::ScaleTwoByTwoPivot, include/kernels.h:10-17 */
+int bbk_kernel_scale_2x2(bbk_handle_t handle, int count, double a, double b,
+                         double c, const double* d_s1, int s1_stride,
+                         const double* d_s2, int s2_stride, double* d_d1,
+                         int d1_stride, double* d_d2, int d2_stride);
+/* kernels::UpdateTrailingBlock, include/kernels.h:19-23:
+ * target(i,j) -= sum_k left(k,i) * work(k,j) on MFMA. */
+int bbk_kernel_update_trailing(bbk_handle_t handle, int rows, int cols,
+                               int inner, const double* d_left,
+                               int left_stride, const double* d_work,
+                               int work_stride, double* d_target,
+                               int target_stride);
+
+/* ---- synthetic inputs (bench/test plumbing) ---------------------------
+ * A(i,j) = A(j,i) = scale * u(seed,i,j), u uniform in [-1,1) from
+ * SplitMix64 keyed by (seed, i, j); A(i,i) = 0 when zero_diag_stride > 0
+ * and i % zero_diag_stride == 0.  Writes the full n x n (row-major, lda). */
+int bbk_dgen_synth(bbk_handle_t handle, int n, uint64_t seed, double scale,
+                   int zero_diag_stride, double* d_A, int64_t lda);
+
+/* Kernel timing of the last bbk_?sytrf call (milliseconds, measured with
+ * HIP events on the handle's stream): panel kernels, trailing updates,
+ * row interchanges.  Any pointer may be NULL.  Valid after synchronize. */
+int bbk_last_timing(bbk_handle_t handle, double* panel_ms, double* update_ms,
+                    double* swap_ms, int* panels);
+/* Enable/disable per-kernel event timing (off by default). */
+int bbk_set_timing(bbk_handle_t handle, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBK_LDLT_H_ */
