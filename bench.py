@@ -1,0 +1,221 @@
+#!/usr/bin/env python
+"""Headline benchmark: blocked Bunch-Kaufman LDL^T on one MI355X.
+
+Workload (n = 16384): A = Q diag(lambda) Q^T with Q from the QR of a
+seeded Gaussian matrix, |lambda| log-spaced over [1e-6, 1] with seeded
+random signs (condition 1e6), b ~ N(0, 1).  A "step" is one plain FP64
+factorization (bbk_dsytrf) of a resident copy of A.
+
+Printed JSON line:
+  value      GFLOP/s of the FP64 factorization, n^3/3 convention
+  roofline   dominant kernel vs its MI355X roofline (HIP events, live)
+  fp64 / mixed  time to backward error <= 1e-12 for A x = b:
+             FP64 factor + solve, and FP32 factor + FP64 refinement
+  cpu_baseline  the reference's own Factorize() (built from its sources,
+             OpenMP on all host cores) on a bounded sample of the same
+             matrix family
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
+PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
+PEAK_HBM = 8.0e12     # bytes/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def c3_matrix(torch, n, seed, lda, kappa=1e6):
+    """A = Q diag(lambda) Q^T, Q from QR of a seeded Gaussian (on the GPU)."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    G = torch.randn((n, n), generator=gen, dtype=torch.float64, device="cuda")
+    Q, _ = torch.linalg.qr(G)
+    del G
+    i = torch.arange(n, dtype=torch.float64, device="cuda")
+    lam = torch.pow(torch.tensor(10.0, dtype=torch.float64, device="cuda"),
+                    -math.log10(kappa) * i / max(n - 1, 1))
+    sgn = torch.randint(0, 2, (n,), generator=gen, device="cuda").to(torch.float64) * 2 - 1
+    A = (Q * (lam * sgn)) @ Q.T.contiguous()
+    del Q
+    A = 0.5 * (A + A.T)
+    out = torch.zeros((n, lda), dtype=torch.float64, device="cuda")
+    out[:, :n] = A
+    del A
+    b = torch.randn((n, 1), generator=gen, dtype=torch.float64, device="cuda")
+    return out, b
+
+
+def cpu_baseline(n_sample=4096, bs=128):
+    """The reference's Factorize() (oracle/_ref, OpenMP, all host cores) on a
+    bounded sample; falls back to the oracle restatement if the reference
+    checker was not built."""
+    import numpy as np
+    import oracle as O
+    A = O.gen_synth_symmetric(n_sample, 424242, 1.0, 7)
+    cores = os.cpu_count() or 1
+    try:
+        if not O.ref_available():
+            raise FileNotFoundError
+        secs = O.ref_time_factorize(A, bs)
+        kind = "reference"
+        what = "reference BlockLdltDecomposition::Factorize() (kCompactOnly, OpenMP)"
+    except Exception:
+        t0 = time.perf_counter()
+        O.dsytrf(A, bs, omp=True)
+        secs = time.perf_counter() - t0
+        kind = "port"
+        what = "oracle restatement of Factorize() (OpenMP)"
+    gf = n_sample ** 3 / 3.0 / secs / 1e9
+    return {"value": round(gf, 3), "unit": "GFLOP/s", "cores": cores, "kind": kind,
+            "sample": f"{what}, n={n_sample}, bs={bs}, uniform[-1,1] entries, "
+                      f"{secs:.2f} s on {cores} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="factorizations only (for rocprofv3 runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the factorization's pivot chain is not sharded across GPUs)", file=sys.stderr)
+        return 2
+
+    import torch
+    import blockedbunchkaufmanldlt_amd as B
+
+    n = args.n
+    lda = B.padded_ld(n)
+    h = B.Handle()
+    log(f"generating n={n} eigenvalue-constructed matrix")
+    A0, b = c3_matrix(torch, n, args.seed, lda)
+    torch.cuda.synchronize()
+
+    # ---- timed FP64 factorizations on resident copies -------------------
+    W, K = args.warmup, args.steps
+    copies = [A0.clone() for _ in range(W + K)]
+    torch.cuda.synchronize()
+    for k in range(W):
+        log(f"warmup {k}")
+        h.sytrf(copies[k])
+    h.synchronize()
+    h.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        h.sytrf(copies[W + k])
+    h.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tm = h.last_timing()
+    h.set_timing(False)
+    log(f"timed {K} factorizations: {elapsed * 1e3 / K:.2f} ms each; kernels {tm}")
+    del copies
+    torch.cuda.empty_cache()
+
+    flops = n ** 3 / 3.0
+    ms_per_step = elapsed * 1e3 / K
+    gflops = flops / (elapsed / K) / 1e9
+
+    # roofline of the kernels (per-launch averages over the timed region)
+    panels_per_step = tm["panels"] / K
+    upd_ms = tm["update_ms"] / K
+    pan_ms = tm["panel_ms"] / K
+    upd_flops = flops  # the trailing updates carry the n^3/3 flops
+    upd_launches = panels_per_step
+    upd_avg_ms = upd_ms / max(upd_launches, 1)
+    upd_achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    # panel kernel: algorithmic bytes = read A rows k (+ candidate rows) and
+    # write U rows + W rows, 8 B each: about 4 n^2 / 2 * 8 B per factorization
+    pan_bytes = 4.0 * n * n / 2 * 8
+    pan_achieved = pan_bytes / (pan_ms * 1e-3) / 1e9 if pan_ms > 0 else 0.0
+    roof_update = {"kernel": "bk_trailing_update_kernel<double>", "bound": "mfma",
+                   "achieved": round(upd_achieved, 3), "peak": PEAK_FP64 / 1e12,
+                   "unit": "TFLOP/s", "frac": round(upd_achieved / (PEAK_FP64 / 1e12), 4),
+                   "traffic": None, "avg_launch_ms": round(upd_avg_ms, 4),
+                   "launches_per_step": upd_launches}
+    roof_panel = {"kernel": "bk_panel_kernel<double>", "bound": "hbm",
+                  "achieved": round(pan_achieved, 3), "peak": PEAK_HBM / 1e9,
+                  "unit": "GB/s", "frac": round(pan_achieved / (PEAK_HBM / 1e9), 5),
+                  "traffic": None, "avg_launch_ms": round(pan_ms / max(panels_per_step, 1), 4),
+                  "us_per_pivot": round(pan_ms * 1e3 / n, 3)}
+    dominant = roof_update if upd_ms >= pan_ms else roof_panel
+
+    # ---- time to backward error 1e-12: FP64 vs mixed ----------------------
+    solve = {"fp64": None, "mixed": None, "speedup_mixed_vs_fp64": None}
+    if not args.no_solve and not args.profile:
+        X = torch.zeros_like(b)
+        try:
+            res = []
+            for rep in range(2):
+                torch.cuda.synchronize(); t0 = time.perf_counter()
+                be = h.sysv(A0, b, X)
+                torch.cuda.synchronize(); res.append((time.perf_counter() - t0, be))
+            t64, be64 = min(res)
+            res = []
+            for rep in range(2):
+                X.zero_()
+                torch.cuda.synchronize(); t0 = time.perf_counter()
+                info = h.sysv_mixed(A0, b, X, tol=1e-12, max_iter=30)
+                torch.cuda.synchronize(); res.append((time.perf_counter() - t0, info))
+            tmx, infomx = min(res, key=lambda r: r[0])
+            solve = {"fp64": {"time_ms": round(t64 * 1e3, 3), "berr": be64},
+                     "mixed": {"time_ms": round(tmx * 1e3, 3), "berr": infomx["berr"],
+                               "iters": infomx["iters"], "converged": infomx["converged"]},
+                     "speedup_mixed_vs_fp64": round(t64 / tmx, 3)}
+        except B.BbkError as e:
+            log(f"solve path unavailable: {e}")
+
+    cpu = None
+    if not args.no_cpu and not args.profile:
+        log("cpu baseline")
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": "GFLOP/s (n^3/3) and time to 1e-12 backward error, mixed vs FP64, 1 GPU",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"FP64 blocked Bunch-Kaufman LDL^T, n={n}, "
+                               f"A=Q diag(lambda) Q^T (cond 1e6)",
+                   "n": n, "lda": lda, "panel_width": 64, "parallelism": "single GPU"},
+        "roofline": dominant,
+        "roofline_update": roof_update,
+        "roofline_panel": roof_panel,
+        "fraction_of_fp64_peak": round(gflops * 1e9 / PEAK_FP64, 4),
+        "time_to_1e-12": solve,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

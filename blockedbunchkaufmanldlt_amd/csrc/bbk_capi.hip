@@ -47,6 +47,24 @@ int Handle::reserve(size_t bytes) {
   return BBK_OK;
 }
 
+int Handle::reserve_buf(void** p, size_t* cur, size_t want, const char* what) {
+  if (want <= *cur) return BBK_OK;
+  if (*p) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cur = 0;
+  }
+  hipError_t e = hipMalloc(p, want);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(e, what);
+  }
+  *cur = want;
+  return BBK_OK;
+}
+
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Workspace carve-up for one factorization of order n in type T.
@@ -115,7 +133,6 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
 
   hipStream_t st = h->stream;
   launch_init(n, perm, piv, fs.state, st);
-  h->timing_reset();
   int panels = 0;
   for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
     const int groups = (n - s_nom + kPanelThreads - 1) / kPanelThreads;
@@ -150,7 +167,7 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
                               fs.state, n, n - s_nom - b, st);
     h->timing_mark(Handle::kUpdate, false);
   }
-  h->panels = panels;
+  if (h->timing) h->panels += panels;
   e = hipGetLastError();
   if (e != hipSuccess) return h->hip_fail(e, "kernel launch");
   return BBK_OK;
@@ -189,8 +206,11 @@ double Handle::timing_total(int group) {
 Handle::~Handle() {
   for (auto& v : marks)
     for (hipEvent_t ev : v) hipEventDestroy(ev);
-  if (ws) hipFree(ws);
-  if (own_stream && stream) hipStreamDestroy(stream);
+  if (ws) (void)hipFree(ws);
+  if (solve_ws) (void)hipFree(solve_ws);
+  if (berr_ws) (void)hipFree(berr_ws);
+  if (mixed_ws) (void)hipFree(mixed_ws);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
 }  // namespace bbk
@@ -309,7 +329,11 @@ int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
 
 int bbk_set_timing(bbk_handle_t handle, int enabled) {
   if (!handle) return -1;
-  reinterpret_cast<Handle*>(handle)->timing = enabled != 0;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  hipStreamSynchronize(h->stream);
+  h->timing_reset();
+  h->panels = 0;
+  h->timing = enabled != 0;
   return BBK_OK;
 }
 

@@ -15,8 +15,16 @@ struct Handle {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  void* ws = nullptr;       // device scratch, grown on demand
+  // Device scratch regions, each grown on demand and kept for reuse.
+  void* ws = nullptr;        // factorization: exchange area, W rows
   size_t ws_bytes = 0;
+  void* solve_ws = nullptr;  // triangular sweeps: flags, Z, V
+  size_t solve_bytes = 0;
+  void* berr_ws = nullptr;   // residual partial sums and norms
+  size_t berr_bytes = 0;
+  void* mixed_ws = nullptr;  // refinement driver: factor copy, r, d
+  size_t mixed_bytes = 0;
+  double* last_anorm_dev = nullptr;
   std::string last_error;
   bool timing = false;
   int panels = 0;
@@ -26,6 +34,10 @@ struct Handle {
   int fail(int code, const char* what);
   int hip_fail(hipError_t e, const char* what);
   int reserve(size_t bytes);
+  int reserve_buf(void** p, size_t* cur, size_t want, const char* what);
+  int reserve_solve(size_t b) { return reserve_buf(&solve_ws, &solve_bytes, b, "solve"); }
+  int reserve_berr(size_t b) { return reserve_buf(&berr_ws, &berr_bytes, b, "residual"); }
+  int reserve_mixed(size_t b) { return reserve_buf(&mixed_ws, &mixed_bytes, b, "refinement"); }
   void timing_reset();
   void timing_mark(int group, bool begin);
   double timing_total(int group);
@@ -36,5 +48,11 @@ int default_panel_width(int n, int elem_bytes);
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
               int* piv);
+template <typename T>
+int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
+          const int* piv, T* B, int64_t ldb);
+int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
+         const double* B, int64_t ldb, const double* X, int64_t ldx, double* R,
+         int64_t ldr, double* d_eta, double anorm, bool want_norm);
 
 }  // namespace bbk

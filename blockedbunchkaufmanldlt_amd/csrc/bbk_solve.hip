@@ -1,13 +1,707 @@
-// bbk_solve.hip -- solves, refinement and batched entry points (in progress).
+// bbk_solve.hip -- solves with the compact factors, the binary64 residual /
+// backward error, and the mixed-precision refinement driver.
+//
+// Solve: x = P L^-T D^-1 L^-1 P^T b with (L D L^T)(i,j) = A(perm[i],perm[j]).
+// L is stored transposed in the upper triangle (L(j,i) = F(i,j), i < j,
+// except F(i,i+1) of a 2x2 pivot, which is D).  Both triangular sweeps
+// run as one cooperative launch each: 64-row blocks, workgroups take
+// blocks in dependency order and publish each finished block with a flag.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
 #include "../../include/bbk_ldlt.h"
 #include "bbk_handle.hpp"
-extern "C" {
-int bbk_dsytrs(bbk_handle_t, int, int, const double*, int64_t, const int*, const int*, double*, int64_t) { return BBK_ERR_UNSUPPORTED; }
-int bbk_ssytrs(bbk_handle_t, int, int, const float*, int64_t, const int*, const int*, float*, int64_t) { return BBK_ERR_UNSUPPORTED; }
-int bbk_dsyberr(bbk_handle_t, int, int, const double*, int64_t, const double*, int64_t, const double*, int64_t, double*) { return BBK_ERR_UNSUPPORTED; }
-int bbk_dsysv_mixed(bbk_handle_t, int, int, const double*, int64_t, const double*, int64_t, double*, int64_t, double, int, int*, double*) { return BBK_ERR_UNSUPPORTED; }
-int bbk_dsysv(bbk_handle_t, int, int, const double*, int64_t, const double*, int64_t, double*, int64_t, double*) { return BBK_ERR_UNSUPPORTED; }
-int bbk_dsytrf_batched(bbk_handle_t, int, const int*, double* const*, const int64_t*, int* const*, int* const*) { return BBK_ERR_UNSUPPORTED; }
-int bbk_ssytrf_batched(bbk_handle_t, int, const int*, float* const*, const int64_t*, int* const*, int* const*) { return BBK_ERR_UNSUPPORTED; }
+#include "bbk_kernels.hpp"
+
+namespace bbk {
+
+constexpr int kSolveBlock = 64;
+constexpr int kSolveThreads = 256;
+constexpr int kSolveMaxRhs = 4;  // right-hand sides per launch
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  int2 u = __builtin_bit_cast(int2, v);
+  u.x = __builtin_amdgcn_readlane(u.x, lane);
+  u.y = __builtin_amdgcn_readlane(u.y, lane);
+  return __builtin_bit_cast(double, u);
 }
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane);
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int lane) {
+  return readlane_d(v, lane);
+}
+template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int lane) {
+  return readlane_f(v, lane);
+}
+
+__device__ __forceinline__ void wait_flag(const int* flag) {
+  if (threadIdx.x == 0) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      __builtin_amdgcn_s_sleep(1);
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void post_flag(int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// L value at (row j, col i), i < j, from the upper-stored factor.
+template <typename T>
+__device__ __forceinline__ T lval(const T* F, int64_t lda, const int* piv,
+                                  int i, int j) {
+  if (j == i + 1 && piv[i] == 2) return T(0);
+  return F[(int64_t)i * lda + j];
+}
+
+// Forward sweep: Z = L^-1 P^T B for nr (<= kSolveMaxRhs) columns.
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
+    int n, int nr, const T* __restrict__ F, int64_t lda,
+    const int* __restrict__ piv, const int* __restrict__ perm,
+    const T* __restrict__ B, int64_t ldb, T* __restrict__ Z, int64_t ldz,
+    int* __restrict__ flags, int nblk) {
+  constexpr int NB = kSolveBlock;
+  __shared__ T zin[NB * kSolveMaxRhs];
+  __shared__ T part[4 * NB * kSolveMaxRhs];
+  const int t = threadIdx.x;
+  const int col = t & (NB - 1), grp = t >> 6;
+  for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
+    const int j0 = J * NB;
+    const int jn = min(NB, n - j0);
+    const int j = j0 + col;
+    T acc[kSolveMaxRhs];
+    for (int q = 0; q < kSolveMaxRhs; ++q) acc[q] = T(0);
+    for (int I = 0; I < J; ++I) {
+      wait_flag(&flags[I]);
+      const int i0 = I * NB;
+      for (int e = t; e < NB * nr; e += kSolveThreads) {
+        const int ii = e / nr, q = e % nr;
+        zin[ii * kSolveMaxRhs + q] = Z[(int64_t)(i0 + ii) * ldz + q];
+      }
+      __syncthreads();
+      T s[kSolveMaxRhs];
+      for (int q = 0; q < kSolveMaxRhs; ++q) s[q] = T(0);
+      if (col < jn) {
+        for (int ii = grp * 16; ii < grp * 16 + 16; ++ii) {
+          const T l = lval(F, lda, piv, i0 + ii, j);
+          for (int q = 0; q < nr; ++q) s[q] += l * zin[ii * kSolveMaxRhs + q];
+        }
+      }
+      for (int q = 0; q < nr; ++q) part[(grp * NB + col) * kSolveMaxRhs + q] = s[q];
+      __syncthreads();
+      if (grp == 0)
+        for (int q = 0; q < nr; ++q)
+          acc[q] += part[(0 * NB + col) * kSolveMaxRhs + q] +
+                    part[(1 * NB + col) * kSolveMaxRhs + q] +
+                    part[(2 * NB + col) * kSolveMaxRhs + q] +
+                    part[(3 * NB + col) * kSolveMaxRhs + q];
+      __syncthreads();
+    }
+    if (grp == 0) {
+      // diagonal block, one wavefront: lane = column
+      T v[kSolveMaxRhs];
+      for (int q = 0; q < kSolveMaxRhs; ++q)
+        v[q] = (col < jn && q < nr) ? B[(int64_t)perm[j] * ldb + q] - acc[q] : T(0);
+      T Lc[NB];
+#pragma unroll
+      for (int ii = 0; ii < NB; ++ii)
+        Lc[ii] = (col < jn && ii < col) ? lval(F, lda, piv, j0 + ii, j) : T(0);
+#pragma unroll
+      for (int ii = 0; ii < NB; ++ii) {
+        if (ii < jn) {
+          for (int q = 0; q < kSolveMaxRhs; ++q) {
+            const T zi = readlane_t<T>(v[q], ii);
+            v[q] -= Lc[ii] * zi;  // Lc[ii] == 0 unless ii < col
+          }
+        }
+      }
+      if (col < jn)
+        for (int q = 0; q < nr; ++q) Z[(int64_t)j * ldz + q] = v[q];
+    }
+    post_flag(&flags[J]);
+  }
+}
+
+// Backward sweep with the D solve fused in front:
+// V = L^-T D^-1 Z, X[perm[i]] = V[i].
+template <typename T>
+__global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
+    int n, int nr, const T* __restrict__ F, int64_t lda,
+    const int* __restrict__ piv, const int* __restrict__ perm,
+    const T* __restrict__ Z, int64_t ldz, T* __restrict__ V, int64_t ldv,
+    T* __restrict__ X, int64_t ldx, int* __restrict__ flags, int nblk) {
+  constexpr int NB = kSolveBlock;
+  __shared__ T vin[NB * kSolveMaxRhs];
+  __shared__ T part[4 * NB * kSolveMaxRhs];
+  const int t = threadIdx.x;
+  const int row = t & (NB - 1), grp = t >> 6;
+  for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
+    const int i0 = I * NB;
+    const int in = min(NB, n - i0);
+    const int i = i0 + row;
+    T acc[kSolveMaxRhs];
+    for (int q = 0; q < kSolveMaxRhs; ++q) acc[q] = T(0);
+    for (int J = nblk - 1; J > I; --J) {
+      wait_flag(&flags[J]);
+      const int jj0 = J * NB;
+      const int jn = min(NB, n - jj0);
+      for (int e = t; e < NB * nr; e += kSolveThreads) {
+        const int jj = e / nr, q = e % nr;
+        vin[jj * kSolveMaxRhs + q] = jj < jn ? V[(int64_t)(jj0 + jj) * ldv + q] : T(0);
+      }
+      __syncthreads();
+      T s[kSolveMaxRhs];
+      for (int q = 0; q < kSolveMaxRhs; ++q) s[q] = T(0);
+      if (row < in) {
+        for (int jj = grp * 16; jj < grp * 16 + 16; ++jj) {
+          if (jj < jn) {
+            const T l = lval(F, lda, piv, i, jj0 + jj);
+            for (int q = 0; q < nr; ++q) s[q] += l * vin[jj * kSolveMaxRhs + q];
+          }
+        }
+      }
+      for (int q = 0; q < nr; ++q) part[(grp * NB + row) * kSolveMaxRhs + q] = s[q];
+      __syncthreads();
+      if (grp == 0)
+        for (int q = 0; q < nr; ++q)
+          acc[q] += part[(0 * NB + row) * kSolveMaxRhs + q] +
+                    part[(1 * NB + row) * kSolveMaxRhs + q] +
+                    part[(2 * NB + row) * kSolveMaxRhs + q] +
+                    part[(3 * NB + row) * kSolveMaxRhs + q];
+      __syncthreads();
+    }
+    if (grp == 0) {
+      T w[kSolveMaxRhs];
+      for (int q = 0; q < kSolveMaxRhs; ++q) w[q] = T(0);
+      if (row < in) {
+        const int pv = piv[i];
+        if (pv == 2 || pv == -2) {
+          // 2x2 block (k, k+1): LAPACK dsytrs's scaled formula, binary64
+          const int k = pv == 2 ? i : i - 1;
+          const double a = F[(int64_t)k * lda + k];
+          const double b = F[(int64_t)k * lda + k + 1];
+          const double c = F[(int64_t)(k + 1) * lda + k + 1];
+          const double akm1 = a / b, ak = c / b;
+          const double denom = akm1 * ak - 1.0;
+          for (int q = 0; q < nr; ++q) {
+            const double bkm1 = (double)Z[(int64_t)k * ldz + q] / b;
+            const double bk = (double)Z[(int64_t)(k + 1) * ldz + q] / b;
+            w[q] = (T)(pv == 2 ? (ak * bkm1 - bk) / denom : (akm1 * bk - bkm1) / denom);
+          }
+        } else {
+          const double d = F[(int64_t)i * lda + i];
+          for (int q = 0; q < nr; ++q) w[q] = (T)((double)Z[(int64_t)i * ldz + q] / d);
+        }
+      }
+      T v[kSolveMaxRhs];
+      for (int q = 0; q < kSolveMaxRhs; ++q) v[q] = w[q] - acc[q];
+      T Lr[NB];
+#pragma unroll
+      for (int jj = 0; jj < NB; ++jj)
+        Lr[jj] = (row < in && jj > row && jj < in) ? lval(F, lda, piv, i, i0 + jj) : T(0);
+#pragma unroll
+      for (int jj = NB - 1; jj >= 0; --jj) {
+        if (jj < in) {
+          for (int q = 0; q < kSolveMaxRhs; ++q) {
+            const T vj = readlane_t<T>(v[q], jj);
+            v[q] -= Lr[jj] * vj;  // Lr[jj] == 0 unless jj > row
+          }
+        }
+      }
+      if (row < in)
+        for (int q = 0; q < nr; ++q) {
+          V[(int64_t)i * ldv + q] = v[q];
+          X[(int64_t)perm[i] * ldx + q] = v[q];
+        }
+    }
+    post_flag(&flags[I]);
+  }
+}
+
+// ---- binary64 residual / backward error --------------------------------
+// Tile (I, J), I <= J, of the upper triangle: row sums sum_j A(i,j) x_j
+// into prow[J][i] and column sums sum_{i<j} A(i,j) x_i into pcol[I][j]
+// (strictly upper part only), plus the same with |A| and x = 1 when
+// with_norm.  Reduced in a fixed order by k_symv_reduce.
+constexpr int kSymvTile = 64;
+
+__global__ __launch_bounds__(256) void k_symv_tiles(
+    int n, int nr, const double* __restrict__ A, int64_t lda,
+    const double* __restrict__ X, int64_t ldx, double* __restrict__ prow,
+    double* __restrict__ pcol, double* __restrict__ pabs_row,
+    double* __restrict__ pabs_col, int ntile, int with_norm) {
+  constexpr int TB = kSymvTile;
+  __shared__ double xs_i[TB * kSolveMaxRhs];
+  __shared__ double xs_j[TB * kSolveMaxRhs];
+  __shared__ double red[4][TB * (kSolveMaxRhs + 1)];
+  int idx = blockIdx.x;
+  int J = static_cast<int>((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while (J * (J + 1) / 2 > idx) --J;
+  while ((J + 1) * (J + 2) / 2 <= idx) ++J;
+  const int I = idx - J * (J + 1) / 2;
+  const int i0 = I * TB, j0 = J * TB;
+  const int t = threadIdx.x;
+  for (int e = t; e < TB * nr; e += 256) {
+    const int r = e / nr, q = e % nr;
+    xs_i[r * kSolveMaxRhs + q] = (i0 + r < n) ? X[(int64_t)(i0 + r) * ldx + q] : 0.0;
+    xs_j[r * kSolveMaxRhs + q] = (j0 + r < n) ? X[(int64_t)(j0 + r) * ldx + q] : 0.0;
+  }
+  __syncthreads();
+  // row sums: thread (r = t & 63, part = t >> 6) covers 16 columns
+  {
+    const int r = t & 63, part = t >> 6;
+    const int i = i0 + r;
+    double s[kSolveMaxRhs] = {0, 0, 0, 0};
+    double sa = 0.0;
+    if (i < n) {
+      for (int c = part * 16; c < part * 16 + 16; ++c) {
+        const int j = j0 + c;
+        if (j < n && j >= i) {
+          const double a = A[(int64_t)i * lda + j];
+          for (int q = 0; q < nr; ++q) s[q] += a * xs_j[c * kSolveMaxRhs + q];
+          sa += fabs(a);
+        }
+      }
+    }
+    for (int q = 0; q < nr; ++q) red[part][r * (kSolveMaxRhs + 1) + q] = s[q];
+    red[part][r * (kSolveMaxRhs + 1) + kSolveMaxRhs] = sa;
+  }
+  __syncthreads();
+  if (t < TB && i0 + t < n) {
+    for (int q = 0; q < nr; ++q) {
+      double v = 0.0;
+      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + q];
+      prow[((int64_t)J * n + i0 + t) * kSolveMaxRhs + q] = v;
+    }
+    if (with_norm) {
+      double v = 0.0;
+      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + kSolveMaxRhs];
+      pabs_row[(int64_t)J * n + i0 + t] = v;
+    }
+  }
+  __syncthreads();
+  // column sums over the strictly upper part: thread (c = t & 63, part)
+  {
+    const int c = t & 63, part = t >> 6;
+    const int j = j0 + c;
+    double s[kSolveMaxRhs] = {0, 0, 0, 0};
+    double sa = 0.0;
+    if (j < n) {
+      for (int r = part * 16; r < part * 16 + 16; ++r) {
+        const int i = i0 + r;
+        if (i < n && i < j) {
+          const double a = A[(int64_t)i * lda + j];
+          for (int q = 0; q < nr; ++q) s[q] += a * xs_i[r * kSolveMaxRhs + q];
+          sa += fabs(a);
+        }
+      }
+    }
+    for (int q = 0; q < nr; ++q) red[part][c * (kSolveMaxRhs + 1) + q] = s[q];
+    red[part][c * (kSolveMaxRhs + 1) + kSolveMaxRhs] = sa;
+  }
+  __syncthreads();
+  if (t < TB && j0 + t < n) {
+    for (int q = 0; q < nr; ++q) {
+      double v = 0.0;
+      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + q];
+      pcol[((int64_t)I * n + j0 + t) * kSolveMaxRhs + q] = v;
+    }
+    if (with_norm) {
+      double v = 0.0;
+      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + kSolveMaxRhs];
+      pabs_col[(int64_t)I * n + j0 + t] = v;
+    }
+  }
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  // non-negative binary64 values order like their bit patterns
+  if (!(v >= 0.0)) v = __builtin_huge_val();  // NaN poisons the norm
+  atomicMax(reinterpret_cast<unsigned long long*>(addr),
+            __double_as_longlong(v));
+}
+
+// y_i = sum_{J >= I(i)} prow[J][i] + sum_{I' <= J(i)} pcol[I'][i] in a fixed
+// order; R = B - Y (binary64); norms into norms[0..]:
+//   norms[q] = max|r_q|, norms[4+q] = max|x_q|, norms[8+q] = max|b_q|,
+//   norms[12] = ||A||_inf (when with_norm).
+__global__ __launch_bounds__(256) void k_symv_reduce(
+    int n, int nr, const double* __restrict__ prow,
+    const double* __restrict__ pcol, const double* __restrict__ pabs_row,
+    const double* __restrict__ pabs_col, const double* __restrict__ Bv,
+    int64_t ldb, const double* __restrict__ X, int64_t ldx,
+    double* __restrict__ R, int64_t ldr, double* __restrict__ norms,
+    int with_norm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int T = (n + kSymvTile - 1) / kSymvTile;
+  const int ti = i / kSymvTile;
+  for (int q = 0; q < nr; ++q) {
+    double y = 0.0;
+    for (int J = ti; J < T; ++J) y += prow[((int64_t)J * n + i) * kSolveMaxRhs + q];
+    for (int I = 0; I <= ti; ++I) y += pcol[((int64_t)I * n + i) * kSolveMaxRhs + q];
+    const double b = Bv[(int64_t)i * ldb + q];
+    const double r = b - y;
+    if (R) R[(int64_t)i * ldr + q] = r;
+    atomic_max_nonneg(&norms[q], fabs(r));
+    atomic_max_nonneg(&norms[4 + q], fabs(X[(int64_t)i * ldx + q]));
+    atomic_max_nonneg(&norms[8 + q], fabs(b));
+  }
+  if (with_norm) {
+    double s = 0.0;
+    for (int J = ti; J < T; ++J) s += pabs_row[(int64_t)J * n + i];
+    for (int I = 0; I <= ti; ++I) s += pabs_col[(int64_t)I * n + i];
+    atomic_max_nonneg(&norms[12], s);
+  }
+}
+
+__global__ void k_berr(int nr, const double* __restrict__ norms,
+                       double anorm_override, double* __restrict__ eta) {
+  const int q = threadIdx.x;
+  if (q >= nr) return;
+  const double anorm = anorm_override >= 0.0 ? anorm_override : norms[12];
+  const double den = anorm * norms[4 + q] + norms[8 + q];
+  eta[q] = den > 0.0 ? norms[q] / den : (norms[q] > 0.0 ? __builtin_huge_val() : 0.0);
+}
+
+__global__ void k_fill_f64(double* p, int count, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+__global__ void k_fill_i32(int* p, int count, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// Upper-triangle cast binary64 -> binary32 (row-major, leading dims).
+__global__ void k_cast_upper(int n, const double* __restrict__ A, int64_t lda,
+                             float* __restrict__ S, int64_t lds) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (i < n && j < n && j >= i) S[(int64_t)i * lds + j] = (float)A[(int64_t)i * lda + j];
+}
+
+template <typename S, typename D>
+__global__ void k_copy_cols(int n, int nr, const S* __restrict__ src,
+                            int64_t lds, D* __restrict__ dst, int64_t ldd) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nr) return;
+  const int i = e / nr, q = e % nr;
+  dst[(int64_t)i * ldd + q] = (D)src[(int64_t)i * lds + q];
+}
+
+// X += (double) Dv
+__global__ void k_axpy_f32_to_f64(int n, int nr, const float* __restrict__ dv,
+                                  int64_t ldd, double* __restrict__ X,
+                                  int64_t ldx) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * nr) return;
+  const int i = e / nr, q = e % nr;
+  X[(int64_t)i * ldx + q] += (double)dv[(int64_t)i * ldd + q];
+}
+
+static unsigned blocks_for(int64_t count, int threads) {
+  return static_cast<unsigned>((count + threads - 1) / threads);
+}
+
+template <typename K>
+static int coop_groups(K kernel, int want) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel),
+                                                   kSolveThreads, 0) != hipSuccess)
+    return 1;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1;
+  return std::max(1, std::min(want, per_cu * cus));
+}
+
+// Scratch for one solve: flags (2 * nblk ints), Z, V (n x kSolveMaxRhs of T).
+template <typename T>
+int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
+          const int* piv, T* B, int64_t ldb) {
+  if (n <= 0) return -2;
+  if (nrhs < 0) return -3;
+  if (!F) return -4;
+  if (lda < n) return -5;
+  if (!perm) return -6;
+  if (!piv) return -7;
+  if (!B) return -8;
+  if (ldb < std::max(1, nrhs)) return -9;
+  if (nrhs == 0) return BBK_OK;
+  const int nblk = (n + kSolveBlock - 1) / kSolveBlock;
+  const size_t flag_bytes = (2 * nblk * sizeof(int) + 255) / 256 * 256;
+  const size_t vec_bytes = ((size_t)n * kSolveMaxRhs * sizeof(T) + 255) / 256 * 256;
+  // solve scratch lives after the factorization scratch region
+  int rc = h->reserve_solve(flag_bytes + 2 * vec_bytes);
+  if (rc != BBK_OK) return rc;
+  char* base = reinterpret_cast<char*>(h->solve_ws);
+  int* flags = reinterpret_cast<int*>(base);
+  T* Z = reinterpret_cast<T*>(base + flag_bytes);
+  T* V = reinterpret_cast<T*>(base + flag_bytes + vec_bytes);
+  const int gf = coop_groups(k_trsv_fwd<T>, nblk);
+  const int gb = coop_groups(k_trsv_bwd<T>, nblk);
+  for (int q0 = 0; q0 < nrhs; q0 += kSolveMaxRhs) {
+    int nr = std::min(kSolveMaxRhs, nrhs - q0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(2 * nblk, 256)), dim3(256), 0, h->stream,
+                       flags, 2 * nblk, 0);
+    const T* Bq = B + q0;
+    T* Xq = B + q0;
+    int64_t ldz = kSolveMaxRhs;
+    {
+      int n_ = n, nr_ = nr, nblk_ = nblk;
+      int64_t lda_ = lda, ldb_ = ldb;
+      const T* F_ = F;
+      const int *piv_ = piv, *perm_ = perm;
+      const T* B_ = Bq;
+      T* Z_ = Z;
+      int* fl = flags;
+      void* args[] = {&n_, &nr_, &F_, &lda_, &piv_, &perm_, &B_, &ldb_, &Z_, &ldz, &fl, &nblk_};
+      hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T>),
+                                                dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
+      if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
+    }
+    {
+      int n_ = n, nr_ = nr, nblk_ = nblk;
+      int64_t lda_ = lda, ldv = kSolveMaxRhs, ldx = ldb;
+      const T* F_ = F;
+      const int *piv_ = piv, *perm_ = perm;
+      const T* Z_ = Z;
+      T* V_ = V;
+      T* X_ = Xq;
+      int* fl = flags + nblk;
+      void* args[] = {&n_, &nr_, &F_, &lda_, &piv_, &perm_, &Z_, &ldz, &V_, &ldv, &X_, &ldx,
+                      &fl, &nblk_};
+      hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T>),
+                                                dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
+      if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
+    }
+  }
+  return BBK_OK;
+}
+
+// Backward error of X for A X = B (binary64).  R (may be NULL) receives
+// B - A X.  anorm: pass < 0 to compute ||A||_inf (written to *anorm_out).
+int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
+         const double* B, int64_t ldb, const double* X, int64_t ldx, double* R,
+         int64_t ldr, double* d_eta, double anorm, bool want_norm) {
+  const int T = (n + kSymvTile - 1) / kSymvTile;
+  const size_t part = (size_t)T * n * kSolveMaxRhs * sizeof(double);
+  const size_t partabs = (size_t)T * n * sizeof(double);
+  const size_t nbytes = 256;
+  int rc = h->reserve_berr(2 * part + 2 * partabs + nbytes);
+  if (rc != BBK_OK) return rc;
+  char* base = reinterpret_cast<char*>(h->berr_ws);
+  double* prow = reinterpret_cast<double*>(base);
+  double* pcol = reinterpret_cast<double*>(base + part);
+  double* parow = reinterpret_cast<double*>(base + 2 * part);
+  double* pacol = reinterpret_cast<double*>(base + 2 * part + partabs);
+  double* norms = reinterpret_cast<double*>(base + 2 * part + 2 * partabs);
+  const int with_norm = want_norm ? 1 : 0;
+  for (int q0 = 0; q0 < nrhs; q0 += kSolveMaxRhs) {
+    const int nr = std::min(kSolveMaxRhs, nrhs - q0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(32), 0, h->stream, norms, 16, 0.0);
+    const int tiles = T * (T + 1) / 2;
+    hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, h->stream, n, nr, A, lda,
+                       X + q0, ldx, prow, pcol, parow, pacol, T, with_norm);
+    hipLaunchKernelGGL(k_symv_reduce, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, nr,
+                       prow, pcol, parow, pacol, B + q0, ldb, X + q0, ldx,
+                       R ? R + q0 : nullptr, ldr, norms, with_norm);
+    hipLaunchKernelGGL(k_berr, dim3(1), dim3(32), 0, h->stream, nr, norms,
+                       want_norm ? -1.0 : anorm, d_eta + q0);
+    if (want_norm && q0 == 0) {
+      // keep ||A||_inf for later calls (copied by the caller if needed)
+      h->last_anorm_dev = norms + 12;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h->hip_fail(e, "backward error kernels");
+  return BBK_OK;
+}
+
+template int solve<double>(Handle*, int, int, const double*, int64_t, const int*, const int*,
+                           double*, int64_t);
+template int solve<float>(Handle*, int, int, const float*, int64_t, const int*, const int*,
+                          float*, int64_t);
+
+}  // namespace bbk
+
+using bbk::Handle;
+
+extern "C" {
+
+int bbk_dsytrs(bbk_handle_t handle, int n, int nrhs, const double* d_F, int64_t lda,
+               const int* d_perm, const int* d_piv, double* d_B, int64_t ldb) {
+  if (!handle) return -1;
+  return bbk::solve<double>(reinterpret_cast<Handle*>(handle), n, nrhs, d_F, lda, d_perm,
+                            d_piv, d_B, ldb);
+}
+
+int bbk_ssytrs(bbk_handle_t handle, int n, int nrhs, const float* d_F, int64_t lda,
+               const int* d_perm, const int* d_piv, float* d_B, int64_t ldb) {
+  if (!handle) return -1;
+  return bbk::solve<float>(reinterpret_cast<Handle*>(handle), n, nrhs, d_F, lda, d_perm,
+                           d_piv, d_B, ldb);
+}
+
+int bbk_dsyberr(bbk_handle_t handle, int n, int nrhs, const double* d_A, int64_t lda,
+                const double* d_B, int64_t ldb, const double* d_X, int64_t ldx,
+                double* d_eta) {
+  if (!handle) return -1;
+  if (n <= 0) return -2;
+  if (nrhs < 0) return -3;
+  if (!d_A) return -4;
+  if (lda < n) return -5;
+  if (!d_B) return -6;
+  if (ldb < std::max(1, nrhs)) return -7;
+  if (!d_X) return -8;
+  if (ldx < std::max(1, nrhs)) return -9;
+  if (!d_eta) return -10;
+  if (nrhs == 0) return BBK_OK;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  return bbk::berr(h, n, nrhs, d_A, lda, d_B, ldb, d_X, ldx, nullptr, 0, d_eta, -1.0, true);
+}
+
+static int check_solve_args(int n, int nrhs, const void* A, int64_t lda, const void* B,
+                            int64_t ldb, const void* X, int64_t ldx) {
+  if (n <= 0) return -2;
+  if (nrhs <= 0) return -3;
+  if (!A) return -4;
+  if (lda < n) return -5;
+  if (!B) return -6;
+  if (ldb < nrhs) return -7;
+  if (!X) return -8;
+  if (ldx < nrhs) return -9;
+  return BBK_OK;
+}
+
+int bbk_dsysv(bbk_handle_t handle, int n, int nrhs, const double* d_A, int64_t lda,
+              const double* d_B, int64_t ldb, double* d_X, int64_t ldx, double* berr_out) {
+  if (!handle) return -1;
+  int rc = check_solve_args(n, nrhs, d_A, lda, d_B, ldb, d_X, ldx);
+  if (rc != BBK_OK) return rc;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  int64_t ldf = (n + 63) / 64 * 64;
+  if (ldf % 1024 == 0) ldf += 64;
+  rc = h->reserve_mixed((size_t)ldf * n * sizeof(double) + 2 * (size_t)n * sizeof(int) +
+                        (size_t)nrhs * sizeof(double) + 1024);
+  if (rc != BBK_OK) return rc;
+  char* base = reinterpret_cast<char*>(h->mixed_ws);
+  double* F = reinterpret_cast<double*>(base);
+  int* perm = reinterpret_cast<int*>(base + (size_t)ldf * n * sizeof(double));
+  int* piv = perm + n;
+  double* eta = reinterpret_cast<double*>(base + (size_t)ldf * n * sizeof(double) +
+                                          (2 * (size_t)n * sizeof(int) + 255) / 256 * 256);
+  hipError_t e = hipMemcpy2DAsync(F, ldf * sizeof(double), d_A, lda * sizeof(double),
+                                  n * sizeof(double), n, hipMemcpyDeviceToDevice, h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "copy A");
+  rc = bbk::factorize<double>(h, n, 0, F, ldf, perm, piv);
+  if (rc != BBK_OK) return rc;
+  hipLaunchKernelGGL((bbk::k_copy_cols<double, double>), dim3(bbk::blocks_for((int64_t)n * nrhs, 256)),
+                     dim3(256), 0, h->stream, n, nrhs, d_B, ldb, d_X, ldx);
+  rc = bbk::solve<double>(h, n, nrhs, F, ldf, perm, piv, d_X, ldx);
+  if (rc != BBK_OK) return rc;
+  rc = bbk::berr(h, n, nrhs, d_A, lda, d_B, ldb, d_X, ldx, nullptr, 0, eta, -1.0, true);
+  if (rc != BBK_OK) return rc;
+  std::vector<double> he(nrhs);
+  e = hipMemcpyAsync(he.data(), eta, nrhs * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "read backward error");
+  if (berr_out) *berr_out = *std::max_element(he.begin(), he.end());
+  return BBK_OK;
+}
+
+int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int64_t lda,
+                    const double* d_B, int64_t ldb, double* d_X, int64_t ldx, double tol,
+                    int max_iter, int* iters, double* berr_out) {
+  if (!handle) return -1;
+  int rc = check_solve_args(n, nrhs, d_A, lda, d_B, ldb, d_X, ldx);
+  if (rc != BBK_OK) return rc;
+  if (max_iter < 0) return -11;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  int64_t ldf = (n + 63) / 64 * 64;
+  if (ldf % 1024 == 0) ldf += 64;
+  const size_t fbytes = ((size_t)ldf * n * sizeof(float) + 255) / 256 * 256;
+  const size_t ibytes = (2 * (size_t)n * sizeof(int) + 255) / 256 * 256;
+  const size_t rbytes = ((size_t)n * nrhs * sizeof(double) + 255) / 256 * 256;
+  const size_t dbytes = ((size_t)n * nrhs * sizeof(float) + 255) / 256 * 256;
+  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + 1024);
+  if (rc != BBK_OK) return rc;
+  char* base = reinterpret_cast<char*>(h->mixed_ws);
+  float* S = reinterpret_cast<float*>(base);
+  int* perm = reinterpret_cast<int*>(base + fbytes);
+  int* piv = perm + n;
+  double* R = reinterpret_cast<double*>(base + fbytes + ibytes);
+  float* Dv = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes);
+  double* eta = reinterpret_cast<double*>(base + fbytes + ibytes + rbytes + dbytes);
+  // FP32 copy of the upper triangle, FP32 factorization
+  dim3 cg(bbk::blocks_for(n, 256), n);
+  hipLaunchKernelGGL(bbk::k_cast_upper, cg, dim3(256), 0, h->stream, n, d_A, lda, S, ldf);
+  rc = bbk::factorize<float>(h, n, 0, S, ldf, perm, piv);
+  if (rc != BBK_OK) return rc;
+  // x0 = solve32(b)
+  const unsigned eb = bbk::blocks_for((int64_t)n * nrhs, 256);
+  hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream, n,
+                     nrhs, d_B, ldb, Dv, (int64_t)nrhs);
+  rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs);
+  if (rc != BBK_OK) return rc;
+  hipLaunchKernelGGL((bbk::k_copy_cols<float, double>), dim3(eb), dim3(256), 0, h->stream, n,
+                     nrhs, Dv, (int64_t)nrhs, d_X, ldx);
+  std::vector<double> he(nrhs);
+  double anorm = -1.0;
+  int it = 0;
+  double best = INFINITY;
+  for (;;) {
+    // r = b - A x and eta in binary64
+    rc = bbk::berr(h, n, nrhs, d_A, lda, d_B, ldb, d_X, ldx, R, nrhs, eta, anorm, anorm < 0.0);
+    if (rc != BBK_OK) return rc;
+    if (anorm < 0.0) {
+      hipError_t e = hipMemcpyAsync(&anorm, h->last_anorm_dev, sizeof(double),
+                                    hipMemcpyDeviceToHost, h->stream);
+      if (e != hipSuccess) return h->hip_fail(e, "read anorm");
+    }
+    hipError_t e = hipMemcpyAsync(he.data(), eta, nrhs * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "read backward error");
+    best = *std::max_element(he.begin(), he.end());
+    if (best <= tol || it >= max_iter || !(best == best)) break;
+    // d = solve32(r); x += d
+    hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream, n,
+                       nrhs, R, (int64_t)nrhs, Dv, (int64_t)nrhs);
+    rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs);
+    if (rc != BBK_OK) return rc;
+    hipLaunchKernelGGL(bbk::k_axpy_f32_to_f64, dim3(eb), dim3(256), 0, h->stream, n, nrhs, Dv,
+                       (int64_t)nrhs, d_X, ldx);
+    ++it;
+  }
+  if (iters) *iters = it;
+  if (berr_out) *berr_out = best;
+  return best <= tol ? BBK_OK : BBK_ERR_NOT_CONVERGED;
+}
+
+int bbk_dsytrf_batched(bbk_handle_t, int, const int*, double* const*, const int64_t*,
+                       int* const*, int* const*) {
+  return BBK_ERR_UNSUPPORTED;
+}
+int bbk_ssytrf_batched(bbk_handle_t, int, const int*, float* const*, const int64_t*,
+                       int* const*, int* const*) {
+  return BBK_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

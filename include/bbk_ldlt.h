@@ -151,12 +151,15 @@ int bbk_kernel_update_trailing(bbk_handle_t handle, int rows, int cols,
 int bbk_dgen_synth(bbk_handle_t handle, int n, uint64_t seed, double scale,
                    int zero_diag_stride, double* d_A, int64_t lda);
 
-/* Kernel timing of the last bbk_?sytrf call (milliseconds, measured with
- * HIP events on the handle's stream): panel kernels, trailing updates,
- * row interchanges.  Any pointer may be NULL.  Valid after synchronize. */
+/* Kernel timing accumulated over every bbk_?sytrf call since timing was
+ * last enabled (milliseconds, HIP events recorded on the handle's stream
+ * around each kernel group): panel kernels, trailing updates, row
+ * interchanges, and the number of panels.  Any pointer may be NULL.
+ * Valid after bbk_synchronize. */
 int bbk_last_timing(bbk_handle_t handle, double* panel_ms, double* update_ms,
                     double* swap_ms, int* panels);
-/* Enable/disable per-kernel event timing (off by default). */
+/* Enable/disable per-kernel event timing (off by default); either call
+ * clears the accumulated totals. */
 int bbk_set_timing(bbk_handle_t handle, int enabled);
 
 #ifdef __cplusplus

@@ -51,7 +51,7 @@ def lib():
         L.bko_dsytrf.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip,
                                  _ip, _ip, ctypes.POINTER(Decision)]
         L.bko_dsytrf_omp.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip,
-                                     _ip, _ip, _ip]
+                                     _ip, _ip, _ip, ctypes.POINTER(Decision)]
         L.bko_relative_residual.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip]
         L.bko_relative_residual.restype = ctypes.c_double
         L.bko_sampled_residual.argtypes = [ctypes.c_int, _dp, _dp, _ip, _ip,
@@ -137,7 +137,7 @@ class Factorization:
         self.residual = residual
 
 
-def dsytrf(A: np.ndarray, bs: int, trace: bool = False, omp: bool = False) -> Factorization:
+def dsytrf(A: np.ndarray, bs: int, trace: bool = False, omp: Optional[bool] = None) -> Factorization:
     """Oracle factorization of the symmetric matrix whose upper triangle is A's."""
     n = A.shape[0]
     F = np.ascontiguousarray(np.triu(A), dtype=np.float64)
@@ -148,8 +148,11 @@ def dsytrf(A: np.ndarray, bs: int, trace: bool = False, omp: bool = False) -> Fa
     ps = np.zeros(nb, dtype=np.int32)
     pe = np.zeros(nb, dtype=np.int32)
     tr = (Decision * n)() if trace else None
+    if omp is None:
+        omp = n >= 400
     if omp:
-        rc = lib().bko_dsytrf_omp(n, bs, _d(F), _d(W), _i(perm), _i(piv), _i(ps), _i(pe))
+        rc = lib().bko_dsytrf_omp(n, bs, _d(F), _d(W), _i(perm), _i(piv), _i(ps), _i(pe),
+                                  tr if trace else None)
     else:
         rc = lib().bko_dsytrf(n, bs, _d(F), _d(W), _i(perm), _i(piv), _i(ps), _i(pe),
                               tr if trace else None)

@@ -258,13 +258,23 @@ static void update_trailing(bko_ctx* c, int panel) {
   if (e <= s) return;
   int par = c->omp && ((c->nb - panel - 1) * (c->nb - panel) / 2 >
                        BKO_MIN_PARALLEL_BLOCKS);
-#pragma omp parallel for schedule(dynamic, 8) if (par)
-  for (int i = e; i < n; ++i) {
-    for (int j = i; j < n; ++j) {
-      double update = 0.0;
-      for (int p = s; p < e; ++p) update += AT(c->F, p, i) * AT(c->W, p, j);
-      AT(c->F, i, j) -= update;
+#pragma omp parallel if (par)
+  {
+    /* Per-row accumulator: each element still sums p = s..e-1 in order
+     * into its own 'update' before the single subtraction, exactly as the
+     * reference's i-j-k loop; only the loop nest is reordered. */
+    double* upd = (double*)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(dynamic, 8)
+    for (int i = e; i < n; ++i) {
+      for (int j = i; j < n; ++j) upd[j] = 0.0;
+      for (int p = s; p < e; ++p) {
+        const double l = AT(c->F, p, i);
+        const double* wrow = &AT(c->W, p, 0);
+        for (int j = i; j < n; ++j) upd[j] += l * wrow[j];
+      }
+      for (int j = i; j < n; ++j) AT(c->F, i, j) -= upd[j];
     }
+    free(upd);
   }
 }
 
@@ -380,8 +390,8 @@ int bko_dsytrf(int n, int bs, double* F, double* W, int* perm, int* piv,
 }
 
 int bko_dsytrf_omp(int n, int bs, double* F, double* W, int* perm, int* piv,
-                   int* panel_start, int* panel_end) {
-  return factorize(n, bs, F, W, perm, piv, panel_start, panel_end, NULL, 1);
+                   int* panel_start, int* panel_end, bko_decision* trace) {
+  return factorize(n, bs, F, W, perm, piv, panel_start, panel_end, trace, 1);
 }
 
 /* ---------------------------------------------------------------------- */

@@ -54,7 +54,7 @@ int bko_dsytrf(int n, int bs, double* F, double* W, int* perm, int* piv,
  * Bitwise identical to bko_dsytrf: every region writes disjoint elements
  * with serial inner sums.  Used as the CPU baseline. */
 int bko_dsytrf_omp(int n, int bs, double* F, double* W, int* perm, int* piv,
-                   int* panel_start, int* panel_end);
+                   int* panel_start, int* panel_end, bko_decision* trace);
 
 /* Dense L (unit lower) and D (block diagonal) from compact factors,
  * restating ExtractRegularFactorsFromCompact

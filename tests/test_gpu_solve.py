@@ -1,0 +1,121 @@
+"""GPU tests of the solve, backward-error and mixed-precision refinement
+entry points (no reference counterpart; pinned by the oracle's solve,
+numpy's dense solve and the Rigal-Gaches backward-error formula)."""
+import numpy as np
+import pytest
+
+from gpu_util import to_device
+
+pytestmark = pytest.mark.gpu
+
+
+def eig_matrix(n, kappa, seed):
+    """A = Q diag(lambda) Q^T, |lambda| log-spaced over [1/kappa, 1], random signs."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = 10.0 ** (-np.log10(kappa) * np.arange(n) / max(n - 1, 1))
+    lam *= np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    A = (Q * lam) @ Q.T
+    return 0.5 * (A + A.T)
+
+
+def berr_np(A, B, X):
+    R = B - A @ X
+    return (np.abs(R).max(0) / (np.abs(A).sum(1).max() * np.abs(X).max(0) + np.abs(B).max(0)))
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 2), (64, 1), (65, 3), (200, 5), (1000, 4)])
+def test_dsytrs(oracle, handle, n, nrhs):
+    import torch
+    A = oracle.gen_synth_symmetric(n, 40 + n)
+    T = to_device(A)
+    perm, piv = handle.sytrf(T, 0)
+    rng = np.random.default_rng(n)
+    Bh = rng.standard_normal((n, nrhs))
+    Bd = torch.from_numpy(Bh.copy()).cuda()
+    handle.sytrs(T, perm, piv, Bd)
+    handle.synchronize()
+    X = Bd.cpu().numpy()
+    # against the oracle's solve on the GPU's own factors
+    F = np.triu(T[:, :n].cpu().numpy())
+    f = oracle.Factorization(n, 0, F, perm.cpu().numpy().astype(np.int32),
+                             piv.cpu().numpy().astype(np.int32))
+    Xo = oracle.dsytrs(f, Bh)
+    assert np.allclose(X, Xo, rtol=1e-10, atol=1e-11)
+    assert berr_np(A, Bh, X).max() < 1e-14
+
+
+@pytest.mark.parametrize("n", [100, 777])
+def test_ssytrs(oracle, handle, n):
+    import torch
+    A = oracle.gen_synth_symmetric(n, 90 + n)
+    T = to_device(A, "float32")
+    perm, piv = handle.sytrf(T, 0)
+    Bh = np.random.default_rng(2).standard_normal((n, 2))
+    Bd = torch.from_numpy(Bh.astype(np.float32)).cuda()
+    handle.sytrs(T, perm, piv, Bd)
+    handle.synchronize()
+    X = Bd.double().cpu().numpy()
+    assert berr_np(A, Bh, X).max() < 1e-5
+
+
+def test_dsyberr(oracle, handle):
+    import torch
+    n = 300
+    A = oracle.gen_synth_symmetric(n, 5)
+    rng = np.random.default_rng(9)
+    B = rng.standard_normal((n, 6))
+    X = np.linalg.solve(A, B) + 1e-9 * rng.standard_normal((n, 6))
+    eta = handle.syberr(to_device(A), torch.from_numpy(B).cuda(), torch.from_numpy(X).cuda())
+    handle.synchronize()
+    assert np.allclose(eta.cpu().numpy(), berr_np(A, B, X), rtol=1e-10)
+
+
+@pytest.mark.parametrize("n,kappa", [(256, 1e4), (1500, 1e6)])
+def test_dsysv_fp64_and_mixed(oracle, handle, n, kappa):
+    import torch
+    A = eig_matrix(n, kappa, 7 + n)
+    b = np.random.default_rng(11).standard_normal((n, 1))
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()  # full matrix; only the upper is read
+    bd = torch.from_numpy(b).cuda()
+    X = torch.zeros_like(bd)
+    be64 = handle.sysv(Ad, bd, X)
+    x64 = X.cpu().numpy()
+    assert be64 < 1e-14 and berr_np(A, b, x64).max() < 1e-14
+    X.zero_()
+    info = handle.sysv_mixed(Ad, bd, X, tol=1e-12, max_iter=30)
+    xm = X.cpu().numpy()
+    assert info["converged"] and info["berr"] <= 1e-12
+    assert berr_np(A, b, xm).max() <= 1.5e-12
+    # forward error bounded by condition number times backward error
+    xr = np.linalg.solve(A, b)
+    assert np.abs(xm - xr).max() / np.abs(xr).max() < kappa * 1e-11
+
+
+def test_mixed_reports_non_convergence(oracle, handle):
+    """Condition 1e12 is beyond what FP32 factors can refine: the driver must
+    say so rather than return a silently wrong answer."""
+    import torch
+    n = 200
+    A = eig_matrix(n, 1e12, 3)
+    b = np.random.default_rng(4).standard_normal((n, 1))
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()
+    X = torch.zeros((n, 1), dtype=torch.float64, device="cuda")
+    info = handle.sysv_mixed(Ad, torch.from_numpy(b).cuda(), X, tol=1e-15, max_iter=3)
+    assert not info["converged"] and info["iters"] == 3
+
+
+def test_multi_rhs_mixed(oracle, handle):
+    """Batched-config shape: 4 right-hand sides refined together."""
+    import torch
+    n = 512
+    A = eig_matrix(n, 1e5, 21)
+    B = np.random.default_rng(22).standard_normal((n, 4))
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()
+    X = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    info = handle.sysv_mixed(Ad, torch.from_numpy(B).cuda(), X, tol=1e-12)
+    assert info["converged"]
+    assert berr_np(A, B, X.cpu().numpy()).max() <= 1.5e-12
