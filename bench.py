@@ -214,6 +214,12 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    # release device state before interpreter teardown (profilers attached
+    # to the process write their results after this point)
+    del A0, b
+    h.close()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     return 0
 
 

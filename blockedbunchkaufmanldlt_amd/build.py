@@ -25,6 +25,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libbbk_ldlt.so")
 CXXLIB = os.path.join(PKG, "libnaive_block_ldlt.so")
+DUMP = os.path.join(BUILD, "drop_in_dump")
+PROFDRV = os.path.join(BUILD, "bbk_profile_driver")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -67,6 +69,17 @@ def build(verbose: bool = False) -> str:
               "-I" + os.path.join(ROOT, "include", "naive_block_ldlt"),
               "-o", tmp] + cxx_srcs + ["-L" + PKG, "-lbbk_ldlt", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, CXXLIB)
+        # parity driver of the drop-in API (tests/cpp/drop_in_dump.cc)
+        _run(["g++", "-std=c++20", "-O2",
+              "-I" + os.path.join(ROOT, "include"),
+              "-I" + os.path.join(ROOT, "include", "naive_block_ldlt"),
+              "-o", DUMP, os.path.join(ROOT, "tests", "cpp", "drop_in_dump.cc"),
+              "-L" + PKG, "-lnaive_block_ldlt", "-lbbk_ldlt",
+              "-Wl,-rpath," + os.path.relpath(PKG, BUILD).join(["$ORIGIN/", ""])])
+        # profiler driver of the C ABI (tools/bbk_profile_driver.cc)
+        _run([HIPCC, "--offload-arch=gfx950", "-O2", "-I" + os.path.join(ROOT, "include"),
+              "-o", PROFDRV, os.path.join(PKG, "tools", "bbk_profile_driver.cc"),
+              "-L" + PKG, "-lbbk_ldlt", "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

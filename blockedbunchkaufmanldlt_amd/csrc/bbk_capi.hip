@@ -13,7 +13,7 @@
 #include "bbk_kernels.hpp"
 
 using bbk::PanelArgs;
-using bbk::PanelExchange;
+
 using bbk::PanelState;
 
 namespace bbk {
@@ -69,9 +69,8 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Workspace carve-up for one factorization of order n in type T.
 struct FactorScratch {
-  unsigned* bar;
-  double* x;
-  int* xi;
+  XRec* xrec;
+  size_t xrec_bytes;
   int* state;
   int* swaps;
   int* plan;
@@ -88,10 +87,10 @@ static size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
     return base ? base + o : nullptr;
   };
   const int64_t ldw = align_up(static_cast<size_t>(n), 64);
+  const int groups = (n + kPanelThreads - 1) / kPanelThreads;
   FactorScratch f;
-  f.bar = reinterpret_cast<unsigned*>(take(256));
-  f.x = reinterpret_cast<double*>(take(sizeof(double) * PanelExchange::kDoubles));
-  f.xi = reinterpret_cast<int*>(take(sizeof(int) * kMaxPanelGroups));
+  f.xrec_bytes = sizeof(XRec) * panel_exchange_records(groups, sizeof(T));
+  f.xrec = reinterpret_cast<XRec*>(take(f.xrec_bytes));
   f.state = reinterpret_cast<int*>(take(sizeof(int) * PanelState::kInts));
   f.swaps = reinterpret_cast<int*>(take(sizeof(int) * 2 * (kPanelMaxWidth + 1)));
   f.plan = reinterpret_cast<int*>(take(sizeof(int) * 4 * (kPanelMaxWidth + 1)));
@@ -122,16 +121,25 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
   hipError_t e = panel_max_groups<T>(&max_groups);
   if (e != hipSuccess) return h->hip_fail(e, "occupancy query");
   const int groups0 = (n + kPanelThreads - 1) / kPanelThreads;
-  if (groups0 > std::min(max_groups, kMaxPanelGroups))
+  if (groups0 > std::min(max_groups, kMaxPanelGroupsLds))
     return h->fail(BBK_ERR_UNSUPPORTED,
                    "matrix order exceeds the co-resident panel grid");
   FactorScratch fs;
   const size_t bytes = factor_scratch_bytes<T>(n, nullptr, nullptr);
+  void* old_ws = h->ws;
   int rc = h->reserve(bytes);
   if (rc != BBK_OK) return rc;
   factor_scratch_bytes<T>(n, &fs, reinterpret_cast<char*>(h->ws));
 
   hipStream_t st = h->stream;
+  // Exchange-record tags must never repeat within the live record area:
+  // each launch takes a fresh range of 1024 tags; fresh or wrapped areas
+  // are cleared first.
+  if (h->ws != old_ws || h->epoch > (1 << 30)) {
+    e = hipMemsetAsync(h->ws, 0, h->ws_bytes, st);
+    if (e != hipSuccess) return h->hip_fail(e, "clear exchange area");
+    h->epoch = 1024;
+  }
   launch_init(n, perm, piv, fs.state, st);
   int panels = 0;
   for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
@@ -144,13 +152,12 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     pa.swaps = fs.swaps;
     pa.perm = perm;
     pa.piv = piv;
-    pa.bar = fs.bar;
-    pa.x = fs.x;
-    pa.xi = fs.xi;
+    pa.xrec = fs.xrec;
+    pa.epoch = h->epoch;
+    h->epoch += 1024;
     pa.Wg = fs.Wg;
     pa.ldw = fs.ldw;
     h->timing_mark(Handle::kPanel, true);
-    launch_zero_u32(fs.bar, 1, st);
     e = launch_panel<T>(A, pa, groups, st);
     if (e != hipSuccess) return h->hip_fail(e, "panel launch");
     h->timing_mark(Handle::kPanel, false);

@@ -25,6 +25,7 @@ struct Handle {
   void* mixed_ws = nullptr;  // refinement driver: factor copy, r, d
   size_t mixed_bytes = 0;
   double* last_anorm_dev = nullptr;
+  int epoch = 1024;          // next panel-exchange tag range
   std::string last_error;
   bool timing = false;
   int panels = 0;

@@ -18,6 +18,15 @@ constexpr double kDblEps = 2.220446049250313080847e-16;
 constexpr int kPanelThreads = 256;
 constexpr int kPanelMaxWidth = 128;  // largest panel width of any type
 constexpr int kMaxPanelGroups = 1024;
+constexpr int kMaxPanelGroupsLds = 256;  // panel grid limit (n <= 65536)
+
+// 16-byte exchange record {tag, aux, value} of the panel kernel; a record
+// is valid for exchange e when tag == e.
+struct __attribute__((aligned(16))) XRec {
+  int tag;
+  int aux;
+  double v;
+};
 
 // Widest panel whose W rows (width + 1) x 256 fit the 160 KB LDS.
 template <typename T>

@@ -14,12 +14,14 @@ struct PanelArgs {
   int* swaps;       // 2 * (kPanelMaxWidth + 1) ints: interchanges (a, r)
   int* perm;
   int* piv;
-  unsigned* bar;    // grid barrier counter, zero at launch
-  double* x;        // PanelExchange::kDoubles
-  int* xi;          // kMaxPanelGroups partial argmax indices
+  XRec* xrec;       // panel_exchange_records(G, sizeof(T)) records
+  int epoch;        // record tags of this launch are epoch + 1, epoch + 2, ...
   void* Wg;         // (kPanelMaxWidth + 1) x ldw elements of T
   int64_t ldw;
 };
+
+// Exchange records the panel kernel needs for a grid of `groups`.
+size_t panel_exchange_records(int groups, int elem_bytes);
 
 template <typename T>
 hipError_t launch_panel(T* A, const PanelArgs& args, int groups,

@@ -151,3 +151,127 @@ int bbk_dgen_synth(bbk_handle_t handle, int n, uint64_t seed, double scale,
 }
 
 }  // extern "C"
+
+// ---- host-pointer forms of the kernels.h helpers -----------------------
+namespace {
+
+// Device mirror of a strided host vector of `count` elements.
+struct Mirror {
+  double* d = nullptr;
+  size_t elems = 0;
+  hipError_t err = hipSuccess;
+  Mirror(const double* h, int count, int stride, hipStream_t st) {
+    elems = count <= 0 ? 0 : static_cast<size_t>(count - 1) * static_cast<size_t>(stride) + 1;
+    if (elems == 0) return;
+    err = hipMalloc(&d, elems * sizeof(double));
+    if (err == hipSuccess)
+      err = hipMemcpyAsync(d, h, elems * sizeof(double), hipMemcpyHostToDevice, st);
+  }
+  hipError_t back(double* h, hipStream_t st) {
+    if (elems == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(h, d, elems * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e;
+  }
+  ~Mirror() {
+    if (d) (void)hipFree(d);
+  }
+};
+
+// extent of a row-major operand whose rows are `stride` apart
+size_t extent_rows(int nrows, int ncols, int stride) {
+  return static_cast<size_t>(nrows - 1) * static_cast<size_t>(stride) + ncols;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bbk_kernel_scale_1x1_host(bbk_handle_t handle, int count, double pivot, const double* src,
+                              int src_stride, double* dst, int dst_stride) {
+  if (!handle) return -1;
+  if (count <= 0) return BBK_OK;
+  if (!src) return -4;
+  if (src_stride < 0) return -5;
+  if (!dst) return -6;
+  if (dst_stride < 0) return -7;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  Mirror s(src, count, src_stride, h->stream), d(dst, count, dst_stride, h->stream);
+  if (s.err != hipSuccess || d.err != hipSuccess) return h->hip_fail(s.err != hipSuccess ? s.err : d.err, "device copy");
+  int rc = bbk_kernel_scale_1x1(handle, count, pivot, s.d, src_stride, d.d, dst_stride);
+  if (rc != BBK_OK) return rc;
+  hipError_t e = d.back(dst, h->stream);
+  return e == hipSuccess ? BBK_OK : h->hip_fail(e, "copy back");
+}
+
+int bbk_kernel_scale_2x2_host(bbk_handle_t handle, int count, double a, double b, double c,
+                              const double* s1, int s1_stride, const double* s2, int s2_stride,
+                              double* d1, int d1_stride, double* d2, int d2_stride) {
+  if (!handle) return -1;
+  if (count <= 0) return BBK_OK;
+  if (!s1) return -6;
+  if (s1_stride < 0) return -7;
+  if (!s2) return -8;
+  if (s2_stride < 0) return -9;
+  if (!d1) return -10;
+  if (d1_stride < 0) return -11;
+  if (!d2) return -12;
+  if (d2_stride < 0) return -13;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  Mirror m1(s1, count, s1_stride, h->stream), m2(s2, count, s2_stride, h->stream);
+  Mirror o1(d1, count, d1_stride, h->stream), o2(d2, count, d2_stride, h->stream);
+  for (hipError_t e : {m1.err, m2.err, o1.err, o2.err})
+    if (e != hipSuccess) return h->hip_fail(e, "device copy");
+  int rc = bbk_kernel_scale_2x2(handle, count, a, b, c, m1.d, s1_stride, m2.d, s2_stride, o1.d,
+                                d1_stride, o2.d, d2_stride);
+  if (rc != BBK_OK) return rc;
+  hipError_t e = o1.back(d1, h->stream);
+  if (e == hipSuccess) e = o2.back(d2, h->stream);
+  return e == hipSuccess ? BBK_OK : h->hip_fail(e, "copy back");
+}
+
+int bbk_kernel_update_trailing_host(bbk_handle_t handle, int rows, int cols, int inner,
+                                    const double* left, int left_stride, const double* work,
+                                    int work_stride, double* target, int target_stride) {
+  if (!handle) return -1;
+  if (rows <= 0 || cols <= 0 || inner <= 0) return BBK_OK;
+  if (!left) return -5;
+  if (left_stride < 0) return -6;
+  if (!work) return -7;
+  if (work_stride < 0) return -8;
+  if (!target) return -9;
+  if (target_stride < 0) return -10;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  const size_t el = extent_rows(inner, rows, left_stride);
+  const size_t ew = extent_rows(inner, cols, work_stride);
+  const size_t et = extent_rows(rows, cols, target_stride);
+  double *dl = nullptr, *dw = nullptr, *dt = nullptr;
+  hipError_t e = hipMalloc(&dl, el * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dw, ew * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dt, et * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dl, left, el * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dw, work, ew * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dt, target, et * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  int rc = BBK_OK;
+  if (e != hipSuccess) {
+    rc = h->hip_fail(e, "device copy");
+  } else {
+    rc = bbk_kernel_update_trailing(handle, rows, cols, inner, dl, left_stride, dw, work_stride,
+                                    dt, target_stride);
+    if (rc == BBK_OK) {
+      e = hipMemcpyAsync(target, dt, et * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) rc = h->hip_fail(e, "copy back");
+    }
+  }
+  (void)hipStreamSynchronize(h->stream);
+  if (dl) (void)hipFree(dl);
+  if (dw) (void)hipFree(dw);
+  if (dt) (void)hipFree(dt);
+  return rc;
+}
+
+}  // extern "C"

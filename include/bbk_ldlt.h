@@ -144,6 +144,24 @@ int bbk_kernel_update_trailing(bbk_handle_t handle, int rows, int cols,
                                int work_stride, double* d_target,
                                int target_stride);
 
+/* Host-pointer forms of the three helpers (synchronous): operands are
+ * copied to the device, the kernel runs, results are copied back.  This
+ * is what the C++ drop-in naive_block_ldlt::kernels::* calls.  Strides
+ * must be >= 0. */
+int bbk_kernel_scale_1x1_host(bbk_handle_t handle, int count, double pivot,
+                              const double* src, int src_stride, double* dst,
+                              int dst_stride);
+int bbk_kernel_scale_2x2_host(bbk_handle_t handle, int count, double a,
+                              double b, double c, const double* s1,
+                              int s1_stride, const double* s2, int s2_stride,
+                              double* d1, int d1_stride, double* d2,
+                              int d2_stride);
+int bbk_kernel_update_trailing_host(bbk_handle_t handle, int rows, int cols,
+                                    int inner, const double* left,
+                                    int left_stride, const double* work,
+                                    int work_stride, double* target,
+                                    int target_stride);
+
 /* ---- synthetic inputs (bench/test plumbing) ---------------------------
  * A(i,j) = A(j,i) = scale * u(seed,i,j), u uniform in [-1,1) from
  * SplitMix64 keyed by (seed, i, j); A(i,i) = 0 when zero_diag_stride > 0
