@@ -20,6 +20,8 @@ struct Handle {
   size_t ws_bytes = 0;
   void* solve_ws = nullptr;  // triangular sweeps: flags, Z, V
   size_t solve_bytes = 0;
+  void* solve_tagged = nullptr;  // solve_ws whose record area was zeroed
+  int solve_tag = 0;             // last record tag used by a sweep launch
   void* berr_ws = nullptr;   // residual partial sums and norms
   size_t berr_bytes = 0;
   void* mixed_ws = nullptr;  // refinement driver: factor copy, r, d
@@ -39,6 +41,7 @@ struct Handle {
   int reserve_solve(size_t b) { return reserve_buf(&solve_ws, &solve_bytes, b, "solve"); }
   int reserve_berr(size_t b) { return reserve_buf(&berr_ws, &berr_bytes, b, "residual"); }
   int reserve_mixed(size_t b) { return reserve_buf(&mixed_ws, &mixed_bytes, b, "refinement"); }
+  int next_solve_tag() { return ++solve_tag; }
   void timing_reset();
   void timing_mark(int group, bool begin);
   double timing_total(int group);

@@ -40,21 +40,6 @@ struct PanelCfg<float> {
   static constexpr int kMaxWidth = 128;  // 129 x 256 x 4 B = 129 KB
 };
 
-// Exchange area layout (doubles) used by the panel kernel's workgroups
-// between grid barriers.
-struct PanelExchange {
-  static constexpr int kOmega = 0;                          // [G] partial omega
-  static constexpr int kSigma = kOmega + kMaxPanelGroups;   // [G] partial sigma
-  static constexpr int kScalars = kSigma + kMaxPanelGroups; // 16 scalars
-  static constexpr int kColK = kScalars + 16;               // [b+2]
-  static constexpr int kColK1 = kColK + kPanelMaxWidth + 2; // [b+2]
-  static constexpr int kColR = kColK1 + kPanelMaxWidth + 2; // [b+2]
-  static constexpr int kDoubles = kColR + kPanelMaxWidth + 2;
-  // scalar slots
-  static constexpr int sWkk = 0, sWk1 = 1, sWkr = 2, sCrr = 3, sCk = 4,
-                       sCk1 = 5, sAkkOld = 6, sAk1Old = 7;
-};
-
 // Per-panel device state (ints).
 struct PanelState {
   static constexpr int kNextStart = 0;  // in: this panel's start; out: next
@@ -64,24 +49,34 @@ struct PanelState {
   static constexpr int kInts = 4;
 };
 
-// Grid-wide barrier for a cooperatively launched grid: monotonic counter,
-// target = G * (barrier index + 1).  Agent-scope release/acquire via
-// __threadfence around one atomic per workgroup.
-__device__ __forceinline__ void grid_barrier(unsigned* counter,
-                                             unsigned target) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __threadfence();
-  }
-  __syncthreads();
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Device-coherent (sc1) 16-byte record store: one vector store carries the
+// tag and the value together, so a reader that sees the tag sees the value.
+__device__ __forceinline__ void xput(XRec* p, int tag, int aux, double v) {
+  const long long b = __double_as_longlong(v);
+  v4i d = {tag, aux, static_cast<int>(b & 0xffffffffll), static_cast<int>(b >> 32)};
+  asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(d) : "memory");
 }
+
+// Coherent 16-byte record read; the wait sits in the same asm block so the
+// destination registers are never observed before the data arrives.
+// Returns false (outputs untouched) unless the record carries `tag`.
+__device__ __forceinline__ bool xget(const XRec* p, int tag, int* aux, double* v) {
+  v4i d;
+  asm volatile("global_load_dwordx4 %0, %1, off sc1\n\ts_waitcnt vmcnt(0)"
+               : "=v"(d)
+               : "v"(p)
+               : "memory");
+  if (d.x != tag) return false;
+  *aux = d.y;
+  *v = __longlong_as_double((static_cast<long long>(d.w) << 32) |
+                            static_cast<unsigned int>(d.z));
+  return true;
+}
+
+// Wait until this wave's outstanding vector memory operations completed.
+__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Index of the first maximum: larger value wins, ties go to the lower
 // index (matches the strict '>' scan of src/block_ldlt_decomposition.cc:173).

@@ -33,31 +33,6 @@ namespace bbk {
 __device__ unsigned long long g_panel_profile[16];
 #endif
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ void xput(XRec* p, int tag, int aux, double v) {
-  const long long b = __double_as_longlong(v);
-  v4i d = {tag, aux, static_cast<int>(b & 0xffffffffll), static_cast<int>(b >> 32)};
-  asm volatile("global_store_dwordx4 %0, %1, off sc1" ::"v"(p), "v"(d) : "memory");
-}
-
-// Coherent 16-byte read; the wait sits in the same asm block so the
-// destination registers are never observed before the data arrives.
-__device__ __forceinline__ bool xget(const XRec* p, int tag, int* aux, double* v) {
-  v4i d;
-  asm volatile("global_load_dwordx4 %0, %1, off sc1\n\ts_waitcnt vmcnt(0)"
-               : "=v"(d)
-               : "v"(p)
-               : "memory");
-  if (d.x != tag) return false;
-  *aux = d.y;
-  *v = __longlong_as_double((static_cast<long long>(d.w) << 32) |
-                            static_cast<unsigned int>(d.z));
-  return true;
-}
-
-__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
 // Accesses to A inside the panel kernel are device-coherent (sc1) so that
 // no cache-wide writeback/invalidate is needed around an exchange: a
 // writer only waits for its stores to complete before posting records.

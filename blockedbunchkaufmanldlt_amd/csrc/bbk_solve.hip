@@ -43,152 +43,162 @@ __device__ __forceinline__ float readlane_t<float>(float v, int lane) {
   return readlane_f(v, lane);
 }
 
-__device__ __forceinline__ void wait_flag(const int* flag) {
-  if (threadIdx.x == 0) {
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-      __builtin_amdgcn_s_sleep(1);
-    __threadfence();
+// Right-hand-side values cross workgroups as XRec records {tag, q, value}
+// (one per row and right-hand side, index row * kSolveMaxRhs + q): a
+// reader polls the record itself, so publishing needs no flag, fence or
+// second round trip.  Tags are unique per launch.
+//
+// Lane l of a wavefront polls row (l >> 2), right-hand side (l & 3) of
+// the 16 rows the wavefront consumes; on return every lane holds its value
+// (0 for q >= NR or rows outside the matrix).
+template <int NR>
+__device__ __forceinline__ double poll_rows(const XRec* rec, int row0, int nrows,
+                                            int tag) {
+  const int lane = threadIdx.x & 63;
+  const int r = row0 + (lane >> 2), q = lane & 3;
+  bool need = q < NR && (lane >> 2) < nrows;
+  double v = 0.0;
+  const XRec* p = rec + (int64_t)r * kSolveMaxRhs + q;
+  while (__ballot(need) != 0) {
+    if (need) {
+      int aux;
+      double x;
+      if (xget(p, tag, &aux, &x)) {
+        v = x;
+        need = false;
+      }
+    }
   }
-  __syncthreads();
+  return v;
 }
 
-__device__ __forceinline__ void post_flag(int* flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
+__device__ __forceinline__ double bcast(double v, int lane) { return readlane_d(v, lane); }
 
-// L value at (row j, col i), i < j, from the upper-stored factor.
+// 16 consecutive rows i0 .. i0+15 of column j (forward: L(j, i) = F(i, j)).
 template <typename T>
-__device__ __forceinline__ T lval(const T* F, int64_t lda, const int* piv,
-                                  int i, int j) {
-  if (j == i + 1 && piv[i] == 2) return T(0);
-  return F[(int64_t)i * lda + j];
+__device__ __forceinline__ void load_col16(T (&f)[16], const T* F, int64_t lda, int i0,
+                                           int j, bool ok) {
+#pragma unroll
+  for (int ii = 0; ii < 16; ++ii) f[ii] = ok ? F[(int64_t)(i0 + ii) * lda + j] : T(0);
+}
+// 16 consecutive columns j0 .. j0+15 of row i (backward: L(j, i) = F(i, j)),
+// columns >= n read as 0.
+template <typename T>
+__device__ __forceinline__ void load_row16(T (&f)[16], const T* F, int64_t lda, int i,
+                                           int j0, int n, bool ok) {
+#pragma unroll
+  for (int c = 0; c < 16; ++c) f[c] = (ok && j0 + c < n) ? F[(int64_t)i * lda + j0 + c] : T(0);
 }
 
-// Forward sweep: Z = L^-1 P^T B for nr (<= kSolveMaxRhs) columns.
-template <typename T>
+// Forward sweep: Z = L^-1 P^T B for NR (<= kSolveMaxRhs) columns.
+// Workgroup takes 64-row blocks J = blockIdx.x, blockIdx.x + gridDim.x, ...;
+// wavefront w accumulates rows 16w .. 16w+15 of every earlier block I < J
+// into the block's 64 columns (one lane each), F tiles prefetched one block
+// ahead; wavefront 0 then solves the diagonal block serially (readlane
+// broadcast) and publishes the block's records.
+template <typename T, int NR>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
-    int n, int nr, const T* __restrict__ F, int64_t lda,
-    const int* __restrict__ piv, const int* __restrict__ perm,
-    const T* __restrict__ B, int64_t ldb, T* __restrict__ Z, int64_t ldz,
-    int* __restrict__ flags, int nblk) {
+    int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
+    const int* __restrict__ perm, const T* __restrict__ B, int64_t ldb,
+    T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk) {
   constexpr int NB = kSolveBlock;
-  __shared__ T zin[NB * kSolveMaxRhs];
-  __shared__ T part[4 * NB * kSolveMaxRhs];
-  const int t = threadIdx.x;
-  const int col = t & (NB - 1), grp = t >> 6;
+  __shared__ T part[4][NB][NR];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
     const int j0 = J * NB;
     const int jn = min(NB, n - j0);
-    const int j = j0 + col;
-    T acc[kSolveMaxRhs];
-    for (int q = 0; q < kSolveMaxRhs; ++q) acc[q] = T(0);
-    for (int I = 0; I < J; ++I) {
-      wait_flag(&flags[I]);
-      const int i0 = I * NB;
-      for (int e = t; e < NB * nr; e += kSolveThreads) {
-        const int ii = e / nr, q = e % nr;
-        zin[ii * kSolveMaxRhs + q] = Z[(int64_t)(i0 + ii) * ldz + q];
-      }
-      __syncthreads();
-      T s[kSolveMaxRhs];
-      for (int q = 0; q < kSolveMaxRhs; ++q) s[q] = T(0);
-      if (col < jn) {
-        for (int ii = grp * 16; ii < grp * 16 + 16; ++ii) {
-          const T l = lval(F, lda, piv, i0 + ii, j);
-          for (int q = 0; q < nr; ++q) s[q] += l * zin[ii * kSolveMaxRhs + q];
-        }
-      }
-      for (int q = 0; q < nr; ++q) part[(grp * NB + col) * kSolveMaxRhs + q] = s[q];
-      __syncthreads();
-      if (grp == 0)
-        for (int q = 0; q < nr; ++q)
-          acc[q] += part[(0 * NB + col) * kSolveMaxRhs + q] +
-                    part[(1 * NB + col) * kSolveMaxRhs + q] +
-                    part[(2 * NB + col) * kSolveMaxRhs + q] +
-                    part[(3 * NB + col) * kSolveMaxRhs + q];
-      __syncthreads();
-    }
-    if (grp == 0) {
-      // diagonal block, one wavefront: lane = column
-      T v[kSolveMaxRhs];
-      for (int q = 0; q < kSolveMaxRhs; ++q)
-        v[q] = (col < jn && q < nr) ? B[(int64_t)perm[j] * ldb + q] - acc[q] : T(0);
-      T Lc[NB];
+    const int j = j0 + lane;
+    const bool jok = lane < jn;
+    // wavefront 0: right-hand side and the diagonal block's L column
+    T bj[NR];
+    T Lc[NB];
+    if (w == 0) {
+      const int pj = jok ? perm[j] : 0;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) bj[q] = jok ? B[(int64_t)pj * ldb + q] : T(0);
 #pragma unroll
       for (int ii = 0; ii < NB; ++ii)
-        Lc[ii] = (col < jn && ii < col) ? lval(F, lda, piv, j0 + ii, j) : T(0);
+        Lc[ii] = (jok && ii < lane) ? F[(int64_t)(j0 + ii) * lda + j] : T(0);
+      // inside a 2x2 pivot (k, k+1), F(k, k+1) holds D, not L
+      if (jok && lane > 0 && piv[j - 1] == 2) {
+#pragma unroll
+        for (int ii = 0; ii < NB; ++ii)
+          if (ii == lane - 1) Lc[ii] = T(0);
+      }
+    }
+    T acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = T(0);
+    T fc[16], fn[16];
+    if (J > 0) load_col16(fc, F, lda, w * 16, j, jok);
+    for (int I = 0; I < J; ++I) {
+      const int i0 = I * NB + w * 16;
+      if (I + 1 < J) load_col16(fn, F, lda, i0 + NB, j, jok);
+      // row i0 + 15 = j - 1 with j = j0: 2x2 coupling across the block edge
+      if (I == J - 1 && w == 3 && lane == 0 && piv[j0 - 1] == 2) fc[15] = T(0);
+      const double zv = poll_rows<NR>(rec, i0, 16, tag);
+#pragma unroll
+      for (int ii = 0; ii < 16; ++ii) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] += fc[ii] * static_cast<T>(bcast(zv, ii * 4 + q));
+      }
+#pragma unroll
+      for (int ii = 0; ii < 16; ++ii) fc[ii] = fn[ii];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) part[w][lane][q] = acc[q];
+    __syncthreads();
+    if (w == 0) {
+      T v[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        v[q] = jok ? bj[q] - (part[0][lane][q] + part[1][lane][q] + part[2][lane][q] +
+                              part[3][lane][q])
+                   : T(0);
 #pragma unroll
       for (int ii = 0; ii < NB; ++ii) {
         if (ii < jn) {
-          for (int q = 0; q < kSolveMaxRhs; ++q) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
             const T zi = readlane_t<T>(v[q], ii);
-            v[q] -= Lc[ii] * zi;  // Lc[ii] == 0 unless ii < col
+            v[q] -= Lc[ii] * zi;  // Lc[ii] == 0 unless ii < lane
           }
         }
       }
-      if (col < jn)
-        for (int q = 0; q < nr; ++q) Z[(int64_t)j * ldz + q] = v[q];
+      if (jok) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          Z[(int64_t)j * ldz + q] = v[q];
+          xput(rec + (int64_t)j * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+        }
+      }
     }
-    post_flag(&flags[J]);
+    __syncthreads();  // part[] is reused by the next block
   }
 }
 
 // Backward sweep with the D solve fused in front:
-// V = L^-T D^-1 Z, X[perm[i]] = V[i].
-template <typename T>
+// V = L^-T D^-1 Z, X[perm[i]] = V[i].  Block I consumes blocks J > I;
+// lane = row, wavefront w covers columns 16w .. 16w+15 of each block J.
+template <typename T, int NR>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
-    int n, int nr, const T* __restrict__ F, int64_t lda,
-    const int* __restrict__ piv, const int* __restrict__ perm,
-    const T* __restrict__ Z, int64_t ldz, T* __restrict__ V, int64_t ldv,
-    T* __restrict__ X, int64_t ldx, int* __restrict__ flags, int nblk) {
+    int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
+    const int* __restrict__ perm, const T* __restrict__ Z, int64_t ldz,
+    T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk) {
   constexpr int NB = kSolveBlock;
-  __shared__ T vin[NB * kSolveMaxRhs];
-  __shared__ T part[4 * NB * kSolveMaxRhs];
-  const int t = threadIdx.x;
-  const int row = t & (NB - 1), grp = t >> 6;
+  __shared__ T part[4][NB][NR];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
     const int i0 = I * NB;
     const int in = min(NB, n - i0);
-    const int i = i0 + row;
-    T acc[kSolveMaxRhs];
-    for (int q = 0; q < kSolveMaxRhs; ++q) acc[q] = T(0);
-    for (int J = nblk - 1; J > I; --J) {
-      wait_flag(&flags[J]);
-      const int jj0 = J * NB;
-      const int jn = min(NB, n - jj0);
-      for (int e = t; e < NB * nr; e += kSolveThreads) {
-        const int jj = e / nr, q = e % nr;
-        vin[jj * kSolveMaxRhs + q] = jj < jn ? V[(int64_t)(jj0 + jj) * ldv + q] : T(0);
-      }
-      __syncthreads();
-      T s[kSolveMaxRhs];
-      for (int q = 0; q < kSolveMaxRhs; ++q) s[q] = T(0);
-      if (row < in) {
-        for (int jj = grp * 16; jj < grp * 16 + 16; ++jj) {
-          if (jj < jn) {
-            const T l = lval(F, lda, piv, i, jj0 + jj);
-            for (int q = 0; q < nr; ++q) s[q] += l * vin[jj * kSolveMaxRhs + q];
-          }
-        }
-      }
-      for (int q = 0; q < nr; ++q) part[(grp * NB + row) * kSolveMaxRhs + q] = s[q];
-      __syncthreads();
-      if (grp == 0)
-        for (int q = 0; q < nr; ++q)
-          acc[q] += part[(0 * NB + row) * kSolveMaxRhs + q] +
-                    part[(1 * NB + row) * kSolveMaxRhs + q] +
-                    part[(2 * NB + row) * kSolveMaxRhs + q] +
-                    part[(3 * NB + row) * kSolveMaxRhs + q];
-      __syncthreads();
-    }
-    if (grp == 0) {
-      T w[kSolveMaxRhs];
-      for (int q = 0; q < kSolveMaxRhs; ++q) w[q] = T(0);
-      if (row < in) {
+    const int i = i0 + lane;
+    const bool iok = lane < in;
+    T wv[NR];
+    T Lr[NB];
+    if (w == 0) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) wv[q] = T(0);
+      if (iok) {
         const int pv = piv[i];
         if (pv == 2 || pv == -2) {
           // 2x2 block (k, k+1): LAPACK dsytrs's scaled formula, binary64
@@ -198,38 +208,75 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
           const double c = F[(int64_t)(k + 1) * lda + k + 1];
           const double akm1 = a / b, ak = c / b;
           const double denom = akm1 * ak - 1.0;
-          for (int q = 0; q < nr; ++q) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
             const double bkm1 = (double)Z[(int64_t)k * ldz + q] / b;
             const double bk = (double)Z[(int64_t)(k + 1) * ldz + q] / b;
-            w[q] = (T)(pv == 2 ? (ak * bkm1 - bk) / denom : (akm1 * bk - bkm1) / denom);
+            wv[q] = (T)(pv == 2 ? (ak * bkm1 - bk) / denom : (akm1 * bk - bkm1) / denom);
           }
         } else {
           const double d = F[(int64_t)i * lda + i];
-          for (int q = 0; q < nr; ++q) w[q] = (T)((double)Z[(int64_t)i * ldz + q] / d);
+#pragma unroll
+          for (int q = 0; q < NR; ++q) wv[q] = (T)((double)Z[(int64_t)i * ldz + q] / d);
         }
       }
-      T v[kSolveMaxRhs];
-      for (int q = 0; q < kSolveMaxRhs; ++q) v[q] = w[q] - acc[q];
-      T Lr[NB];
 #pragma unroll
       for (int jj = 0; jj < NB; ++jj)
-        Lr[jj] = (row < in && jj > row && jj < in) ? lval(F, lda, piv, i, i0 + jj) : T(0);
+        Lr[jj] = (iok && jj > lane && jj < in) ? F[(int64_t)i * lda + i0 + jj] : T(0);
+      if (iok && lane + 1 < in && piv[i] == 2) {
+#pragma unroll
+        for (int jj = 0; jj < NB; ++jj)
+          if (jj == lane + 1) Lr[jj] = T(0);
+      }
+    }
+    T acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = T(0);
+    T fc[16], fn[16];
+    if (I + 1 < nblk) load_row16(fc, F, lda, i, (nblk - 1) * NB + w * 16, n, iok);
+    for (int J = nblk - 1; J > I; --J) {
+      const int c0 = J * NB + w * 16;
+      if (J - 1 > I) load_row16(fn, F, lda, i, c0 - NB, n, iok);
+      // column c0 = i + 1 with i = i0 + 63: 2x2 coupling across the block edge
+      if (J == I + 1 && w == 0 && lane == 63 && piv[i] == 2) fc[0] = T(0);
+      const double vv = poll_rows<NR>(rec, c0, min(16, n - c0), tag);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] += fc[c] * static_cast<T>(bcast(vv, c * 4 + q));
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) fc[c] = fn[c];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) part[w][lane][q] = acc[q];
+    __syncthreads();
+    if (w == 0) {
+      T v[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        v[q] = wv[q] - (part[0][lane][q] + part[1][lane][q] + part[2][lane][q] +
+                        part[3][lane][q]);
 #pragma unroll
       for (int jj = NB - 1; jj >= 0; --jj) {
         if (jj < in) {
-          for (int q = 0; q < kSolveMaxRhs; ++q) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
             const T vj = readlane_t<T>(v[q], jj);
-            v[q] -= Lr[jj] * vj;  // Lr[jj] == 0 unless jj > row
+            v[q] -= Lr[jj] * vj;  // Lr[jj] == 0 unless jj > lane
           }
         }
       }
-      if (row < in)
-        for (int q = 0; q < nr; ++q) {
-          V[(int64_t)i * ldv + q] = v[q];
-          X[(int64_t)perm[i] * ldx + q] = v[q];
+      if (iok) {
+        const int pi = perm[i];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          X[(int64_t)pi * ldx + q] = v[q];
+          xput(rec + (int64_t)i * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
         }
+      }
     }
-    post_flag(&flags[I]);
+    __syncthreads();
   }
 }
 
@@ -286,7 +333,7 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
     for (int q = 0; q < nr; ++q) {
       double v = 0.0;
       for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + q];
-      prow[((int64_t)J * n + i0 + t) * kSolveMaxRhs + q] = v;
+      prow[((int64_t)J * n + i0 + t) * nr + q] = v;
     }
     if (with_norm) {
       double v = 0.0;
@@ -319,7 +366,7 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
     for (int q = 0; q < nr; ++q) {
       double v = 0.0;
       for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + q];
-      pcol[((int64_t)I * n + j0 + t) * kSolveMaxRhs + q] = v;
+      pcol[((int64_t)I * n + j0 + t) * nr + q] = v;
     }
     if (with_norm) {
       double v = 0.0;
@@ -353,8 +400,8 @@ __global__ __launch_bounds__(256) void k_symv_reduce(
   const int ti = i / kSymvTile;
   for (int q = 0; q < nr; ++q) {
     double y = 0.0;
-    for (int J = ti; J < T; ++J) y += prow[((int64_t)J * n + i) * kSolveMaxRhs + q];
-    for (int I = 0; I <= ti; ++I) y += pcol[((int64_t)I * n + i) * kSolveMaxRhs + q];
+    for (int J = ti; J < T; ++J) y += prow[((int64_t)J * n + i) * nr + q];
+    for (int I = 0; I <= ti; ++I) y += pcol[((int64_t)I * n + i) * nr + q];
     const double b = Bv[(int64_t)i * ldb + q];
     const double r = b - y;
     if (R) R[(int64_t)i * ldr + q] = r;
@@ -431,7 +478,48 @@ static int coop_groups(K kernel, int want) {
   return std::max(1, std::min(want, per_cu * cus));
 }
 
-// Scratch for one solve: flags (2 * nblk ints), Z, V (n x kSolveMaxRhs of T).
+// Scratch for one solve: records for both sweeps (n x kSolveMaxRhs each)
+// and Z (n x kSolveMaxRhs of T).  The record area is zeroed when it is
+// (re)allocated; afterwards every launch uses a fresh tag, so records of
+// earlier launches are never accepted and nothing is cleared per call.
+template <typename T, int NR>
+static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, const int* perm,
+                        const int* piv, const T* B, int64_t ldb, T* X, int64_t ldx, T* Z,
+                        XRec* rf, XRec* rb) {
+  const int gf = coop_groups(k_trsv_fwd<T, NR>, nblk);
+  const int gb = coop_groups(k_trsv_bwd<T, NR>, nblk);
+  int64_t ldz = kSolveMaxRhs;
+  {
+    int tag = h->next_solve_tag();
+    int n_ = n, nblk_ = nblk;
+    int64_t lda_ = lda, ldb_ = ldb;
+    const T* F_ = F;
+    const int *piv_ = piv, *perm_ = perm;
+    const T* B_ = B;
+    T* Z_ = Z;
+    XRec* r_ = rf;
+    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &B_, &ldb_, &Z_, &ldz, &r_, &tag, &nblk_};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR>),
+                                              dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
+  }
+  {
+    int tag = h->next_solve_tag();
+    int n_ = n, nblk_ = nblk;
+    int64_t lda_ = lda, ldx_ = ldx;
+    const T* F_ = F;
+    const int *piv_ = piv, *perm_ = perm;
+    const T* Z_ = Z;
+    T* X_ = X;
+    XRec* r_ = rb;
+    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &Z_, &ldz, &X_, &ldx_, &r_, &tag, &nblk_};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR>),
+                                              dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
+  }
+  return BBK_OK;
+}
+
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
           const int* piv, T* B, int64_t ldb) {
@@ -445,52 +533,32 @@ int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
   if (ldb < std::max(1, nrhs)) return -9;
   if (nrhs == 0) return BBK_OK;
   const int nblk = (n + kSolveBlock - 1) / kSolveBlock;
-  const size_t flag_bytes = (2 * nblk * sizeof(int) + 255) / 256 * 256;
+  const size_t rec_bytes = (size_t)n * kSolveMaxRhs * sizeof(XRec);
   const size_t vec_bytes = ((size_t)n * kSolveMaxRhs * sizeof(T) + 255) / 256 * 256;
-  // solve scratch lives after the factorization scratch region
-  int rc = h->reserve_solve(flag_bytes + 2 * vec_bytes);
+  int rc = h->reserve_solve(2 * rec_bytes + vec_bytes);
   if (rc != BBK_OK) return rc;
   char* base = reinterpret_cast<char*>(h->solve_ws);
-  int* flags = reinterpret_cast<int*>(base);
-  T* Z = reinterpret_cast<T*>(base + flag_bytes);
-  T* V = reinterpret_cast<T*>(base + flag_bytes + vec_bytes);
-  const int gf = coop_groups(k_trsv_fwd<T>, nblk);
-  const int gb = coop_groups(k_trsv_bwd<T>, nblk);
+  XRec* rf = reinterpret_cast<XRec*>(base);
+  XRec* rb = reinterpret_cast<XRec*>(base + rec_bytes);
+  T* Z = reinterpret_cast<T*>(base + 2 * rec_bytes);
+  if (h->solve_tagged != h->solve_ws || h->solve_tag > (1 << 30)) {
+    const int words = static_cast<int>(2 * rec_bytes / sizeof(int));
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(words, 256)), dim3(256), 0, h->stream,
+                       reinterpret_cast<int*>(base), words, 0);
+    h->solve_tagged = h->solve_ws;
+    h->solve_tag = 0;
+  }
   for (int q0 = 0; q0 < nrhs; q0 += kSolveMaxRhs) {
-    int nr = std::min(kSolveMaxRhs, nrhs - q0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(2 * nblk, 256)), dim3(256), 0, h->stream,
-                       flags, 2 * nblk, 0);
+    const int nr = std::min(kSolveMaxRhs, nrhs - q0);
     const T* Bq = B + q0;
     T* Xq = B + q0;
-    int64_t ldz = kSolveMaxRhs;
-    {
-      int n_ = n, nr_ = nr, nblk_ = nblk;
-      int64_t lda_ = lda, ldb_ = ldb;
-      const T* F_ = F;
-      const int *piv_ = piv, *perm_ = perm;
-      const T* B_ = Bq;
-      T* Z_ = Z;
-      int* fl = flags;
-      void* args[] = {&n_, &nr_, &F_, &lda_, &piv_, &perm_, &B_, &ldb_, &Z_, &ldz, &fl, &nblk_};
-      hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T>),
-                                                dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
-      if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
+    switch (nr) {
+      case 1: rc = solve_launch<T, 1>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
+      case 2: rc = solve_launch<T, 2>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
+      case 3: rc = solve_launch<T, 3>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
+      default: rc = solve_launch<T, 4>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
     }
-    {
-      int n_ = n, nr_ = nr, nblk_ = nblk;
-      int64_t lda_ = lda, ldv = kSolveMaxRhs, ldx = ldb;
-      const T* F_ = F;
-      const int *piv_ = piv, *perm_ = perm;
-      const T* Z_ = Z;
-      T* V_ = V;
-      T* X_ = Xq;
-      int* fl = flags + nblk;
-      void* args[] = {&n_, &nr_, &F_, &lda_, &piv_, &perm_, &Z_, &ldz, &V_, &ldv, &X_, &ldx,
-                      &fl, &nblk_};
-      hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T>),
-                                                dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
-      if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
-    }
+    if (rc != BBK_OK) return rc;
   }
   return BBK_OK;
 }
