@@ -14,6 +14,7 @@ otherwise, plus ``perm`` and ``piv`` int32 vectors.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 from typing import Optional, Sequence, Tuple
@@ -50,6 +51,7 @@ def lib() -> ctypes.CDLL:
     ip = ctypes.POINTER(ctypes.c_int)
     L.bbk_create.argtypes = [ctypes.POINTER(vp), vp]
     L.bbk_destroy.argtypes = [vp]
+    L.bbk_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
     L.bbk_synchronize.argtypes = [vp]
     L.bbk_last_error.argtypes = [vp]
     L.bbk_last_error.restype = ctypes.c_char_p
@@ -65,6 +67,16 @@ def lib() -> ctypes.CDLL:
                             ctypes.POINTER(f64)]
     for name in ("bbk_dsytrf_batched", "bbk_ssytrf_batched"):
         getattr(L, name).argtypes = [vp, i32, ip, vp, vp, vp, vp]
+    L.bbk_batched_prepare.argtypes = [vp, i32, ip, i32]
+    L.bbk_batched_get_widths.argtypes = [vp, i32, ip]
+    L.bbk_batched_set_widths.argtypes = [vp, i32, ip]
+    L.bbk_batched_tune.argtypes = [vp, i32, i32, ip]
+    for name in ("bbk_graph_create_dsytrf_batched", "bbk_graph_create_ssytrf_batched"):
+        getattr(L, name).argtypes = [vp, i32, ip, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    L.bbk_graph_launch.argtypes = [vp]
+    L.bbk_graph_destroy.argtypes = [vp]
+    for name in ("bbk_dgen_synth_batched", "bbk_sgen_synth_batched"):
+        getattr(L, name).argtypes = [vp, i32, vp, ctypes.c_uint64, f64, i32, vp, vp]
     L.bbk_kernel_scale_1x1.argtypes = [vp, i32, f64, vp, i32, vp, i32]
     L.bbk_kernel_scale_2x2.argtypes = [vp, i32, f64, f64, f64, vp, i32, vp, i32,
                                        vp, i32, vp, i32]
@@ -83,7 +95,14 @@ def _ptr(t) -> Optional[int]:
 
 
 class Handle:
-    """A bbk handle bound to torch's current stream on the current device."""
+    """A bbk handle ordered with torch's current stream on the current device.
+
+    With a non-default torch stream the handle queues its work on that
+    stream.  On torch's default stream the handle owns a non-blocking
+    stream (cooperative launches are refused on streams tied to the null
+    stream) and every call is fenced both ways with stream waits, so torch
+    work before a call is visible to it and torch work after it sees its
+    results."""
 
     def __init__(self, stream=None):
         import torch
@@ -91,12 +110,30 @@ class Handle:
             raise RuntimeError("blockedbunchkaufmanldlt_amd needs a HIP GPU")
         self._torch = torch
         st = stream if stream is not None else torch.cuda.current_stream()
-        self.stream = st
         h = ctypes.c_void_p()
-        rc = lib().bbk_create(ctypes.byref(h), ctypes.c_void_p(st.cuda_stream))
+        rc = lib().bbk_create(ctypes.byref(h), ctypes.c_void_p(st.cuda_stream or None))
         if rc != 0:
             raise BbkError(rc, "bbk_create failed")
         self.h = h
+        self._own = None
+        if not st.cuda_stream:
+            sp = ctypes.c_void_p()
+            lib().bbk_get_stream(h, ctypes.byref(sp))
+            self._own = torch.cuda.ExternalStream(sp.value)
+        self.stream = self._own if self._own is not None else st
+
+    @contextlib.contextmanager
+    def ordered(self):
+        """Fence a block of library calls against torch's current stream."""
+        if self._own is None:
+            yield
+            return
+        cur = self._torch.cuda.current_stream()
+        self._own.wait_stream(cur)
+        try:
+            yield
+        finally:
+            cur.wait_stream(self._own)
 
     def close(self):
         if getattr(self, "h", None):
@@ -140,9 +177,11 @@ class Handle:
         if piv is None:
             piv = torch.empty(n, dtype=torch.int32, device=A.device)
         if A.dtype == torch.float64:
-            rc = lib().bbk_dsytrf(self.h, n, nb, _ptr(A), lda, _ptr(perm), _ptr(piv))
+            with self.ordered():
+                rc = lib().bbk_dsytrf(self.h, n, nb, _ptr(A), lda, _ptr(perm), _ptr(piv))
         elif A.dtype == torch.float32:
-            rc = lib().bbk_ssytrf(self.h, n, nb, _ptr(A), lda, _ptr(perm), _ptr(piv))
+            with self.ordered():
+                rc = lib().bbk_ssytrf(self.h, n, nb, _ptr(A), lda, _ptr(perm), _ptr(piv))
         else:
             raise TypeError("A must be float64 or float32")
         self.check(rc, "sytrf")
@@ -155,8 +194,9 @@ class Handle:
         nrhs = B.shape[1] if B.dim() == 2 else 1
         ldb = B.stride(0) if B.dim() == 2 else 1
         fn = lib().bbk_dsytrs if F.dtype == torch.float64 else lib().bbk_ssytrs
-        self.check(fn(self.h, n, nrhs, _ptr(F), lda, _ptr(perm), _ptr(piv), _ptr(B), ldb),
-                   "sytrs")
+        with self.ordered():
+            self.check(fn(self.h, n, nrhs, _ptr(F), lda, _ptr(perm), _ptr(piv), _ptr(B), ldb),
+                       "sytrs")
         return B
 
     def syberr(self, A, B, X):
@@ -166,8 +206,9 @@ class Handle:
         ldb = B.stride(0) if B.dim() == 2 else 1
         ldx = X.stride(0) if X.dim() == 2 else 1
         eta = torch.empty(nrhs, dtype=torch.float64, device=A.device)
-        self.check(lib().bbk_dsyberr(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb,
-                                     _ptr(X), ldx, _ptr(eta)), "syberr")
+        with self.ordered():
+            self.check(lib().bbk_dsyberr(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb,
+                                         _ptr(X), ldx, _ptr(eta)), "syberr")
         return eta
 
     def sysv_mixed(self, A, B, X, tol: float = 1e-12, max_iter: int = 30):
@@ -177,8 +218,9 @@ class Handle:
         ldx = X.stride(0) if X.dim() == 2 else 1
         it = ctypes.c_int()
         be = ctypes.c_double()
-        rc = lib().bbk_dsysv_mixed(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb, _ptr(X),
-                                   ldx, tol, max_iter, ctypes.byref(it), ctypes.byref(be))
+        with self.ordered():
+            rc = lib().bbk_dsysv_mixed(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb, _ptr(X),
+                                       ldx, tol, max_iter, ctypes.byref(it), ctypes.byref(be))
         if rc not in (0, BBK_ERR_NOT_CONVERGED):
             self.check(rc, "sysv_mixed")
         return {"iters": it.value, "berr": be.value, "converged": rc == 0}
@@ -189,50 +231,141 @@ class Handle:
         ldb = B.stride(0) if B.dim() == 2 else 1
         ldx = X.stride(0) if X.dim() == 2 else 1
         be = ctypes.c_double()
-        self.check(lib().bbk_dsysv(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb, _ptr(X),
-                                   ldx, ctypes.byref(be)), "sysv")
+        with self.ordered():
+            self.check(lib().bbk_dsysv(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb, _ptr(X),
+                                       ldx, ctypes.byref(be)), "sysv")
         return be.value
 
-    def sytrf_batched(self, mats: Sequence, perms: Sequence, pivs: Sequence):
-        """Factor a list of (n_i, lda_i) tensors (all float64 or all float32)."""
+    # ---- batched factorization ------------------------------------------
+    def _batch_arrays(self, mats, perms, pivs):
         torch = self._torch
-        count = len(mats)
         dev = mats[0].device
+        count = len(mats)
         ns = (ctypes.c_int * count)(*[m.shape[0] for m in mats])
         aptr = torch.tensor([m.data_ptr() for m in mats], dtype=torch.int64, device=dev)
         lds = torch.tensor([m.stride(0) for m in mats], dtype=torch.int64, device=dev)
         pp = torch.tensor([p.data_ptr() for p in perms], dtype=torch.int64, device=dev)
         vp = torch.tensor([p.data_ptr() for p in pivs], dtype=torch.int64, device=dev)
+        return count, ns, (aptr, lds, pp, vp)
+
+    def sytrf_batched(self, mats: Sequence, perms: Sequence, pivs: Sequence):
+        """Factor a list of (n_i, lda_i) tensors (all float64 or all float32)
+        in place in one persistent launch; perms/pivs: int32 tensors of n_i."""
+        torch = self._torch
+        count, ns, arrs = self._batch_arrays(mats, perms, pivs)
         fn = lib().bbk_dsytrf_batched if mats[0].dtype == torch.float64 else \
             lib().bbk_ssytrf_batched
-        self.check(fn(self.h, count, ns, _ptr(aptr), _ptr(lds), _ptr(pp), _ptr(vp)),
-                   "sytrf_batched")
+        with self.ordered():
+            self.check(fn(self.h, count, ns, *[_ptr(a) for a in arrs]), "sytrf_batched")
         # keep the pointer arrays alive until the queued work has run
-        self._keep = (aptr, lds, pp, vp)
+        self._keep = arrs
+
+    def graph_sytrf_batched(self, mats: Sequence, perms: Sequence, pivs: Sequence):
+        """Capture the batched factorization of these buffers into a HIP graph;
+        returns a BatchGraph whose launch() replays it on this handle's stream."""
+        torch = self._torch
+        count, ns, arrs = self._batch_arrays(mats, perms, pivs)
+        torch.cuda.synchronize()
+        g = ctypes.c_void_p()
+        fn = lib().bbk_graph_create_dsytrf_batched if mats[0].dtype == torch.float64 else \
+            lib().bbk_graph_create_ssytrf_batched
+        self.check(fn(self.h, count, ns, *[_ptr(a) for a in arrs], ctypes.byref(g)),
+                   "graph_sytrf_batched")
+        return BatchGraph(g, arrs, self)
+
+    def batched_prepare(self, ns: Sequence[int], dtype):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * len(ns))(*ns)
+        self.check(lib().bbk_batched_prepare(self.h, len(ns), arr, eb), "batched_prepare")
+
+    def batched_widths(self, dtype):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)()
+        self.check(lib().bbk_batched_get_widths(self.h, eb, arr), "batched_get_widths")
+        return list(arr)
+
+    def set_batched_widths(self, dtype, widths: Sequence[int]):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)(*widths)
+        self.check(lib().bbk_batched_set_widths(self.h, eb, arr), "batched_set_widths")
+
+    def batched_tune(self, dtype, max_n: int):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)()
+        self.check(lib().bbk_batched_tune(self.h, eb, max_n, arr), "batched_tune")
+        return list(arr)
+
+    def gen_synth_batched(self, ns: Sequence[int], seed: int, scale: float = 1.0,
+                          zero_diag_stride: int = 0, dtype=None, pad: int = 0):
+        """Matrix m = gen_synth(ns[m], seed + m, ...) (rounded to float32 for
+        dtype float32); returns a list of (n, n + pad) tensors."""
+        torch = self._torch
+        dtype = dtype or torch.float64
+        mats = [torch.empty((n, n + pad), dtype=dtype, device="cuda") for n in ns]
+        dn = torch.tensor(list(ns), dtype=torch.int32, device="cuda")
+        aptr = torch.tensor([m.data_ptr() for m in mats], dtype=torch.int64, device="cuda")
+        lds = torch.tensor([m.stride(0) for m in mats], dtype=torch.int64, device="cuda")
+        fn = lib().bbk_dgen_synth_batched if dtype == torch.float64 else \
+            lib().bbk_sgen_synth_batched
+        with self.ordered():
+            self.check(fn(self.h, len(ns), _ptr(dn), seed, scale, zero_diag_stride, _ptr(aptr),
+                          _ptr(lds)), "gen_synth_batched")
+        self.synchronize()
+        return mats
 
     def gen_synth(self, n: int, seed: int, scale: float = 1.0, zero_diag_stride: int = 0,
                   lda: Optional[int] = None, dtype=None):
         torch = self._torch
         lda = lda or n
         A = torch.empty((n, lda), dtype=torch.float64, device="cuda")
-        self.check(lib().bbk_dgen_synth(self.h, n, seed, scale, zero_diag_stride,
-                                        _ptr(A), lda), "gen_synth")
+        with self.ordered():
+            self.check(lib().bbk_dgen_synth(self.h, n, seed, scale, zero_diag_stride,
+                                            _ptr(A), lda), "gen_synth")
         return A
 
     # ---- kernels.h counterparts -----------------------------------------
     def kernel_scale_1x1(self, count, pivot, src, src_stride, dst, dst_stride):
-        self.check(lib().bbk_kernel_scale_1x1(self.h, count, pivot, _ptr(src), src_stride,
-                                              _ptr(dst), dst_stride), "scale_1x1")
+        with self.ordered():
+            self.check(lib().bbk_kernel_scale_1x1(self.h, count, pivot, _ptr(src), src_stride,
+                                                  _ptr(dst), dst_stride), "scale_1x1")
 
     def kernel_scale_2x2(self, count, a, b, c, s1, s1s, s2, s2s, d1, d1s, d2, d2s):
-        self.check(lib().bbk_kernel_scale_2x2(self.h, count, a, b, c, _ptr(s1), s1s,
-                                              _ptr(s2), s2s, _ptr(d1), d1s, _ptr(d2), d2s),
-                   "scale_2x2")
+        with self.ordered():
+            self.check(lib().bbk_kernel_scale_2x2(self.h, count, a, b, c, _ptr(s1), s1s,
+                                                  _ptr(s2), s2s, _ptr(d1), d1s, _ptr(d2), d2s),
+                       "scale_2x2")
 
     def kernel_update_trailing(self, rows, cols, inner, left, ls, work, ws, target, ts):
-        self.check(lib().bbk_kernel_update_trailing(self.h, rows, cols, inner, _ptr(left), ls,
-                                                    _ptr(work), ws, _ptr(target), ts),
-                   "update_trailing")
+        with self.ordered():
+            self.check(lib().bbk_kernel_update_trailing(self.h, rows, cols, inner, _ptr(left), ls,
+                                                        _ptr(work), ws, _ptr(target), ts),
+                       "update_trailing")
+
+
+class BatchGraph:
+    """An instantiated HIP graph of one batched factorization."""
+
+    def __init__(self, g, keep, handle):
+        self.g = g
+        self._keep = keep
+        self._handle = handle
+
+    def launch(self):
+        with self._handle.ordered():
+            rc = lib().bbk_graph_launch(self.g)
+        if rc != 0:
+            raise BbkError(rc, "bbk_graph_launch failed")
+
+    def close(self):
+        if getattr(self, "g", None):
+            lib().bbk_graph_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default: Optional[Handle] = None

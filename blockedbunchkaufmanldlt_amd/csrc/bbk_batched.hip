@@ -1,0 +1,739 @@
+// bbk_batched.hip -- many independent Bunch-Kaufman LDL^T factorizations
+// in one persistent launch.
+//
+// Each workgroup (256 threads) pulls matrix numbers from a device work
+// queue (one atomic counter; the host orders the queue largest first) and
+// factors the whole matrix by itself with the reference's blocked loop
+// (src/block_ldlt_decomposition.cc:50-59,80-120): per panel, each pivot
+// row is formed from the panel's unscaled rows W = D L^T, the
+// Bunch-Kaufman tests pick a 1x1, a swapped 1x1 or a 2x2 pivot
+// (:369-390), the symmetric interchanges touch the factored rows and the
+// panel's W rows (:414-587), the pivot rows are scaled (:633-704), and a
+// 2x2 that would end on the panel's last column is deferred to the next
+// panel, which then starts one column earlier (:589-631).  After the
+// panel, the trailing upper triangle is updated, C -= U^T W (:706-744).
+// All synchronization is inside the workgroup (__syncthreads), so matrices
+// proceed independently.
+//
+// Panel width comes per matrix from a size-class table (tuned by
+// bbk_batched_tune).  The panel rows W (width + 1 rows) and the candidate
+// row live in LDS when they fit the dynamic LDS budget, else in a global
+// slice owned by the workgroup; both are reached through one generic
+// pointer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "../../include/bbk_ldlt.h"
+#include "bbk_batched.hpp"
+#include "bbk_handle.hpp"
+
+namespace bbk {
+
+constexpr int kBT = kBatchThreads;
+
+template <typename T>
+struct BatchArgs {
+  int count;
+  const int* order;     // queue position -> matrix number
+  const int* ns;        // matrix number -> order n
+  T* const* A;
+  const int64_t* lda;
+  int* const* perm;
+  int* const* piv;
+  int* queue;           // work counter, zero at launch
+  T* ws;                // global W slices (gridDim.x of ws_stride elements)
+  int64_t ws_stride;
+  int lds_elems;        // dynamic LDS capacity in elements of T
+  BatchWidths widths;
+};
+
+extern __shared__ __attribute__((aligned(16))) unsigned char bbk_batched_lds[];
+
+__host__ __device__ __forceinline__ int batch_class(int n) {
+  int c = 0;
+  while (c < kBatchClasses - 1 && n > batch_class_bound(c)) ++c;
+  return c;
+}
+
+template <typename T>
+__device__ __forceinline__ void swap_elems(T* a, T* b) {
+  const T t = *a;
+  *a = *b;
+  *b = t;
+}
+
+// One matrix, whole workgroup.  F: n x lda row-major upper triangle;
+// W: (bw + 2) rows of n elements (rows 0..bw = panel rows, last = the
+// candidate row C).
+template <typename T>
+__device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
+                           int* __restrict__ piv, int bw, T* W, T* Lk, T* Lr, double* rv,
+                           int* ri) {
+  const int tid = threadIdx.x;
+  T* C = W + (int64_t)(bw + 1) * n;
+  for (int i = tid; i < n; i += kBT) {
+    perm[i] = i;
+    piv[i] = 1;
+  }
+  __syncthreads();
+  int s = 0;
+  for (int P = 0; s < n; ++P) {
+    int e = min((P + 1) * bw, n);  // nominal end; s may sit one column early
+    int kk = 0;
+    while (s + kk < e) {
+      const int k = s + kk;
+      if (tid < kk) Lk[tid] = F[(int64_t)(s + tid) * lda + k];
+      __syncthreads();
+      // W(k, j) = F(k, j) - sum_p L(k, p) W(p, j), j >= k  (:122-162)
+      double best = 0.0;
+      int arg = n;
+      T* Wk = W + (int64_t)kk * n;
+      for (int j = k + tid; j < n; j += kBT) {
+        T acc = T(0);
+        for (int p = 0; p < kk; ++p) acc += Lk[p] * W[(int64_t)p * n + j];
+        const T w = F[(int64_t)k * lda + j] - acc;
+        Wk[j] = w;
+        const double a = fabs(static_cast<double>(w));
+        if (j > k && a > best) {
+          best = a;
+          arg = j;
+        }
+      }
+      block_argmax(best, arg, rv, ri);  // (:164-182) first maximum
+      const double omega = best;
+      const int r = arg;
+      const double akk = fabs(static_cast<double>(Wk[k]));
+      int kind = 1;
+      if (!(akk >= kAlpha * omega || omega == 0.0)) {
+        // candidate row r: C(i) = A(r, i) - sum_p L(r, p) W(p, i), i >= k
+        // (:184-352), sigma over i != r (:354-367)
+        if (tid < kk) Lr[tid] = F[(int64_t)(s + tid) * lda + r];
+        __syncthreads();
+        double sig = 0.0;
+        for (int i = k + tid; i < n; i += kBT) {
+          const T base = i < r ? F[(int64_t)i * lda + r] : F[(int64_t)r * lda + i];
+          T acc = T(0);
+          for (int p = 0; p < kk; ++p) acc += Lr[p] * W[(int64_t)p * n + i];
+          const T cv = base - acc;
+          C[i] = cv;
+          const double a = fabs(static_cast<double>(cv));
+          if (i != r) sig = (sig < a) ? a : sig;
+        }
+        const double sigma = block_max_nan_ignoring(sig, rv);
+        const double arr = fabs(static_cast<double>(C[r]));
+        kind = select_pivot_nc(akk, omega, sigma, arr);  // 3, -1 or 2
+        if (kind == 2 && kk == e - s - 1 && e < n) {
+          // 2x2 across the panel edge: this panel ends before k
+          e = k;
+          __syncthreads();
+          break;
+        }
+        // the candidate's value at k replaces W(k, r) (column part of the
+        // candidate row written over row k's entry)
+        if (tid == 0) Wk[r] = C[k];
+        __syncthreads();
+        if (kind == -1) {
+          // interchange k <-> r (:414-498); row k of W becomes the candidate
+          for (int j = r + 1 + tid; j < n; j += kBT)
+            swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)r * lda + j]);
+          for (int j = k + 1 + tid; j < r; j += kBT)
+            swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)j * lda + r]);
+          for (int i = tid; i < k; i += kBT)
+            swap_elems(&F[(int64_t)i * lda + k], &F[(int64_t)i * lda + r]);
+          for (int p = tid; p < kk; p += kBT)
+            swap_elems(&W[(int64_t)p * n + k], &W[(int64_t)p * n + r]);
+          for (int j = k + 1 + tid; j < n; j += kBT)
+            if (j != r) Wk[j] = C[j];
+          if (tid == 0) {
+            swap_elems(&F[(int64_t)k * lda + k], &F[(int64_t)r * lda + r]);
+            Wk[k] = C[r];
+            const int t = perm[k];
+            perm[k] = perm[r];
+            perm[r] = t;
+          }
+        } else if (kind == 2) {
+          // interchange k+1 <-> r (:500-587); row k+1 of W becomes the candidate
+          const int k1 = k + 1;
+          T* Wk1 = W + (int64_t)(kk + 1) * n;
+          if (r != k1) {
+            for (int j = r + 1 + tid; j < n; j += kBT)
+              swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)r * lda + j]);
+            for (int j = k + 2 + tid; j < r; j += kBT)
+              swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)j * lda + r]);
+            for (int i = tid; i < k; i += kBT)
+              swap_elems(&F[(int64_t)i * lda + k1], &F[(int64_t)i * lda + r]);
+            for (int p = tid; p < kk; p += kBT)
+              swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
+          }
+          for (int j = k + 2 + tid; j < n; j += kBT)
+            if (j != r) Wk1[j] = C[j];
+          if (tid == 0) {
+            if (r != k1) {
+              swap_elems(&F[(int64_t)k1 * lda + k1], &F[(int64_t)r * lda + r]);
+              swap_elems(&F[(int64_t)k * lda + k1], &F[(int64_t)k * lda + r]);
+              swap_elems(&Wk[k1], &Wk[r]);
+              Wk1[r] = C[k1];
+              const int t = perm[k1];
+              perm[k1] = perm[r];
+              perm[r] = t;
+            }
+            Wk1[k1] = C[r];
+          }
+        }
+        __syncthreads();
+      }
+      if (kind != 2) {
+        // ScaleOneByOnePivot (:633-656)
+        const double d = static_cast<double>(Wk[k]);
+        const double den = safe_den(d);
+        for (int j = k + 1 + tid; j < n; j += kBT)
+          F[(int64_t)k * lda + j] = static_cast<T>(div_or_zero(static_cast<double>(Wk[j]), den));
+        if (tid == 0) {
+          F[(int64_t)k * lda + k] = Wk[k];
+          piv[k] = 1;
+        }
+        kk += 1;
+      } else {
+        // ScaleTwoByTwoPivot (:658-704), binary64 without contraction
+        const int k1 = k + 1;
+        const T* Wk1 = W + (int64_t)(kk + 1) * n;
+        const double a = static_cast<double>(Wk[k]);
+        const double b = static_cast<double>(Wk[k1]);
+        const double c = static_cast<double>(Wk1[k1]);
+        double itl, ibr, ioff;
+        inverse_2x2_nc(a, b, c, &itl, &ibr, &ioff);
+        for (int j = k + 2 + tid; j < n; j += kBT) {
+          double u0, u1;
+          apply_2x2_nc(itl, ibr, ioff, static_cast<double>(Wk[j]), static_cast<double>(Wk1[j]),
+                       &u0, &u1);
+          F[(int64_t)k * lda + j] = static_cast<T>(u0);
+          F[(int64_t)k1 * lda + j] = static_cast<T>(u1);
+        }
+        if (tid == 0) {
+          F[(int64_t)k * lda + k] = Wk[k];
+          F[(int64_t)k * lda + k1] = Wk[k1];
+          F[(int64_t)k1 * lda + k1] = Wk1[k1];
+          piv[k] = 2;
+          piv[k1] = -2;
+        }
+        kk += 2;
+      }
+      __syncthreads();
+    }
+    // trailing update F(i, j) -= sum_p F(s+p, i) W(p, j), e <= i <= j < n
+    // (:706-744, src/kernels.cc:67-91), 4 x 4 register tiles
+    const int m = n - e, kp = e - s;
+    if (m > 0 && kp > 0) {
+      const int nt = (m + 3) >> 2;
+      const int tiles = nt * (nt + 1) / 2;
+      for (int t = tid; t < tiles; t += kBT) {
+        int tj = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while (tj * (tj + 1) / 2 > t) --tj;
+        while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
+        const int ti = t - tj * (tj + 1) / 2;
+        const int i0 = e + 4 * ti, j0 = e + 4 * tj;
+        T acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+        for (int p = 0; p < kp; ++p) {
+          const T* urow = F + (int64_t)(s + p) * lda;
+          const T* wrow = W + (int64_t)p * n;
+          T u[4], w[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            u[a] = i0 + a < n ? urow[i0 + a] : T(0);
+            w[a] = j0 + a < n ? wrow[j0 + a] : T(0);
+          }
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += u[a] * w[b];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const int i = i0 + a, j = j0 + b;
+            if (i <= j && j < n) F[(int64_t)i * lda + j] -= acc[a][b];
+          }
+      }
+    }
+    __syncthreads();
+    s = e;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBT) void k_batched_sytrf(BatchArgs<T> a) {
+  __shared__ T Lk[kBatchMaxWidth + 2];
+  __shared__ T Lr[kBatchMaxWidth + 2];
+  __shared__ double rv[kBT / 64];
+  __shared__ int ri[kBT / 64];
+  __shared__ int sq;
+  T* lds = reinterpret_cast<T*>(bbk_batched_lds);
+  for (;;) {
+    if (threadIdx.x == 0) sq = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int q = sq;
+    __syncthreads();
+    if (q >= a.count) break;
+    const int m = a.order[q];
+    const int n = a.ns[m];
+    if (n <= 0) continue;
+    const int bw = a.widths.w[batch_class(n)];
+    const int64_t need = (int64_t)(bw + 2) * n;
+    T* W = need <= a.lds_elems ? lds : a.ws + (int64_t)blockIdx.x * a.ws_stride;
+    factor_one<T>(a.A[m], a.lda[m], n, a.perm[m], a.piv[m], bw, W, Lk, Lr, rv, ri);
+  }
+}
+
+__global__ void k_batch_fill(int* p, int count, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// A_m(i, j) = A_m(j, i) = scale * u(seed + m, i, j) (SplitMix64 uniform in
+// [-1, 1)); A_m(i, i) = 0 when zero_diag_stride > 0 and i % stride == 0.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+template <typename T>
+__global__ void k_batched_gen(int count, const int* __restrict__ ns, unsigned long long seed,
+                              double scale, int zstride, T* const* A,
+                              const int64_t* __restrict__ lda) {
+  for (int m = blockIdx.x; m < count; m += gridDim.x) {
+    const int n = ns[m];
+    T* Am = A[m];
+    const int64_t ld = lda[m];
+    const unsigned long long base = splitmix64(seed + static_cast<unsigned long long>(m));
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
+      const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
+      const int lo = min(i, j), hi = max(i, j);
+      const unsigned long long key =
+          splitmix64(base ^ ((static_cast<unsigned long long>(lo) << 32) |
+                             static_cast<unsigned long long>(hi)));
+      double u = 2.0 * (static_cast<double>(key >> 11) * 0x1.0p-53) - 1.0;
+      if (i == j && zstride > 0 && i % zstride == 0) u = 0.0;
+      Am[(int64_t)i * ld + j] = static_cast<T>(scale * u);
+    }
+  }
+}
+
+// ---- host side -----------------------------------------------------------
+
+static int batch_lds_elems(int elem_bytes) { return kBatchLdsBytes / elem_bytes; }
+
+template <typename T>
+static int batched_grid(Handle* h, int count) {
+  int per_cu = 0, cus = 0;
+  const size_t lds = kBatchLdsBytes;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batched_sytrf<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kBatchLdsBytes);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&k_batched_sytrf<T>), kBT, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess ||
+      cus < 1)
+    cus = 1;
+  return std::max(1, std::min(count, per_cu * cus));
+}
+
+// Sort largest first, upload sizes and queue order, size the global W
+// slices for matrices whose panel rows do not fit LDS.
+int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
+  if (count < 0) return -2;
+  if (count > 0 && !h_n) return -3;
+  if (elem_bytes != 4 && elem_bytes != 8) return -4;
+  for (int i = 0; i < count; ++i)
+    if (h_n[i] < 0) return -3;
+  BatchPlan& p = h->batch;
+  const BatchWidths& wt = h->batch_widths[elem_bytes == 8 ? 0 : 1];
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h_n[a] > h_n[b]; });
+  const int lds_elems = batch_lds_elems(elem_bytes);
+  int64_t stride = 0;
+  for (int i = 0; i < count; ++i) {
+    const int n = h_n[i];
+    const int c = batch_class(n);
+    const int64_t need = (int64_t)(wt.w[c] + 2) * n;
+    if (need > lds_elems) stride = std::max(stride, need);
+  }
+  stride = (stride + 63) / 64 * 64;
+  const int grid = elem_bytes == 8 ? batched_grid<double>(h, std::max(count, 1))
+                                   : batched_grid<float>(h, std::max(count, 1));
+  const size_t ints = 2 * static_cast<size_t>(count) + 64;
+  int rc = h->reserve_buf(&p.buf, &p.buf_bytes, ints * sizeof(int), "batch schedule");
+  if (rc != BBK_OK) return rc;
+  const size_t wsb = static_cast<size_t>(stride) * grid * elem_bytes;
+  if (wsb > 0) {
+    rc = h->reserve_buf(&p.ws, &p.ws_bytes, wsb, "batch workspace");
+    if (rc != BBK_OK) return rc;
+  }
+  std::vector<int> host(ints, 0);
+  std::copy(order.begin(), order.end(), host.begin());
+  std::copy(h_n, h_n + count, host.begin() + count);
+  hipError_t e = hipMemcpyAsync(p.buf, host.data(), ints * sizeof(int), hipMemcpyHostToDevice,
+                                h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "upload batch schedule");
+  p.count = count;
+  p.elem = elem_bytes;
+  p.sizes.assign(h_n, h_n + count);
+  p.widths = wt;
+  p.ws_stride = stride;
+  p.grid = grid;
+  return BBK_OK;
+}
+
+static bool plan_matches(const BatchPlan& p, int count, const int* h_n, int elem_bytes,
+                         const BatchWidths& wt) {
+  if (p.count != count || p.elem != elem_bytes) return false;
+  for (int c = 0; c < kBatchClasses; ++c)
+    if (p.widths.w[c] != wt.w[c]) return false;
+  return count == 0 || std::equal(h_n, h_n + count, p.sizes.begin());
+}
+
+// Queue reset (fill kernel: graph-capture safe) + the persistent launch.
+template <typename T>
+int batched_launch(Handle* h, T* const* A, const int64_t* lda, int* const* perm,
+                   int* const* piv) {
+  const BatchPlan& p = h->batch;
+  if (p.count == 0) return BBK_OK;
+  int* base = reinterpret_cast<int*>(p.buf);
+  BatchArgs<T> a;
+  a.count = p.count;
+  a.order = base;
+  a.ns = base + p.count;
+  a.A = A;
+  a.lda = lda;
+  a.perm = perm;
+  a.piv = piv;
+  a.queue = base + 2 * p.count;
+  a.ws = reinterpret_cast<T*>(p.ws);
+  a.ws_stride = p.ws_stride;
+  a.lds_elems = batch_lds_elems(sizeof(T));
+  a.widths = p.widths;
+  hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, h->stream, a.queue, 1, 0);
+  hipLaunchKernelGGL(k_batched_sytrf<T>, dim3(p.grid), dim3(kBT), kBatchLdsBytes, h->stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h->hip_fail(e, "batched factorization launch");
+  return BBK_OK;
+}
+
+template <typename T>
+int batched_sytrf(Handle* h, int count, const int* h_n, T* const* A, const int64_t* lda,
+                  int* const* perm, int* const* piv) {
+  if (count < 0) return -2;
+  if (count > 0 && (!h_n)) return -3;
+  if (count > 0 && !A) return -4;
+  if (count > 0 && !lda) return -5;
+  if (count > 0 && !perm) return -6;
+  if (count > 0 && !piv) return -7;
+  const int eb = sizeof(T);
+  if (!plan_matches(h->batch, count, h_n, eb, h->batch_widths[eb == 8 ? 0 : 1])) {
+    int rc = batched_prepare(h, count, h_n, eb);
+    if (rc != BBK_OK) return rc;
+  }
+  return batched_launch<T>(h, A, lda, perm, piv);
+}
+
+template int batched_sytrf<double>(Handle*, int, const int*, double* const*, const int64_t*,
+                                   int* const*, int* const*);
+template int batched_sytrf<float>(Handle*, int, const int*, float* const*, const int64_t*,
+                                  int* const*, int* const*);
+
+template <typename T>
+int batched_gen(Handle* h, int count, const int* d_n, uint64_t seed, double scale,
+                int zstride, T* const* A, const int64_t* lda) {
+  if (count < 0) return -2;
+  if (count == 0) return BBK_OK;
+  if (!d_n) return -3;
+  if (!A) return -7;
+  if (!lda) return -8;
+  const int grid = std::min(count, 4096);
+  hipLaunchKernelGGL(k_batched_gen<T>, dim3(grid), dim3(256), 0, h->stream, count, d_n,
+                     static_cast<unsigned long long>(seed), scale, zstride, A, lda);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h->hip_fail(e, "batched generator launch");
+  return BBK_OK;
+}
+
+// ---- autotuner -----------------------------------------------------------
+// For every size class up to max_n: a synthetic batch with sizes spread
+// over the class (uniform [-1, 1] entries, zero diagonal on every 7th
+// index to force 2x2 pivots), each candidate width timed with HIP events
+// (best of two), the fastest kept in the handle's table.
+template <typename T>
+int batched_tune(Handle* h, int max_n, int* widths_out) {
+  static const int cand[] = {8, 16, 24, 32, 48, 64};
+  const int eb = sizeof(T);
+  BatchWidths& tab = h->batch_widths[eb == 8 ? 0 : 1];
+  int lo = 1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return h->fail(BBK_ERR_HIP, "event create");
+  int rc = BBK_OK;
+  for (int c = 0; c < kBatchClasses && rc == BBK_OK; ++c) {
+    const int hi = std::min(batch_class_bound(c), std::max(max_n, 1));
+    if (lo > hi) break;
+    // about 2^28 flops per trial batch, at least 32 and at most 2048 matrices
+    const double mean_n = 0.5 * (lo + hi);
+    int count = static_cast<int>(std::min(2048.0, std::max(32.0, 8.0e8 / (mean_n * mean_n * mean_n))));
+    std::vector<int> ns(count);
+    for (int i = 0; i < count; ++i) ns[i] = lo + static_cast<int>((static_cast<int64_t>(i) * 7919) % (hi - lo + 1));
+    std::vector<int64_t> lds(count);
+    size_t elems = 0;
+    std::vector<size_t> offs(count);
+    for (int i = 0; i < count; ++i) {
+      lds[i] = (ns[i] + 15) / 16 * 16;
+      offs[i] = elems;
+      elems += static_cast<size_t>(lds[i]) * ns[i];
+    }
+    void* mem = nullptr;
+    const size_t ptr_bytes = static_cast<size_t>(count) * (4 * sizeof(void*) + sizeof(int64_t) + sizeof(int));
+    const size_t bytes = 2 * elems * eb + 2 * static_cast<size_t>(count) * hi * sizeof(int) + ptr_bytes + 4096;
+    if (hipMalloc(&mem, bytes) != hipSuccess) {
+      rc = h->fail(BBK_ERR_ALLOC, "tuner buffers");
+      break;
+    }
+    char* b = reinterpret_cast<char*>(mem);
+    T* src = reinterpret_cast<T*>(b);
+    T* work = src + elems;
+    int* ip = reinterpret_cast<int*>(work + elems);
+    std::vector<void*> hA(count), hS(count), hP(count), hV(count);
+    for (int i = 0; i < count; ++i) {
+      hS[i] = src + offs[i];
+      hA[i] = work + offs[i];
+      hP[i] = ip + static_cast<size_t>(i) * hi;
+      hV[i] = ip + static_cast<size_t>(count) * hi + static_cast<size_t>(i) * hi;
+    }
+    char* pb = reinterpret_cast<char*>(ip + 2 * static_cast<size_t>(count) * hi);
+    pb = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(pb) + 255) / 256 * 256);
+    void** dA = reinterpret_cast<void**>(pb);
+    void** dS = dA + count;
+    void** dP = dS + count;
+    void** dV = dP + count;
+    int64_t* dL = reinterpret_cast<int64_t*>(dV + count);
+    int* dN = reinterpret_cast<int*>(dL + count);
+    hipError_t e = hipMemcpy(dA, hA.data(), count * sizeof(void*), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dS, hS.data(), count * sizeof(void*), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dP, hP.data(), count * sizeof(void*), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dV, hV.data(), count * sizeof(void*), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dL, lds.data(), count * sizeof(int64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dN, ns.data(), count * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      hipFree(mem);
+      rc = h->hip_fail(e, "tuner upload");
+      break;
+    }
+    rc = batched_gen<T>(h, count, dN, 0x7475u + c, 1.0, 7, reinterpret_cast<T* const*>(dS), dL);
+    float best_ms = 1e30f;
+    int best_w = tab.w[c];
+    for (int w : cand) {
+      if (rc != BBK_OK) break;
+      if (w > kBatchMaxWidth) continue;
+      BatchWidths saved = tab;
+      tab.w[c] = w;
+      float ms_min = 1e30f;
+      rc = batched_prepare(h, count, ns.data(), eb);
+      for (int rep = 0; rep < 2 && rc == BBK_OK; ++rep) {
+        e = hipMemcpyAsync(work, src, elems * eb, hipMemcpyDeviceToDevice, h->stream);
+        if (e != hipSuccess) { rc = h->hip_fail(e, "tuner copy"); break; }
+        hipEventRecord(e0, h->stream);
+        rc = batched_launch<T>(h, reinterpret_cast<T* const*>(dA), dL,
+                               reinterpret_cast<int* const*>(dP), reinterpret_cast<int* const*>(dV));
+        hipEventRecord(e1, h->stream);
+        e = hipEventSynchronize(e1);
+        if (e != hipSuccess) { rc = h->hip_fail(e, "tuner run"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms_min = std::min(ms_min, ms);
+      }
+      tab = saved;
+      if (ms_min < best_ms) {
+        best_ms = ms_min;
+        best_w = w;
+      }
+    }
+    tab.w[c] = best_w;
+    hipFree(mem);
+    lo = hi + 1;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  h->batch.count = -1;  // force a fresh plan with the tuned table
+  if (widths_out)
+    for (int c = 0; c < kBatchClasses; ++c) widths_out[c] = tab.w[c];
+  return rc;
+}
+
+// Capture one batched factorization (queue reset + persistent kernel) on
+// the handle's stream into a graph.
+template <typename T>
+int batched_graph(Handle* h, int count, const int* h_n, T* const* A, const int64_t* lda,
+                  int* const* perm, int* const* piv, hipGraph_t* g, hipGraphExec_t* x) {
+  const int eb = sizeof(T);
+  if (count < 0) return -2;
+  if (count > 0 && (!h_n || !A || !lda || !perm || !piv)) return -3;
+  if (!plan_matches(h->batch, count, h_n, eb, h->batch_widths[eb == 8 ? 0 : 1])) {
+    int rc = batched_prepare(h, count, h_n, eb);
+    if (rc != BBK_OK) return rc;
+  }
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return h->hip_fail(e, "begin capture");
+  int rc = batched_launch<T>(h, A, lda, perm, piv);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(h->stream, &graph);
+  if (rc != BBK_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return h->hip_fail(e, "end capture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    return h->hip_fail(e, "graph instantiate");
+  }
+  *g = graph;
+  *x = exec;
+  return BBK_OK;
+}
+
+}  // namespace bbk
+
+struct bbk_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  hipStream_t stream;
+};
+
+using bbk::Handle;
+
+extern "C" {
+
+int bbk_dsytrf_batched(bbk_handle_t handle, int count, const int* h_n, double* const* d_A,
+                       const int64_t* d_lda, int* const* d_perm, int* const* d_piv) {
+  if (!handle) return -1;
+  return bbk::batched_sytrf<double>(reinterpret_cast<Handle*>(handle), count, h_n, d_A, d_lda,
+                                    d_perm, d_piv);
+}
+
+int bbk_ssytrf_batched(bbk_handle_t handle, int count, const int* h_n, float* const* d_A,
+                       const int64_t* d_lda, int* const* d_perm, int* const* d_piv) {
+  if (!handle) return -1;
+  return bbk::batched_sytrf<float>(reinterpret_cast<Handle*>(handle), count, h_n, d_A, d_lda,
+                                   d_perm, d_piv);
+}
+
+int bbk_batched_prepare(bbk_handle_t handle, int count, const int* h_n, int elem_bytes) {
+  if (!handle) return -1;
+  return bbk::batched_prepare(reinterpret_cast<Handle*>(handle), count, h_n, elem_bytes);
+}
+
+int bbk_batched_get_widths(bbk_handle_t handle, int elem_bytes, int* widths) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (!widths) return -3;
+  const bbk::BatchWidths& t =
+      reinterpret_cast<Handle*>(handle)->batch_widths[elem_bytes == 8 ? 0 : 1];
+  for (int c = 0; c < bbk::kBatchClasses; ++c) widths[c] = t.w[c];
+  return BBK_OK;
+}
+
+int bbk_batched_set_widths(bbk_handle_t handle, int elem_bytes, const int* widths) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (!widths) return -3;
+  for (int c = 0; c < bbk::kBatchClasses; ++c)
+    if (widths[c] < 1 || widths[c] > bbk::kBatchMaxWidth) return -3;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  bbk::BatchWidths& t = h->batch_widths[elem_bytes == 8 ? 0 : 1];
+  for (int c = 0; c < bbk::kBatchClasses; ++c) t.w[c] = widths[c];
+  h->batch.count = -1;
+  return BBK_OK;
+}
+
+int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n, int* widths_out) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (max_n < 1) return -3;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  return elem_bytes == 8 ? bbk::batched_tune<double>(h, max_n, widths_out)
+                         : bbk::batched_tune<float>(h, max_n, widths_out);
+}
+
+int bbk_graph_create_dsytrf_batched(bbk_handle_t handle, int count, const int* h_n,
+                                    double* const* d_A, const int64_t* d_lda,
+                                    int* const* d_perm, int* const* d_piv,
+                                    bbk_graph_t* graph) {
+  if (!handle) return -1;
+  if (!graph) return -8;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  int rc = bbk::batched_graph<double>(h, count, h_n, d_A, d_lda, d_perm, d_piv, &g, &x);
+  if (rc != BBK_OK) return rc;
+  *graph = new bbk_graph{g, x, h->stream};
+  return BBK_OK;
+}
+
+int bbk_graph_create_ssytrf_batched(bbk_handle_t handle, int count, const int* h_n,
+                                    float* const* d_A, const int64_t* d_lda,
+                                    int* const* d_perm, int* const* d_piv,
+                                    bbk_graph_t* graph) {
+  if (!handle) return -1;
+  if (!graph) return -8;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  int rc = bbk::batched_graph<float>(h, count, h_n, d_A, d_lda, d_perm, d_piv, &g, &x);
+  if (rc != BBK_OK) return rc;
+  *graph = new bbk_graph{g, x, h->stream};
+  return BBK_OK;
+}
+
+int bbk_graph_launch(bbk_graph_t graph) {
+  if (!graph) return -1;
+  return hipGraphLaunch(graph->exec, graph->stream) == hipSuccess ? BBK_OK : BBK_ERR_HIP;
+}
+
+int bbk_graph_destroy(bbk_graph_t graph) {
+  if (!graph) return -1;
+  if (graph->exec) hipGraphExecDestroy(graph->exec);
+  if (graph->graph) hipGraphDestroy(graph->graph);
+  delete graph;
+  return BBK_OK;
+}
+
+int bbk_dgen_synth_batched(bbk_handle_t handle, int count, const int* d_n, uint64_t seed,
+                           double scale, int zero_diag_stride, double* const* d_A,
+                           const int64_t* d_lda) {
+  if (!handle) return -1;
+  return bbk::batched_gen<double>(reinterpret_cast<Handle*>(handle), count, d_n, seed, scale,
+                                  zero_diag_stride, d_A, d_lda);
+}
+
+int bbk_sgen_synth_batched(bbk_handle_t handle, int count, const int* d_n, uint64_t seed,
+                           double scale, int zero_diag_stride, float* const* d_A,
+                           const int64_t* d_lda) {
+  if (!handle) return -1;
+  return bbk::batched_gen<float>(reinterpret_cast<Handle*>(handle), count, d_n, seed, scale,
+                                 zero_diag_stride, d_A, d_lda);
+}
+
+}  // extern "C"

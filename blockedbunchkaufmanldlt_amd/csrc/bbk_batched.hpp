@@ -1,0 +1,46 @@
+// bbk_batched.hpp -- batched factorization: size classes, panel-width
+// table and the per-handle schedule.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "bbk_internal.hpp"
+
+namespace bbk {
+
+constexpr int kBatchThreads = 256;
+constexpr int kBatchClasses = 8;    // n <= 32, 64, 128, 256, 512, 1024, 2048, larger
+constexpr int kBatchMaxWidth = 64;  // widest panel a batched matrix may use
+// Dynamic LDS per workgroup for the panel rows; two workgroups fit a CU.
+constexpr int kBatchLdsBytes = 76 * 1024;
+
+// Upper size bound of class c (the last class is unbounded).
+__host__ __device__ constexpr int batch_class_bound(int c) {
+  return c < kBatchClasses - 1 ? (32 << c) : 0x7fffffff;
+}
+
+struct BatchWidths {
+  int w[kBatchClasses];
+};
+
+// Default table (used until bbk_batched_tune or bbk_batched_set_widths).
+constexpr BatchWidths kDefaultBatchWidthsF64 = {{8, 16, 16, 32, 32, 32, 32, 32}};
+constexpr BatchWidths kDefaultBatchWidthsF32 = {{8, 16, 16, 32, 32, 32, 32, 32}};
+
+// The schedule of the last prepared batch: queue order, sizes, counter.
+struct BatchPlan {
+  int count = -1;
+  int elem = 0;
+  std::vector<int> sizes;
+  BatchWidths widths{};
+  void* buf = nullptr;  // ints: order[count], n[count], queue counter
+  size_t buf_bytes = 0;
+  void* ws = nullptr;   // global panel rows for matrices beyond LDS
+  size_t ws_bytes = 0;
+  int64_t ws_stride = 0;
+  int grid = 0;
+};
+
+}  // namespace bbk

@@ -217,6 +217,8 @@ Handle::~Handle() {
   if (solve_ws) (void)hipFree(solve_ws);
   if (berr_ws) (void)hipFree(berr_ws);
   if (mixed_ws) (void)hipFree(mixed_ws);
+  if (batch.buf) (void)hipFree(batch.buf);
+  if (batch.ws) (void)hipFree(batch.ws);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -238,6 +240,8 @@ int bbk_create(bbk_handle_t* handle, void* stream) {
     h->stream = reinterpret_cast<hipStream_t>(stream);
     h->own_stream = false;
   } else {
+    // non-blocking: cooperative launches are refused on streams that
+    // synchronize with the null stream
     e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       delete h;
@@ -246,6 +250,13 @@ int bbk_create(bbk_handle_t* handle, void* stream) {
     h->own_stream = true;
   }
   *handle = reinterpret_cast<bbk_handle_t>(h);
+  return BBK_OK;
+}
+
+int bbk_get_stream(bbk_handle_t handle, void** stream) {
+  if (!handle) return -1;
+  if (!stream) return -2;
+  *stream = reinterpret_cast<Handle*>(handle)->stream;
   return BBK_OK;
 }
 

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "bbk_batched.hpp"
+
 namespace bbk {
 
 struct Handle {
@@ -32,6 +34,8 @@ struct Handle {
   bool timing = false;
   int panels = 0;
   std::vector<hipEvent_t> marks[kGroups];
+  BatchPlan batch;                 // last prepared batched schedule
+  BatchWidths batch_widths[2] = {kDefaultBatchWidthsF64, kDefaultBatchWidthsF32};
 
   ~Handle();
   int fail(int code, const char* what);
@@ -55,6 +59,10 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
           const int* piv, T* B, int64_t ldb);
+int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes);
+template <typename T>
+int batched_sytrf(Handle* h, int count, const int* h_n, T* const* A, const int64_t* lda,
+                  int* const* perm, int* const* piv);
 int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
          const double* B, int64_t ldb, const double* X, int64_t ldx, double* R,
          int64_t ldr, double* d_eta, double anorm, bool want_norm);

@@ -78,6 +78,40 @@ __device__ __forceinline__ bool xget(const XRec* p, int tag, int* aux, double* v
 // Wait until this wave's outstanding vector memory operations completed.
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Pivot arithmetic kept free of FMA contraction so decisions and the 2x2
+// inverse round like the reference.
+__device__ __forceinline__ int select_pivot_nc(double akk, double omega, double sigma,
+                                               double arr) {
+#pragma clang fp contract(off)
+  if (akk * sigma - kAlpha * omega * omega >= 0.0) return 3;
+  if (arr - kAlpha * sigma >= 0.0) return -1;
+  return 2;
+}
+
+__device__ __forceinline__ void inverse_2x2_nc(double a, double b, double c, double* itl,
+                                               double* ibr, double* ioff) {
+#pragma clang fp contract(off)
+  const double det = a * c - b * b;
+  *itl = c / det;
+  *ibr = a / det;
+  *ioff = -b / det;
+}
+
+__device__ __forceinline__ void apply_2x2_nc(double itl, double ibr, double ioff, double w0,
+                                             double w1, double* u0, double* u1) {
+#pragma clang fp contract(off)
+  *u0 = itl * w0 + ioff * w1;
+  *u1 = ioff * w0 + ibr * w1;
+}
+
+__device__ __forceinline__ double safe_den(double d) {
+  return fabs(d) <= kDblEps ? 0.0 : d;  // SafeDenominator
+}
+
+__device__ __forceinline__ double div_or_zero(double w, double den) {
+  return den == 0.0 ? 0.0 : w / den;
+}
+
 // Index of the first maximum: larger value wins, ties go to the lower
 // index (matches the strict '>' scan of src/block_ldlt_decomposition.cc:173).
 __device__ __forceinline__ void argmax_combine(double& v, int& i, double v2,

@@ -62,40 +62,6 @@ __device__ __forceinline__ T panel_dot(const T* L, const T* Wl, int tid, int kk)
   return (a0 + a1) + (a2 + a3);
 }
 
-// Pivot arithmetic kept free of FMA contraction so decisions and the 2x2
-// inverse round like the reference.
-__device__ __forceinline__ int select_pivot_nc(double akk, double omega, double sigma,
-                                               double arr) {
-#pragma clang fp contract(off)
-  if (akk * sigma - kAlpha * omega * omega >= 0.0) return 3;
-  if (arr - kAlpha * sigma >= 0.0) return -1;
-  return 2;
-}
-
-__device__ __forceinline__ void inverse_2x2_nc(double a, double b, double c, double* itl,
-                                               double* ibr, double* ioff) {
-#pragma clang fp contract(off)
-  const double det = a * c - b * b;
-  *itl = c / det;
-  *ibr = a / det;
-  *ioff = -b / det;
-}
-
-__device__ __forceinline__ void apply_2x2_nc(double itl, double ibr, double ioff, double w0,
-                                             double w1, double* u0, double* u1) {
-#pragma clang fp contract(off)
-  *u0 = itl * w0 + ioff * w1;
-  *u1 = ioff * w0 + ibr * w1;
-}
-
-__device__ __forceinline__ double safe_den(double d) {
-  return fabs(d) <= kDblEps ? 0.0 : d;  // SafeDenominator
-}
-
-__device__ __forceinline__ double div_or_zero(double w, double den) {
-  return den == 0.0 ? 0.0 : w / den;
-}
-
 // Panel-row scaling table: row q of L^T = D^-1 W.
 //   type 1 : U(q) = W(q) / c0            (c0 = SafeDenominator(d), 0 -> 0)
 //   type 2 : U(q) = c0 W(q) + c1 W(q+1)  (c0 = itl, c1 = ioff)

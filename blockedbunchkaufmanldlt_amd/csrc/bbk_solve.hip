@@ -763,13 +763,5 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   return best <= tol ? BBK_OK : BBK_ERR_NOT_CONVERGED;
 }
 
-int bbk_dsytrf_batched(bbk_handle_t, int, const int*, double* const*, const int64_t*,
-                       int* const*, int* const*) {
-  return BBK_ERR_UNSUPPORTED;
-}
-int bbk_ssytrf_batched(bbk_handle_t, int, const int*, float* const*, const int64_t*,
-                       int* const*, int* const*) {
-  return BBK_ERR_UNSUPPORTED;
-}
 
 }  // extern "C"

@@ -47,8 +47,13 @@ extern "C" {
 typedef struct bbk_handle* bbk_handle_t;
 
 /* Create a handle on the current HIP device.  stream may be NULL (the
- * handle then creates its own default-priority stream). */
+ * handle then creates its own default-priority non-blocking stream; the
+ * caller orders it against its own streams, e.g. with events).  A given
+ * stream must not be the null stream or another stream that synchronizes
+ * with it: cooperative launches are refused there. */
 int bbk_create(bbk_handle_t* handle, void* stream);
+/* The stream the handle queues its work on. */
+int bbk_get_stream(bbk_handle_t handle, void** stream);
 int bbk_destroy(bbk_handle_t handle);
 /* Block until all work queued on the handle's stream has finished. */
 int bbk_synchronize(bbk_handle_t handle);
@@ -110,19 +115,66 @@ int bbk_dsysv(bbk_handle_t handle, int n, int nrhs, const double* d_A,
               int64_t lda, const double* d_B, int64_t ldb, double* d_X,
               int64_t ldx, double* berr);
 
-/* ---- batched factorization (new) -------------------------------------
+/* ---- batched factorization (new; no reference counterpart) ----------
  * Factor count independent matrices of uneven sizes in one persistent
- * launch.  h_n: host array of sizes; d_A: device array of device
- * pointers; d_lda: device array of leading dimensions; d_perm/d_piv:
- * device arrays of device pointers (n ints each).  Matrices are scheduled
- * largest first over a device work queue; the panel width per size class
- * comes from the autotuned table (bbk_tune_batched). */
+ * launch: every workgroup takes matrices from a device work queue ordered
+ * largest first and factors each one by itself (same Bunch-Kaufman
+ * decisions and output encoding as bbk_?sytrf).  h_n: host array of
+ * orders; d_A: device array of device pointers (row-major, upper
+ * triangle); d_lda: device array of leading dimensions; d_perm/d_piv:
+ * device arrays of device pointers (n ints each).  The panel width of
+ * each matrix comes from the handle's size-class table.
+ * When h_n and the table are those of the last call (or of
+ * bbk_batched_prepare), the call only queues two kernels on the handle's
+ * stream (no host-device copy, no allocation) and may be captured into a
+ * HIP graph; otherwise it first uploads the schedule (synchronous). */
 int bbk_dsytrf_batched(bbk_handle_t handle, int count, const int* h_n,
                        double* const* d_A, const int64_t* d_lda,
                        int* const* d_perm, int* const* d_piv);
 int bbk_ssytrf_batched(bbk_handle_t handle, int count, const int* h_n,
                        float* const* d_A, const int64_t* d_lda,
                        int* const* d_perm, int* const* d_piv);
+/* Upload the schedule of a batch (sizes on the host) for elem_bytes 8
+ * (binary64) or 4 (binary32).  Synchronous. */
+int bbk_batched_prepare(bbk_handle_t handle, int count, const int* h_n,
+                        int elem_bytes);
+
+/* Size classes n <= 32, 64, 128, 256, 512, 1024, 2048 and larger; one
+ * panel width (1..64) per class and precision. */
+#define BBK_BATCH_CLASSES 8
+int bbk_batched_get_widths(bbk_handle_t handle, int elem_bytes, int* widths);
+int bbk_batched_set_widths(bbk_handle_t handle, int elem_bytes,
+                           const int* widths);
+/* Autotuner: for every size class up to max_n, times candidate panel
+ * widths on a synthetic batch of that class and keeps the fastest in the
+ * handle's table (written to widths_out when not NULL).  Synchronous. */
+int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n,
+                     int* widths_out);
+
+/* HIP graph of one batched factorization (the queue reset and the
+ * persistent kernel), captured on the handle's stream for fixed
+ * arguments; bbk_graph_launch replays it on the same stream. */
+typedef struct bbk_graph* bbk_graph_t;
+int bbk_graph_create_dsytrf_batched(bbk_handle_t handle, int count,
+                                    const int* h_n, double* const* d_A,
+                                    const int64_t* d_lda, int* const* d_perm,
+                                    int* const* d_piv, bbk_graph_t* graph);
+int bbk_graph_create_ssytrf_batched(bbk_handle_t handle, int count,
+                                    const int* h_n, float* const* d_A,
+                                    const int64_t* d_lda, int* const* d_perm,
+                                    int* const* d_piv, bbk_graph_t* graph);
+int bbk_graph_launch(bbk_graph_t graph);
+int bbk_graph_destroy(bbk_graph_t graph);
+
+/* Batched synthetic inputs: matrix m gets bbk_dgen_synth's formula with
+ * seed + m (binary32 form: the same values rounded to binary32).  d_n:
+ * device array of orders. */
+int bbk_dgen_synth_batched(bbk_handle_t handle, int count, const int* d_n,
+                           uint64_t seed, double scale, int zero_diag_stride,
+                           double* const* d_A, const int64_t* d_lda);
+int bbk_sgen_synth_batched(bbk_handle_t handle, int count, const int* d_n,
+                           uint64_t seed, double scale, int zero_diag_stride,
+                           float* const* d_A, const int64_t* d_lda);
 
 /* ---- kernels.h counterparts (device pointers, asynchronous) ----------
  * Element strides, no return value semantics of the reference become

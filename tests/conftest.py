@@ -21,6 +21,20 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP GPU (MI355X)")
 
 
+def pytest_collection_modifyitems(config, items):
+    """Run the tests that start GPU programs in subprocesses first.
+
+    The panel and triangular-solve kernels are cooperative launches, which
+    use the device's global wave sync resource; only one process at a time
+    can hold it.  Once this pytest process has made a cooperative launch,
+    a child process's cooperative launch is refused, so the drop-in tests
+    (tests/test_gpu_dropin.py, which run compiled programs) go before every
+    test that launches in-process."""
+    first = [it for it in items if it.module.__name__.endswith("test_gpu_dropin")]
+    rest = [it for it in items if not it.module.__name__.endswith("test_gpu_dropin")]
+    items[:] = first + rest
+
+
 @pytest.fixture(scope="session")
 def oracle():
     import oracle as O
