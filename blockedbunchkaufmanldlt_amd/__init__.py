@@ -71,6 +71,8 @@ def lib() -> ctypes.CDLL:
     L.bbk_batched_get_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_set_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_tune.argtypes = [vp, i32, i32, ip]
+    L.bbk_batched_get_team_sizes.argtypes = [vp, i32, ip]
+    L.bbk_batched_set_team_sizes.argtypes = [vp, i32, ip]
     for name in ("bbk_graph_create_dsytrf_batched", "bbk_graph_create_ssytrf_batched"):
         getattr(L, name).argtypes = [vp, i32, ip, vp, vp, vp, vp, ctypes.POINTER(vp)]
     L.bbk_graph_launch.argtypes = [vp]
@@ -288,6 +290,17 @@ class Handle:
         eb = 8 if dtype == self._torch.float64 else 4
         arr = (ctypes.c_int * 8)(*widths)
         self.check(lib().bbk_batched_set_widths(self.h, eb, arr), "batched_set_widths")
+
+    def batched_team_sizes(self, dtype):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)()
+        self.check(lib().bbk_batched_get_team_sizes(self.h, eb, arr), "batched_get_team_sizes")
+        return list(arr)
+
+    def set_batched_team_sizes(self, dtype, sizes: Sequence[int]):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)(*sizes)
+        self.check(lib().bbk_batched_set_team_sizes(self.h, eb, arr), "batched_set_team_sizes")
 
     def batched_tune(self, dtype, max_n: int):
         eb = 8 if dtype == self._torch.float64 else 4

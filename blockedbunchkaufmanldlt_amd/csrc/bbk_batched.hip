@@ -1,9 +1,11 @@
 // bbk_batched.hip -- many independent Bunch-Kaufman LDL^T factorizations
 // in one persistent launch.
 //
-// Each workgroup (256 threads) pulls matrix numbers from a device work
-// queue (one atomic counter; the host orders the queue largest first) and
-// factors the whole matrix by itself with the reference's blocked loop
+// Each workgroup -- one wavefront when every matrix of the batch has
+// n <= 256, so that more matrices are in flight, else four -- pulls matrix
+// numbers from a device work queue (one atomic counter; the host orders the
+// queue largest first) and factors the whole matrix by itself with the
+// reference's blocked loop
 // (src/block_ldlt_decomposition.cc:50-59,80-120): per panel, each pivot
 // row is formed from the panel's unscaled rows W = D L^T, the
 // Bunch-Kaufman tests pick a 1x1, a swapped 1x1 or a 2x2 pivot
@@ -33,8 +35,6 @@
 #include "bbk_handle.hpp"
 
 namespace bbk {
-
-constexpr int kBT = kBatchThreads;
 
 template <typename T>
 struct BatchArgs {
@@ -67,16 +67,59 @@ __device__ __forceinline__ void swap_elems(T* a, T* b) {
   *b = t;
 }
 
-// One matrix, whole workgroup.  F: n x lda row-major upper triangle;
-// W: (bw + 2) rows of n elements (rows 0..bw = panel rows, last = the
-// candidate row C).
+// Reductions over the team (one wavefront or the whole workgroup); every
+// thread receives the result.
+template <int NT>
+__device__ __forceinline__ void team_argmax(double& v, int& i, double* sv, int* si) {
+  if constexpr (NT == 64) {
+    wave_argmax(v, i);
+  } else {
+    block_argmax(v, i, sv, si);
+  }
+}
+template <int NT>
+__device__ __forceinline__ double team_max(double v, double* sv) {
+  if constexpr (NT == 64) {
+    return wave_max_nan_ignoring(v);
+  } else {
+    return block_max_nan_ignoring(v, sv);
+  }
+}
+
+// Per-panel scaling record of row p (LDS): ty 1: U = W / c0 (c0 =
+// SafeDenominator(d)); ty 2: U = c0 W(p) + c1 W(p+1); ty -2: U = c0 W(p-1) +
+// c1 W(p).  U(p, col) rebuilt from W equals the value the scaling step wrote
+// to F(s + p, col) bit for bit (same operands, same operations), also after
+// later column interchanges, which move F and W columns together.
+struct ScaleTab {
+  int ty[kBatchMaxWidth + 2];
+  double c0[kBatchMaxWidth + 2];
+  double c1[kBatchMaxWidth + 2];
+};
+
 template <typename T>
+__device__ __forceinline__ T ucol(const ScaleTab& st, const T* W, int n, int p, int col) {
+#pragma clang fp contract(off)
+  const int ty = st.ty[p];
+  const double w = static_cast<double>(W[(int64_t)p * n + col]);
+  if (ty == 1) return static_cast<T>(div_or_zero(w, st.c0[p]));
+  if (ty == 2)
+    return static_cast<T>(st.c0[p] * w +
+                          st.c1[p] * static_cast<double>(W[(int64_t)(p + 1) * n + col]));
+  return static_cast<T>(st.c0[p] * static_cast<double>(W[(int64_t)(p - 1) * n + col]) +
+                        st.c1[p] * w);
+}
+
+// One matrix, whole team of NT threads.  F: n x lda row-major upper
+// triangle; W: (bw + 2) rows of n elements (rows 0..bw = panel rows, last =
+// the candidate row C).
+template <typename T, int NT>
 __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
-                           int* __restrict__ piv, int bw, T* W, T* Lk, T* Lr, double* rv,
-                           int* ri) {
+                           int* __restrict__ piv, int bw, T* W, T* Lk, T* Lr, ScaleTab& st,
+                           double* rv, int* ri) {
   const int tid = threadIdx.x;
   T* C = W + (int64_t)(bw + 1) * n;
-  for (int i = tid; i < n; i += kBT) {
+  for (int i = tid; i < n; i += NT) {
     perm[i] = i;
     piv[i] = 1;
   }
@@ -87,13 +130,13 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
     int kk = 0;
     while (s + kk < e) {
       const int k = s + kk;
-      if (tid < kk) Lk[tid] = F[(int64_t)(s + tid) * lda + k];
+      if (tid < kk) Lk[tid] = ucol<T>(st, W, n, tid, k);  // L(k, s + tid)
       __syncthreads();
       // W(k, j) = F(k, j) - sum_p L(k, p) W(p, j), j >= k  (:122-162)
       double best = 0.0;
       int arg = n;
       T* Wk = W + (int64_t)kk * n;
-      for (int j = k + tid; j < n; j += kBT) {
+      for (int j = k + tid; j < n; j += NT) {
         T acc = T(0);
         for (int p = 0; p < kk; ++p) acc += Lk[p] * W[(int64_t)p * n + j];
         const T w = F[(int64_t)k * lda + j] - acc;
@@ -104,7 +147,8 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
           arg = j;
         }
       }
-      block_argmax(best, arg, rv, ri);  // (:164-182) first maximum
+      team_argmax<NT>(best, arg, rv, ri);  // (:164-182) first maximum
+      __syncthreads();
       const double omega = best;
       const int r = arg;
       const double akk = fabs(static_cast<double>(Wk[k]));
@@ -112,10 +156,10 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
       if (!(akk >= kAlpha * omega || omega == 0.0)) {
         // candidate row r: C(i) = A(r, i) - sum_p L(r, p) W(p, i), i >= k
         // (:184-352), sigma over i != r (:354-367)
-        if (tid < kk) Lr[tid] = F[(int64_t)(s + tid) * lda + r];
+        if (tid < kk) Lr[tid] = ucol<T>(st, W, n, tid, r);  // L(r, s + tid)
         __syncthreads();
         double sig = 0.0;
-        for (int i = k + tid; i < n; i += kBT) {
+        for (int i = k + tid; i < n; i += NT) {
           const T base = i < r ? F[(int64_t)i * lda + r] : F[(int64_t)r * lda + i];
           T acc = T(0);
           for (int p = 0; p < kk; ++p) acc += Lr[p] * W[(int64_t)p * n + i];
@@ -124,7 +168,8 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
           const double a = fabs(static_cast<double>(cv));
           if (i != r) sig = (sig < a) ? a : sig;
         }
-        const double sigma = block_max_nan_ignoring(sig, rv);
+        const double sigma = team_max<NT>(sig, rv);
+        __syncthreads();
         const double arr = fabs(static_cast<double>(C[r]));
         kind = select_pivot_nc(akk, omega, sigma, arr);  // 3, -1 or 2
         if (kind == 2 && kk == e - s - 1 && e < n) {
@@ -139,15 +184,15 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
         __syncthreads();
         if (kind == -1) {
           // interchange k <-> r (:414-498); row k of W becomes the candidate
-          for (int j = r + 1 + tid; j < n; j += kBT)
+          for (int j = r + 1 + tid; j < n; j += NT)
             swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)r * lda + j]);
-          for (int j = k + 1 + tid; j < r; j += kBT)
+          for (int j = k + 1 + tid; j < r; j += NT)
             swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)j * lda + r]);
-          for (int i = tid; i < k; i += kBT)
+          for (int i = tid; i < k; i += NT)
             swap_elems(&F[(int64_t)i * lda + k], &F[(int64_t)i * lda + r]);
-          for (int p = tid; p < kk; p += kBT)
+          for (int p = tid; p < kk; p += NT)
             swap_elems(&W[(int64_t)p * n + k], &W[(int64_t)p * n + r]);
-          for (int j = k + 1 + tid; j < n; j += kBT)
+          for (int j = k + 1 + tid; j < n; j += NT)
             if (j != r) Wk[j] = C[j];
           if (tid == 0) {
             swap_elems(&F[(int64_t)k * lda + k], &F[(int64_t)r * lda + r]);
@@ -161,16 +206,16 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
           const int k1 = k + 1;
           T* Wk1 = W + (int64_t)(kk + 1) * n;
           if (r != k1) {
-            for (int j = r + 1 + tid; j < n; j += kBT)
+            for (int j = r + 1 + tid; j < n; j += NT)
               swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)r * lda + j]);
-            for (int j = k + 2 + tid; j < r; j += kBT)
+            for (int j = k + 2 + tid; j < r; j += NT)
               swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)j * lda + r]);
-            for (int i = tid; i < k; i += kBT)
+            for (int i = tid; i < k; i += NT)
               swap_elems(&F[(int64_t)i * lda + k1], &F[(int64_t)i * lda + r]);
-            for (int p = tid; p < kk; p += kBT)
+            for (int p = tid; p < kk; p += NT)
               swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
           }
-          for (int j = k + 2 + tid; j < n; j += kBT)
+          for (int j = k + 2 + tid; j < n; j += NT)
             if (j != r) Wk1[j] = C[j];
           if (tid == 0) {
             if (r != k1) {
@@ -191,11 +236,13 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
         // ScaleOneByOnePivot (:633-656)
         const double d = static_cast<double>(Wk[k]);
         const double den = safe_den(d);
-        for (int j = k + 1 + tid; j < n; j += kBT)
+        for (int j = k + 1 + tid; j < n; j += NT)
           F[(int64_t)k * lda + j] = static_cast<T>(div_or_zero(static_cast<double>(Wk[j]), den));
         if (tid == 0) {
           F[(int64_t)k * lda + k] = Wk[k];
           piv[k] = 1;
+          st.ty[kk] = 1;
+          st.c0[kk] = den;
         }
         kk += 1;
       } else {
@@ -207,7 +254,7 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
         const double c = static_cast<double>(Wk1[k1]);
         double itl, ibr, ioff;
         inverse_2x2_nc(a, b, c, &itl, &ibr, &ioff);
-        for (int j = k + 2 + tid; j < n; j += kBT) {
+        for (int j = k + 2 + tid; j < n; j += NT) {
           double u0, u1;
           apply_2x2_nc(itl, ibr, ioff, static_cast<double>(Wk[j]), static_cast<double>(Wk1[j]),
                        &u0, &u1);
@@ -220,6 +267,12 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
           F[(int64_t)k1 * lda + k1] = Wk1[k1];
           piv[k] = 2;
           piv[k1] = -2;
+          st.ty[kk] = 2;
+          st.c0[kk] = itl;
+          st.c1[kk] = ioff;
+          st.ty[kk + 1] = -2;
+          st.c0[kk + 1] = ioff;
+          st.c1[kk + 1] = ibr;
         }
         kk += 2;
       }
@@ -231,7 +284,7 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
     if (m > 0 && kp > 0) {
       const int nt = (m + 3) >> 2;
       const int tiles = nt * (nt + 1) / 2;
-      for (int t = tid; t < tiles; t += kBT) {
+      for (int t = tid; t < tiles; t += NT) {
         int tj = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
         while (tj * (tj + 1) / 2 > t) --tj;
         while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
@@ -270,12 +323,13 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBT) void k_batched_sytrf(BatchArgs<T> a) {
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_batched_sytrf(BatchArgs<T> a) {
   __shared__ T Lk[kBatchMaxWidth + 2];
   __shared__ T Lr[kBatchMaxWidth + 2];
-  __shared__ double rv[kBT / 64];
-  __shared__ int ri[kBT / 64];
+  __shared__ ScaleTab st;
+  __shared__ double rv[NT / 64];
+  __shared__ int ri[NT / 64];
   __shared__ int sq;
   T* lds = reinterpret_cast<T*>(bbk_batched_lds);
   for (;;) {
@@ -290,7 +344,7 @@ __global__ __launch_bounds__(kBT) void k_batched_sytrf(BatchArgs<T> a) {
     const int bw = a.widths.w[batch_class(n)];
     const int64_t need = (int64_t)(bw + 2) * n;
     T* W = need <= a.lds_elems ? lds : a.ws + (int64_t)blockIdx.x * a.ws_stride;
-    factor_one<T>(a.A[m], a.lda[m], n, a.perm[m], a.piv[m], bw, W, Lk, Lr, rv, ri);
+    factor_one<T, NT>(a.A[m], a.lda[m], n, a.perm[m], a.piv[m], bw, W, Lk, Lr, st, rv, ri);
   }
 }
 
@@ -332,16 +386,14 @@ __global__ void k_batched_gen(int count, const int* __restrict__ ns, unsigned lo
 
 // ---- host side -----------------------------------------------------------
 
-static int batch_lds_elems(int elem_bytes) { return kBatchLdsBytes / elem_bytes; }
-
-template <typename T>
-static int batched_grid(Handle* h, int count) {
+template <typename T, int NT>
+static int batched_grid_nt(Handle* h, int count) {
   int per_cu = 0, cus = 0;
-  const size_t lds = kBatchLdsBytes;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batched_sytrf<T>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kBatchLdsBytes);
+  const int lds = batch_lds_bytes(NT);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batched_sytrf<T, NT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(&k_batched_sytrf<T>), kBT, lds) != hipSuccess ||
+          &per_cu, reinterpret_cast<const void*>(&k_batched_sytrf<T, NT>), NT, lds) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) !=
@@ -349,6 +401,12 @@ static int batched_grid(Handle* h, int count) {
       cus < 1)
     cus = 1;
   return std::max(1, std::min(count, per_cu * cus));
+}
+
+static int batched_grid(Handle* h, int count, int elem_bytes, int nt) {
+  if (elem_bytes == 8)
+    return nt == 64 ? batched_grid_nt<double, 64>(h, count) : batched_grid_nt<double, 256>(h, count);
+  return nt == 64 ? batched_grid_nt<float, 64>(h, count) : batched_grid_nt<float, 256>(h, count);
 }
 
 // Sort largest first, upload sizes and queue order, size the global W
@@ -364,7 +422,10 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   std::vector<int> order(count);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h_n[a] > h_n[b]; });
-  const int lds_elems = batch_lds_elems(elem_bytes);
+  int nmax = 0;
+  for (int i = 0; i < count; ++i) nmax = std::max(nmax, h_n[i]);
+  const int nt = wt.nt[batch_class(nmax)] == 64 ? 64 : 256;
+  const int lds_elems = batch_lds_bytes(nt) / elem_bytes;
   int64_t stride = 0;
   for (int i = 0; i < count; ++i) {
     const int n = h_n[i];
@@ -373,8 +434,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
     if (need > lds_elems) stride = std::max(stride, need);
   }
   stride = (stride + 63) / 64 * 64;
-  const int grid = elem_bytes == 8 ? batched_grid<double>(h, std::max(count, 1))
-                                   : batched_grid<float>(h, std::max(count, 1));
+  const int grid = batched_grid(h, std::max(count, 1), elem_bytes, nt);
   const size_t ints = 2 * static_cast<size_t>(count) + 64;
   int rc = h->reserve_buf(&p.buf, &p.buf_bytes, ints * sizeof(int), "batch schedule");
   if (rc != BBK_OK) return rc;
@@ -396,6 +456,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   p.widths = wt;
   p.ws_stride = stride;
   p.grid = grid;
+  p.nt = nt;
   return BBK_OK;
 }
 
@@ -403,7 +464,7 @@ static bool plan_matches(const BatchPlan& p, int count, const int* h_n, int elem
                          const BatchWidths& wt) {
   if (p.count != count || p.elem != elem_bytes) return false;
   for (int c = 0; c < kBatchClasses; ++c)
-    if (p.widths.w[c] != wt.w[c]) return false;
+    if (p.widths.w[c] != wt.w[c] || p.widths.nt[c] != wt.nt[c]) return false;
   return count == 0 || std::equal(h_n, h_n + count, p.sizes.begin());
 }
 
@@ -425,10 +486,15 @@ int batched_launch(Handle* h, T* const* A, const int64_t* lda, int* const* perm,
   a.queue = base + 2 * p.count;
   a.ws = reinterpret_cast<T*>(p.ws);
   a.ws_stride = p.ws_stride;
-  a.lds_elems = batch_lds_elems(sizeof(T));
+  a.lds_elems = batch_lds_bytes(p.nt) / static_cast<int>(sizeof(T));
   a.widths = p.widths;
   hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, h->stream, a.queue, 1, 0);
-  hipLaunchKernelGGL(k_batched_sytrf<T>, dim3(p.grid), dim3(kBT), kBatchLdsBytes, h->stream, a);
+  if (p.nt == 64)
+    hipLaunchKernelGGL((k_batched_sytrf<T, 64>), dim3(p.grid), dim3(64), batch_lds_bytes(64),
+                       h->stream, a);
+  else
+    hipLaunchKernelGGL((k_batched_sytrf<T, 256>), dim3(p.grid), dim3(256), batch_lds_bytes(256),
+                       h->stream, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return h->hip_fail(e, "batched factorization launch");
   return BBK_OK;
@@ -475,8 +541,8 @@ int batched_gen(Handle* h, int count, const int* d_n, uint64_t seed, double scal
 // ---- autotuner -----------------------------------------------------------
 // For every size class up to max_n: a synthetic batch with sizes spread
 // over the class (uniform [-1, 1] entries, zero diagonal on every 7th
-// index to force 2x2 pivots), each candidate width timed with HIP events
-// (best of two), the fastest kept in the handle's table.
+// index to force 2x2 pivots), each candidate (panel width, team size) timed
+// with HIP events (best of two), the fastest kept in the handle's table.
 template <typename T>
 int batched_tune(Handle* h, int max_n, int* widths_out) {
   static const int cand[] = {8, 16, 24, 32, 48, 64};
@@ -490,9 +556,11 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
   for (int c = 0; c < kBatchClasses && rc == BBK_OK; ++c) {
     const int hi = std::min(batch_class_bound(c), std::max(max_n, 1));
     if (lo > hi) break;
-    // about 2^28 flops per trial batch, at least 32 and at most 2048 matrices
+    // enough matrices to keep every CU's teams busy for small classes
+    // (concurrency decides the team size there), about 2.5e10 element
+    // operations per trial batch for the large ones
     const double mean_n = 0.5 * (lo + hi);
-    int count = static_cast<int>(std::min(2048.0, std::max(32.0, 8.0e8 / (mean_n * mean_n * mean_n))));
+    int count = static_cast<int>(std::min(3072.0, std::max(64.0, 2.5e10 / (mean_n * mean_n * mean_n))));
     std::vector<int> ns(count);
     for (int i = 0; i < count; ++i) ns[i] = lo + static_cast<int>((static_cast<int64_t>(i) * 7919) % (hi - lo + 1));
     std::vector<int64_t> lds(count);
@@ -542,12 +610,15 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
     }
     rc = batched_gen<T>(h, count, dN, 0x7475u + c, 1.0, 7, reinterpret_cast<T* const*>(dS), dL);
     float best_ms = 1e30f;
-    int best_w = tab.w[c];
-    for (int w : cand) {
+    int best_w = tab.w[c], best_nt = tab.nt[c];
+    for (int trial = 0; trial < 2 * static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++trial) {
+      const int w = cand[trial >> 1];
+      const int nt = (trial & 1) ? 256 : 64;
       if (rc != BBK_OK) break;
       if (w > kBatchMaxWidth) continue;
       BatchWidths saved = tab;
       tab.w[c] = w;
+      tab.nt[c] = nt;
       float ms_min = 1e30f;
       rc = batched_prepare(h, count, ns.data(), eb);
       for (int rep = 0; rep < 2 && rc == BBK_OK; ++rep) {
@@ -567,9 +638,11 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
       if (ms_min < best_ms) {
         best_ms = ms_min;
         best_w = w;
+        best_nt = nt;
       }
     }
     tab.w[c] = best_w;
+    tab.nt[c] = best_nt;
     hipFree(mem);
     lo = hi + 1;
   }
@@ -664,6 +737,29 @@ int bbk_batched_set_widths(bbk_handle_t handle, int elem_bytes, const int* width
   Handle* h = reinterpret_cast<Handle*>(handle);
   bbk::BatchWidths& t = h->batch_widths[elem_bytes == 8 ? 0 : 1];
   for (int c = 0; c < bbk::kBatchClasses; ++c) t.w[c] = widths[c];
+  h->batch.count = -1;
+  return BBK_OK;
+}
+
+int bbk_batched_get_team_sizes(bbk_handle_t handle, int elem_bytes, int* team_sizes) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (!team_sizes) return -3;
+  const bbk::BatchWidths& t =
+      reinterpret_cast<Handle*>(handle)->batch_widths[elem_bytes == 8 ? 0 : 1];
+  for (int c = 0; c < bbk::kBatchClasses; ++c) team_sizes[c] = t.nt[c];
+  return BBK_OK;
+}
+
+int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes, const int* team_sizes) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (!team_sizes) return -3;
+  for (int c = 0; c < bbk::kBatchClasses; ++c)
+    if (team_sizes[c] != 64 && team_sizes[c] != 256) return -3;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  bbk::BatchWidths& t = h->batch_widths[elem_bytes == 8 ? 0 : 1];
+  for (int c = 0; c < bbk::kBatchClasses; ++c) t.nt[c] = team_sizes[c];
   h->batch.count = -1;
   return BBK_OK;
 }

@@ -10,24 +10,33 @@
 
 namespace bbk {
 
-constexpr int kBatchThreads = 256;
 constexpr int kBatchClasses = 8;    // n <= 32, 64, 128, 256, 512, 1024, 2048, larger
 constexpr int kBatchMaxWidth = 64;  // widest panel a batched matrix may use
-// Dynamic LDS per workgroup for the panel rows; two workgroups fit a CU.
+// Dynamic LDS per workgroup for the panel rows: six one-wavefront teams or
+// two four-wavefront teams fit a CU's 160 KB.
+constexpr int kBatchLdsBytesSmall = 24 * 1024;
 constexpr int kBatchLdsBytes = 76 * 1024;
+__host__ __device__ constexpr int batch_lds_bytes(int nt) {
+  return nt == 64 ? kBatchLdsBytesSmall : kBatchLdsBytes;
+}
 
 // Upper size bound of class c (the last class is unbounded).
 __host__ __device__ constexpr int batch_class_bound(int c) {
   return c < kBatchClasses - 1 ? (32 << c) : 0x7fffffff;
 }
 
+// Per size class: panel width w and team size nt (threads per matrix, 64
+// or 256).  A batch runs with the team size of its largest matrix's class.
 struct BatchWidths {
   int w[kBatchClasses];
+  int nt[kBatchClasses];
 };
 
-// Default table (used until bbk_batched_tune or bbk_batched_set_widths).
-constexpr BatchWidths kDefaultBatchWidthsF64 = {{8, 16, 16, 32, 32, 32, 32, 32}};
-constexpr BatchWidths kDefaultBatchWidthsF32 = {{8, 16, 16, 32, 32, 32, 32, 32}};
+// Default table (used until bbk_batched_tune or bbk_batched_set_*).
+constexpr BatchWidths kDefaultBatchWidthsF64 = {{8, 16, 16, 32, 32, 32, 32, 32},
+                                                {256, 256, 256, 256, 256, 256, 256, 256}};
+constexpr BatchWidths kDefaultBatchWidthsF32 = {{8, 16, 16, 32, 32, 32, 32, 32},
+                                                {256, 256, 256, 256, 256, 256, 256, 256}};
 
 // The schedule of the last prepared batch: queue order, sizes, counter.
 struct BatchPlan {
@@ -41,6 +50,7 @@ struct BatchPlan {
   size_t ws_bytes = 0;
   int64_t ws_stride = 0;
   int grid = 0;
+  int nt = 64;          // threads per matrix team
 };
 
 }  // namespace bbk

@@ -145,9 +145,16 @@ int bbk_batched_prepare(bbk_handle_t handle, int count, const int* h_n,
 int bbk_batched_get_widths(bbk_handle_t handle, int elem_bytes, int* widths);
 int bbk_batched_set_widths(bbk_handle_t handle, int elem_bytes,
                            const int* widths);
-/* Autotuner: for every size class up to max_n, times candidate panel
- * widths on a synthetic batch of that class and keeps the fastest in the
- * handle's table (written to widths_out when not NULL).  Synchronous. */
+/* Team size per class (threads per matrix: 64 or 256); a batch runs with
+ * the team size of the class of its largest matrix. */
+int bbk_batched_get_team_sizes(bbk_handle_t handle, int elem_bytes,
+                               int* team_sizes);
+int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes,
+                               const int* team_sizes);
+/* Autotuner: for every size class up to max_n, times candidate (panel
+ * width, team size) pairs on a synthetic batch of that class and keeps the
+ * fastest in the handle's table (widths written to widths_out when not
+ * NULL).  Synchronous. */
 int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n,
                      int* widths_out);
 

@@ -80,12 +80,14 @@ def test_batched_fp64_matches_oracle(oracle, handle):
         check(oracle, A, g, width_for(widths, A.shape[0]))
 
 
-@pytest.mark.parametrize("bw", [1, 2, 3, 5])
-def test_batched_small_widths_defer_2x2(oracle, handle, bw):
+@pytest.mark.parametrize("bw,nt", [(1, 256), (2, 64), (3, 256), (5, 64)])
+def test_batched_small_widths_defer_2x2(oracle, handle, bw, nt):
     """Narrow panels make 2x2 pivots straddle panel edges often; the
-    reference defers them (the next panel starts one column early)."""
+    reference defers them (the next panel starts one column early).  Both
+    team sizes (one wavefront / four per matrix)."""
     import torch
     handle.set_batched_widths(torch.float64, [bw] * 8)
+    handle.set_batched_team_sizes(torch.float64, [nt] * 8)
     try:
         ns = [9, 16, 31, 40, 57, 64]
         mats = []
@@ -105,6 +107,7 @@ def test_batched_small_widths_defer_2x2(oracle, handle, bw):
         assert any((g.piv == 2).any() for g in outs)
     finally:
         handle.set_batched_widths(torch.float64, [8, 16, 16, 32, 32, 32, 32, 32])
+        handle.set_batched_team_sizes(torch.float64, [256] * 8)
 
 
 def test_batched_global_workspace_path(oracle, handle):
@@ -120,8 +123,10 @@ def test_batched_global_workspace_path(oracle, handle):
         check(oracle, A, g, width_for(widths, A.shape[0]))
 
 
-def test_batched_fp32(oracle, handle):
+@pytest.mark.parametrize("nt", [64, 256])
+def test_batched_fp32(oracle, handle, nt):
     import torch
+    handle.set_batched_team_sizes(torch.float32, [nt] * 8)
     ns = [20, 64, 100, 180, 256, 400]
     mats = handle.gen_synth_batched(ns, seed=31337, scale=1.0, zero_diag_stride=7,
                                     dtype=torch.float32)
@@ -193,10 +198,12 @@ def test_batched_graph_replay(oracle, handle):
 def test_batched_autotuner(oracle, handle):
     import torch
     saved = handle.batched_widths(torch.float64)
+    saved_nt = handle.batched_team_sizes(torch.float64)
     try:
         widths = handle.batched_tune(torch.float64, 128)
         assert all(w in (8, 16, 24, 32, 48, 64) for w in widths[:3])
         assert widths == handle.batched_widths(torch.float64)
+        assert all(t in (64, 256) for t in handle.batched_team_sizes(torch.float64))
         ns = [30, 60, 120]
         mats = handle.gen_synth_batched(ns, seed=8, zero_diag_stride=7, dtype=torch.float64)
         hosts = [host_matrix(m) for m in mats]
@@ -205,3 +212,4 @@ def test_batched_autotuner(oracle, handle):
             check(oracle, A, g, width_for(widths, A.shape[0]))
     finally:
         handle.set_batched_widths(torch.float64, saved)
+        handle.set_batched_team_sizes(torch.float64, saved_nt)
