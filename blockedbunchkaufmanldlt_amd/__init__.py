@@ -71,6 +71,8 @@ def lib() -> ctypes.CDLL:
     L.bbk_batched_get_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_set_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_tune.argtypes = [vp, i32, i32, ip]
+    L.bbk_dsysv_mixed_batched.argtypes = [vp, i32, ip, i32, vp, vp, vp, vp, vp, vp, f64, i32,
+                                          vp, vp]
     L.bbk_batched_get_team_sizes.argtypes = [vp, i32, ip]
     L.bbk_batched_set_team_sizes.argtypes = [vp, i32, ip]
     for name in ("bbk_graph_create_dsytrf_batched", "bbk_graph_create_ssytrf_batched"):
@@ -274,6 +276,33 @@ class Handle:
         self.check(fn(self.h, count, ns, *[_ptr(a) for a in arrs], ctypes.byref(g)),
                    "graph_sytrf_batched")
         return BatchGraph(g, arrs, self)
+
+    def sysv_mixed_batched(self, mats: Sequence, Bs: Sequence, Xs: Sequence,
+                           tol: float = 1e-12, max_iter: int = 30):
+        """Batched FP32 factorization + FP64 refinement of A_m X_m = B_m
+        (float64 (n, lda) matrices, upper triangle read, not modified;
+        B_m, X_m float64 (n, nrhs)).  Returns a dict with per-matrix
+        iterations and backward errors (numpy arrays) and `converged`."""
+        torch = self._torch
+        count = len(mats)
+        dev = mats[0].device
+        nrhs = Bs[0].shape[1]
+        ns = (ctypes.c_int * count)(*[m.shape[0] for m in mats])
+        def arr(vals):
+            return torch.tensor(vals, dtype=torch.int64, device=dev)
+        keep = [arr([m.data_ptr() for m in mats]), arr([m.stride(0) for m in mats]),
+                arr([b.data_ptr() for b in Bs]), arr([b.stride(0) for b in Bs]),
+                arr([x.data_ptr() for x in Xs]), arr([x.stride(0) for x in Xs])]
+        iters = torch.zeros(count, dtype=torch.int32, device=dev)
+        berr = torch.zeros(count, dtype=torch.float64, device=dev)
+        with self.ordered():
+            rc = lib().bbk_dsysv_mixed_batched(self.h, count, ns, nrhs,
+                                               *[_ptr(k) for k in keep], tol, max_iter,
+                                               _ptr(iters), _ptr(berr))
+        if rc not in (0, BBK_ERR_NOT_CONVERGED):
+            self.check(rc, "sysv_mixed_batched")
+        return {"iters": iters.cpu().numpy(), "berr": berr.cpu().numpy(),
+                "converged": rc == 0}
 
     def batched_prepare(self, ns: Sequence[int], dtype):
         eb = 8 if dtype == self._torch.float64 else 4

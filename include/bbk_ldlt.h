@@ -173,6 +173,25 @@ int bbk_graph_create_ssytrf_batched(bbk_handle_t handle, int count,
 int bbk_graph_launch(bbk_graph_t graph);
 int bbk_graph_destroy(bbk_graph_t graph);
 
+/* Batched mixed-precision solve (new): every matrix m is factored in
+ * binary32 (bbk_ssytrf_batched on a binary32 copy of its upper triangle),
+ * then X_m is refined in binary64 until its normwise backward error
+ * max_q ||b_q - A x_q||_inf / (||A||_inf ||x_q||_inf + ||b_q||_inf) <= tol
+ * or max_iter corrections were applied.  One persistent workgroup per
+ * matrix runs the whole refinement loop.  nrhs: 1..4 right-hand sides per
+ * matrix, row-major n x nrhs (leading dimensions ldb, ldx >= nrhs).  d_A,
+ * d_B, d_X: device arrays of device pointers; d_lda, d_ldb, d_ldx: device
+ * arrays; d_iters / d_berr: device arrays of count entries receiving the
+ * corrections applied and the final backward error.  n * nrhs must be at
+ * most 9600 (the right-hand sides live in LDS).  Synchronous; returns
+ * BBK_ERR_NOT_CONVERGED when some matrix stayed above tol. */
+int bbk_dsysv_mixed_batched(bbk_handle_t handle, int count, const int* h_n,
+                            int nrhs, const double* const* d_A,
+                            const int64_t* d_lda, const double* const* d_B,
+                            const int64_t* d_ldb, double* const* d_X,
+                            const int64_t* d_ldx, double tol, int max_iter,
+                            int* d_iters, double* d_berr);
+
 /* Batched synthetic inputs: matrix m gets bbk_dgen_synth's formula with
  * seed + m (binary32 form: the same values rounded to binary32).  d_n:
  * device array of orders. */
