@@ -57,6 +57,124 @@ def c3_matrix(torch, n, seed, lda, kappa=1e6):
     return out, b
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "factor_fp64_n16384_pmc_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch per kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel)."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["traffic_bytes_per_launch"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return {}
+
+
+def householder_eig(torch, n, kappa, gen, reflectors=4):
+    """A = U diag(lambda) U^T with U a product of random Householder
+    reflectors (as LAPACK's dlagsy builds test matrices): exact eigenvalues
+    |lambda| log-spaced over [1/kappa, 1] with random signs."""
+    lam = torch.pow(torch.tensor(10.0, dtype=torch.float64, device="cuda"),
+                    -math.log10(kappa) * torch.arange(n, dtype=torch.float64, device="cuda")
+                    / max(n - 1, 1))
+    sgn = torch.randint(0, 2, (n,), generator=gen, device="cuda").to(torch.float64) * 2 - 1
+    A = torch.diag(lam * sgn)
+    for _ in range(reflectors):
+        v = torch.randn(n, generator=gen, dtype=torch.float64, device="cuda")
+        v = v / torch.linalg.norm(v)
+        Av = A @ v
+        A = A - 2 * torch.outer(v, Av) - 2 * torch.outer(Av, v) + 4 * torch.dot(v, Av) * torch.outer(v, v)
+    return 0.5 * (A + A.T)
+
+
+def bench_batched_small(torch, B, h, reps=5, seed=4096):
+    """4096 matrices, n uniform in [16, 256], entries uniform [-1, 1]: one
+    persistent launch replayed from a HIP graph, FP64 and FP32; panel
+    widths / team sizes from the autotuner (tuning time reported apart)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in rng.integers(16, 257, size=4096)]
+    flops = sum(n ** 3 / 3.0 for n in ns)
+    out = {"workload": "4096 matrices, n uniform in [16, 256], entries uniform [-1, 1]",
+           "flops": flops}
+    for dt, name, peak in ((torch.float64, "fp64", PEAK_FP64), (torch.float32, "fp32", PEAK_FP32)):
+        eb = 8 if dt == torch.float64 else 4
+        t0 = time.perf_counter()
+        h.batched_tune(dt, 256)
+        tune_s = time.perf_counter() - t0
+        src = h.gen_synth_batched(ns, seed=seed, dtype=dt)
+        work = [s.clone() for s in src]
+        perms = [torch.empty(n, dtype=torch.int32, device="cuda") for n in ns]
+        pivs = [torch.empty(n, dtype=torch.int32, device="cuda") for n in ns]
+        g = h.graph_sytrf_batched(work, perms, pivs)
+        times = []
+        for _ in range(reps):
+            for a, b in zip(work, src):
+                a.copy_(b)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(h.stream)
+            g.launch()
+            e1.record(h.stream)
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        g.close()
+        ms = min(times)
+        alg_bytes = 2.0 * sum(n * (n + 1) / 2.0 for n in ns) * eb
+        out[name] = {"ms": round(ms, 3), "GFLOP/s": round(flops / ms / 1e6, 1),
+                     "frac_matrix_peak": round(flops / (ms * 1e-3) / peak, 5),
+                     "hbm_GB/s": round(alg_bytes / ms / 1e6, 1),
+                     "frac_hbm": round(alg_bytes / (ms * 1e-3) / PEAK_HBM, 5),
+                     "widths": h.batched_widths(dt), "team_sizes": h.batched_team_sizes(dt),
+                     "tune_s": round(tune_s, 3)}
+        del src, work, perms, pivs
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_batched_mixed(torch, B, h, reps=2, seed=1024):
+    """1024 matrices, n uniform in [256, 2048], condition 10^U[4,7]
+    (Householder eigen-construction), 4 right-hand sides N(0, 1): binary32
+    batched factorization + binary64 refinement to backward error 1e-12."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in rng.integers(256, 2049, size=1024)]
+    kap = 10.0 ** rng.uniform(4, 7, size=1024)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    mats = []
+    for n, k in zip(ns, kap):
+        T = torch.empty((n, B.padded_ld(n)), dtype=torch.float64, device="cuda")
+        T[:, :n] = householder_eig(torch, n, k, gen)
+        mats.append(T)
+    Bs = [torch.randn((n, 4), generator=gen, dtype=torch.float64, device="cuda") for n in ns]
+    Xs = [torch.zeros((n, 4), dtype=torch.float64, device="cuda") for n in ns]
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        for x in Xs:
+            x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = h.sysv_mixed_batched(mats, Bs, Xs, tol=1e-12, max_iter=30)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if best is None or dt < best[0]:
+            best = (dt, info)
+    dt, info = best
+    flops = sum(n ** 3 / 3.0 for n in ns)
+    out = {"workload": "1024 matrices, n uniform in [256, 2048], cond 10^U[4,7], 4 RHS",
+           "time_ms": round(dt * 1e3, 3), "factor_GFLOP/s_equiv": round(flops / dt / 1e9, 1),
+           "converged": bool(info["converged"]), "max_berr": float(info["berr"].max()),
+           "iters_mean": round(float(info["iters"].mean()), 3),
+           "iters_max": int(info["iters"].max())}
+    del mats, Bs, Xs
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n_sample=4096, bs=128):
     """The reference's Factorize() (oracle/_ref, OpenMP, all host cores) on a
     bounded sample; falls back to the oracle restatement if the reference
@@ -92,6 +210,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="factorizations only (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -158,6 +277,12 @@ def main():
                   "unit": "GB/s", "frac": round(pan_achieved / (PEAK_HBM / 1e9), 5),
                   "traffic": None, "avg_launch_ms": round(pan_ms / max(panels_per_step, 1), 4),
                   "us_per_pivot": round(pan_ms * 1e3 / n, 3)}
+    traffic = measured_traffic() if n == 16384 else {}
+    for roof, key in ((roof_update, "bk_trailing_update_kernel<double>"),
+                      (roof_panel, "bk_panel_kernel<double>")):
+        if key in traffic:
+            roof["traffic"] = traffic[key]
+            roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
     dominant = roof_update if upd_ms >= pan_ms else roof_panel
 
     # ---- time to backward error 1e-12: FP64 vs mixed ----------------------
@@ -185,6 +310,12 @@ def main():
         except B.BbkError as e:
             log(f"solve path unavailable: {e}")
 
+    batched = None
+    if not args.no_batched and not args.profile:
+        log("batched workloads")
+        batched = {"small_fp64_fp32": bench_batched_small(torch, B, h),
+                   "mixed_refined": bench_batched_mixed(torch, B, h)}
+
     cpu = None
     if not args.no_cpu and not args.profile:
         log("cpu baseline")
@@ -211,6 +342,7 @@ def main():
         "roofline_panel": roof_panel,
         "fraction_of_fp64_peak": round(gflops * 1e9 / PEAK_FP64, 4),
         "time_to_1e-12": solve,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
