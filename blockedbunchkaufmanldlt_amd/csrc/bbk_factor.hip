@@ -18,8 +18,8 @@ namespace bbk {
 // Trailing update C(i,j) -= sum_q U(s+q, i) W(q, j) over the upper
 // triangle of [e, n)^2, e = s + width (UpdateTrailingBlocks +
 // kernels::UpdateTrailingBlock, src/block_ldlt_decomposition.cc:706-744,
-// src/kernels.cc:67-91).  128x128 output tile per workgroup, four
-// wavefronts of 64x64, 16x16x4 MFMA, K staged through LDS in chunks.
+// src/kernels.cc:67-91).  Square output tiles per workgroup, four
+// wavefronts, 16x16x4 MFMA, K staged through LDS in chunks.
 template <typename T>
 struct Mfma;
 template <>
@@ -56,15 +56,17 @@ __device__ __forceinline__ void tri_tile(int idx, int* I, int* J) {
 // The accumulators start from the C tile itself and the U operand is
 // negated while it is staged, so the MFMA chain computes C - U^T W in
 // place; K is staged through two LDS buffers so the next chunk's global
-// loads overlap the current chunk's MFMAs.
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void bk_trailing_update_kernel(
+// loads overlap the current chunk's MFMAs.  TM x TM output tile per
+// workgroup, four wavefronts of (TM/2) x (TM/2); MINB workgroups per CU;
+// with XCD, consecutive tiles (sharing U and W strips) go to one XCD.
+template <typename T, int TM, int MINB, int KC = 16, bool XCD = false>
+__global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
     T* __restrict__ A, int64_t lda, const T* __restrict__ Wg, int64_t ldw,
     const int* __restrict__ state, int n) {
-  constexpr int TM = 128;
-  constexpr int KC = 16;
   constexpr int LDS_LD = TM + 4;
   constexpr int PER = KC * TM / 256;  // staged elements per thread per operand
+  constexpr int WT = TM / 2;          // wavefront tile
+  constexpr int TI = WT / 16;         // 16 x 16 MFMA blocks per wavefront side
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T Us[2][KC * LDS_LD];
   __shared__ T Ws[2][KC * LDS_LD];
@@ -75,23 +77,31 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void bk_trailing_updat
   const int m = n - e;
   if (m <= 0 || bw <= 0) return;
   const int tiles = (m + TM - 1) / TM;
+  int idx = blockIdx.x;
+  if (XCD) {
+    // workgroups are dealt to the 8 XCDs round-robin; give XCD x a
+    // contiguous run of tiles (neighbouring W and U strips share its L2)
+    const int nb = gridDim.x, per = (nb + 7) / 8, x = idx & 7, r = idx >> 3;
+    const int full = nb - 8 * (per - 1);  // XCDs that get `per` tiles
+    idx = x < full ? x * per + r : full * per + (x - full) * (per - 1) + r;
+  }
   int I, J;
-  tri_tile(blockIdx.x, &I, &J);
+  tri_tile(idx, &I, &J);
   if (J >= tiles) return;
   const int i0 = e + I * TM, j0 = e + J * TM;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
 
-  acc_t acc[4][4];
+  acc_t acc[TI][TI];
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+    for (int tj = 0; tj < TI; ++tj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * 64 + ti * 16 + Mfma<T>::row(lane, v);
-        const int jj = j0 + wc * 64 + tj * 16 + (lane & 15);
+        const int i = i0 + wr * WT + ti * 16 + Mfma<T>::row(lane, v);
+        const int jj = j0 + wc * WT + tj * 16 + (lane & 15);
         acc[ti][tj][v] = (i < n && jj < n && jj >= i) ? A[(int64_t)i * lda + jj] : T(0);
       }
 
@@ -125,33 +135,45 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void bk_trailing_updat
     if (kc + 1 < nk) fetch((kc + 1) * KC);
 #pragma unroll
     for (int k4 = 0; k4 < KC; k4 += 4) {
-      T a[4], b[4];
+      T a[TI], b[TI];
       const int kr = k4 + (lane >> 4);
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        a[t] = Us[buf][kr * LDS_LD + wr * 64 + t * 16 + (lane & 15)];
-        b[t] = Ws[buf][kr * LDS_LD + wc * 64 + t * 16 + (lane & 15)];
+      for (int t = 0; t < TI; ++t) {
+        a[t] = Us[buf][kr * LDS_LD + wr * WT + t * 16 + (lane & 15)];
+        b[t] = Ws[buf][kr * LDS_LD + wc * WT + t * 16 + (lane & 15)];
       }
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
+      for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj)
+        for (int tj = 0; tj < TI; ++tj)
           acc[ti][tj] = Mfma<T>::mma(a[ti], b[tj], acc[ti][tj]);
     }
     if (kc + 1 < nk) stash(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+    for (int tj = 0; tj < TI; ++tj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * 64 + ti * 16 + Mfma<T>::row(lane, v);
-        const int jj = j0 + wc * 64 + tj * 16 + (lane & 15);
+        const int i = i0 + wr * WT + ti * 16 + Mfma<T>::row(lane, v);
+        const int jj = j0 + wc * WT + tj * 16 + (lane & 15);
         if (i < n && jj < n && jj >= i) A[(int64_t)i * lda + jj] = acc[ti][tj][v];
       }
 }
+
+// Tile shape (measured on MI355X at m = 16384, K = 64 / 128): 64 x 64
+// tiles with four workgroups resident per CU let one workgroup's C traffic
+// overlap another's MFMAs (binary64 15.7 -> 24.7 TFLOP/s, binary32 40 ->
+// 55 TFLOP/s against 128 x 128 tiles); K chunks of 16; XCD-contiguous tile
+// order.
+template <typename T>
+struct UpdateCfg {
+  static constexpr int kTM = 64;
+  static constexpr int kMinBlocks = 2;
+  static constexpr int kKC = 16;
+};
 
 // ---------------------------------------------------------------------
 // Deferred interchanges of the rows above the panel: the panel's
@@ -242,10 +264,12 @@ void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                             const int* state, int n, int m_bound,
                             hipStream_t stream) {
   if (m_bound <= 0) return;
-  const int tiles = (m_bound + 127) / 128;
+  constexpr int TM = UpdateCfg<T>::kTM;
+  const int tiles = (m_bound + TM - 1) / TM;
   const int blocks = tiles * (tiles + 1) / 2;
-  hipLaunchKernelGGL(bk_trailing_update_kernel<T>, dim3(blocks), dim3(256), 0,
-                     stream, A, lda, Wg, ldw, state, n);
+  hipLaunchKernelGGL((bk_trailing_update_kernel<T, TM, UpdateCfg<T>::kMinBlocks,
+                                                 UpdateCfg<T>::kKC, true>),
+                     dim3(blocks), dim3(256), 0, stream, A, lda, Wg, ldw, state, n);
 }
 
 template <typename T>
