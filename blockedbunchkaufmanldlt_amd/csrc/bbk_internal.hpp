@@ -75,6 +75,14 @@ __device__ __forceinline__ bool xget(const XRec* p, int tag, int* aux, double* v
   return true;
 }
 
+// Poll one record until it carries `tag` (the writer is a co-resident
+// workgroup of the same cooperative launch, so the wait ends).
+__device__ __forceinline__ double xwait(const XRec* p, int tag, int* aux) {
+  double v;
+  while (!xget(p, tag, aux, &v)) __builtin_amdgcn_s_sleep(1);
+  return v;
+}
+
 // Wait until this wave's outstanding vector memory operations completed.
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
@@ -153,6 +161,39 @@ __device__ __forceinline__ void block_argmax(double& v, int& i, double* sv,
   i = si[0];
   for (int t = 1; t < nw; ++t) argmax_combine(v, i, sv[t], si[t]);
   __syncthreads();
+}
+
+// One-barrier forms: the scratch alternates between two halves (sv, si
+// hold 2 * blockDim.x / 64 entries; `parity` flips on every call), so a
+// later reduction never overwrites values an earlier one may still be
+// reading -- the barrier of the reduction in between orders them.
+__device__ __forceinline__ void block_argmax1(double& v, int& i, double* sv, int* si,
+                                              int& parity) {
+  wave_argmax(v, i);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* svp = sv + parity * nw;
+  int* sip = si + parity * nw;
+  if ((threadIdx.x & 63) == 0) {
+    svp[w] = v;
+    sip[w] = i;
+  }
+  __syncthreads();
+  v = svp[0];
+  i = sip[0];
+  for (int t = 1; t < nw; ++t) argmax_combine(v, i, svp[t], sip[t]);
+  parity ^= 1;
+}
+
+__device__ __forceinline__ double block_max1(double v, double* sv, int& parity) {
+  v = wave_max_nan_ignoring(v);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* svp = sv + parity * nw;
+  if ((threadIdx.x & 63) == 0) svp[w] = v;
+  __syncthreads();
+  v = svp[0];
+  for (int t = 1; t < nw; ++t) v = (v < svp[t]) ? svp[t] : v;
+  parity ^= 1;
+  return v;
 }
 
 __device__ __forceinline__ double block_max_nan_ignoring(double v,

@@ -85,22 +85,35 @@ enum PubId {
   kNumPub = 16
 };
 
+// L(r, s + q) from W(q, r) (the candidate's column) and row q's scaling
+// record; `partner` is W(q + 1, r) for a 2x2 start, W(q - 1, r) for its
+// second row.  Same operations as ucol().
+__device__ __forceinline__ double ucol_pair(int ty, double c0, double c1, double self,
+                                            double partner) {
+  if (ty == 1) return div_or_zero(self, c0);
+  double u0, u1;
+  if (ty == 2) {
+    apply_2x2_nc(c0, 0.0, c1, self, partner, &u0, &u1);
+    return u0;
+  }
+  apply_2x2_nc(0.0, c1, c0, partner, self, &u0, &u1);
+  return u1;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__ A, PanelArgs p) {
   constexpr int NT = kPanelThreads;
   constexpr int BM = PanelCfg<T>::kMaxWidth;
   __shared__ T Wl[(BM + 1) * NT];
-  __shared__ T Lk[BM + 2];
-  __shared__ T Lnext[BM + 2];
+  __shared__ T Lbuf[2][BM + 2];  // L(k, panel) of this pivot / of the next
   __shared__ T Lr[BM + 2];
   __shared__ double colR[BM + 2], colK[BM + 2], colK1[BM + 2], colK2[BM + 2];
   __shared__ int dt[BM + 2];
   __shared__ double dc0[BM + 2], dc1[BM + 2];
   __shared__ double xval[kMaxPanelGroupsLds];
-  __shared__ int xidx[kMaxPanelGroupsLds];
   __shared__ double pubv[kNumPub];
-  __shared__ double red_v[NT / 64];
-  __shared__ int red_i[NT / 64];
+  __shared__ double red_v[2 * (NT / 64)];
+  __shared__ int red_i[2 * (NT / 64)];
 
   const int n = p.n;
   const int64_t lda = p.lda;
@@ -124,6 +137,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   const int PS = G + kNumPub + 3 * BM + G * BM;
   XRec* const xbase = p.xrec;
   int ex = p.epoch;  // exchange number = record tag
+  int par = 0;       // reduction scratch half
+  int lp = 0;        // Lbuf index of the current pivot
 
   auto area = [&](int tag) -> XRec* { return xbase + (tag & 1) * PS; };
   auto is_mine = [&](int col) { return col >= c0 && col < c0 + NT && col < n; };
@@ -148,6 +163,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     BBK_PT(0);
     const int k = s + kk;
     const bool has1 = k + 1 < n, has2 = k + 2 < n;
+    T* const Lk = Lbuf[lp];
+    T* const Lnext = Lbuf[lp ^ 1];
     // ---- phase A: W(k, j) for own j >= k, local argmax over j > k -------
     const bool active = own && j >= k;
     T w = T(0);
@@ -164,13 +181,14 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         vi = j;
       }
     }
-    block_argmax(v, vi, red_v, red_i);  // ends with a workgroup barrier
+    // this thread's earlier (coherent) A writes have completed; the barrier
+    // of the reduction then orders every thread's before the records below
+    wait_vm();
+    block_argmax1(v, vi, red_v, red_i, par);
     BBK_PT(1);
     // ---- exchange 1: post --------------------------------------------------
     ++ex;
     {
-      wait_vm();  // this group's earlier (coherent) A writes have completed
-      __syncthreads();
       XRec* ar = area(ex);
       // slot: local argmax column and its signed value W(k, r_g)
       if (tid == 0)
@@ -200,55 +218,38 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
     }
     BBK_PT(2);
-    // ---- exchange 1: gather ------------------------------------------------
+    // ---- exchange 1: gather (each thread waits for its own records) --------
+    double omega = 0.0;
+    int r = INT_MAX;
     {
       const XRec* ar = area(ex);
       const int npub = 2 + (has1 ? 2 : 0) + (has2 ? 1 : 0);
       const int ncol = kk * (1 + (has1 ? 1 : 0) + (has2 ? 1 : 0));
       const int total = G + npub + ncol;
-      int ok;
-      do {
-        ok = 1;
-        for (int id = tid; id < total; id += NT) {
-          int aux;
-          double val;
-          if (id < G) {
-            if (xget(ar + id, ex, &aux, &val)) {
-              xval[id] = val;
-              xidx[id] = aux;
-            } else {
-              ok = 0;
-            }
-          } else if (id < G + npub) {
-            const int u = id - G;
-            const int pid = u < 2 ? u : (has1 && u < 4 ? u : kPubWk2);
-            if (xget(ar + G + pid, ex, &aux, &val))
-              pubv[pid] = val;
-            else
-              ok = 0;
-          } else {
-            const int u = id - G - npub;
-            const int which = u / kk, q = u % kk;  // 0: k, 1: k+1, 2: k+2
-            if (xget(ar + G + kNumPub + which * BM + q, ex, &aux, &val)) {
-              if (which == 0)
-                colK[q] = val;
-              else if (which == 1)
-                colK1[q] = val;
-              else
-                colK2[q] = val;
-            } else {
-              ok = 0;
-            }
-          }
+      for (int id = tid; id < total; id += NT) {
+        int aux;
+        if (id < G) {
+          const double val = xwait(ar + id, ex, &aux);
+          xval[id] = val;
+          argmax_combine(omega, r, fabs(val), aux);
+        } else if (id < G + npub) {
+          const int u = id - G;
+          const int pid = u < 2 ? u : (has1 && u < 4 ? u : kPubWk2);
+          pubv[pid] = xwait(ar + G + pid, ex, &aux);
+        } else {
+          const int u = id - G - npub;
+          const int which = u / kk, q = u % kk;  // 0: k, 1: k+1, 2: k+2
+          const double val = xwait(ar + G + kNumPub + which * BM + q, ex, &aux);
+          if (which == 0)
+            colK[q] = val;
+          else if (which == 1)
+            colK1[q] = val;
+          else
+            colK2[q] = val;
         }
-        ok = __syncthreads_and(ok);
-        if (!ok) __builtin_amdgcn_s_sleep(1);
-      } while (!ok);
+      }
+      block_argmax1(omega, r, red_v, red_i, par);  // also publishes xval/pubv/colK*
     }
-    double omega = 0.0;
-    int r = INT_MAX;
-    for (int g2 = tid; g2 < G; g2 += NT) argmax_combine(omega, r, fabs(xval[g2]), xidx[g2]);
-    block_argmax(omega, r, red_v, red_i);
     // W(k, r) with its sign, from the slot of r's owner
     const double wkr = (r != INT_MAX) ? xval[(r - s) / NT] : 0.0;
     const double wkk = pubv[kPubWk];
@@ -257,7 +258,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     // prefetch rows k+1, k+2 of A for the next pivot (patched after the swap)
     const T P1 = (has1 && own && j >= k + 1) ? ldc(&A[(int64_t)(k + 1) * lda + j]) : T(0);
     const T P2 = (has2 && own && j >= k + 2) ? ldc(&A[(int64_t)(k + 2) * lda + j]) : T(0);
-
     int kind;
     T c = T(0);
     double crr = 0.0, ck = 0.0, ck1 = 0.0, ck2 = 0.0, a11 = 0.0, a12 = 0.0;
@@ -269,22 +269,20 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       const int gstar = (r - s) / NT;
       T a = T(0);
       if (active) a = (j < r) ? ldc(&A[(int64_t)j * lda + r]) : ldc(&A[(int64_t)r * lda + j]);
-      {
-        int ok;
-        do {
-          ok = 1;
-          if (tid < kk) {
-            int aux;
-            double val;
-            if (xget(ar1 + G + kNumPub + 3 * BM + gstar * BM + tid, ex, &aux, &val))
-              colR[tid] = val;
-            else
-              ok = 0;
-          }
-          ok = __syncthreads_and(ok);
-        } while (!ok);
+      if (tid < kk) {
+        // W(q, r) from r's owner; L(r, s+q) needs the 2x2 partner's value too
+        const XRec* cr = ar1 + G + kNumPub + 3 * BM + gstar * BM;
+        int aux;
+        const double cq = xwait(cr + tid, ex, &aux);
+        const int ty = dt[tid];
+        double partner = 0.0;
+        if (ty == 2)
+          partner = xwait(cr + tid + 1, ex, &aux);
+        else if (ty == -2)
+          partner = xwait(cr + tid - 1, ex, &aux);
+        colR[tid] = cq;
+        Lr[tid] = static_cast<T>(ucol_pair(ty, dc0[tid], dc1[tid], cq, partner));
       }
-      if (tid < kk) Lr[tid] = static_cast<T>(ucol(dt, dc0, dc1, colR, tid));
       __syncthreads();
       if (active) {
         c = a - panel_dot<T, NT>(Lr, Wl, tid, kk);
@@ -294,7 +292,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         const double aa = fabs(static_cast<double>(c));
         sv = (aa >= 0.0) ? aa : 0.0;  // NaN ignored, as std::max does
       }
-      sv = block_max_nan_ignoring(sv, red_v);
+      sv = block_max1(sv, red_v, par);
       BBK_PT(4);
       // ---- exchange 2 ------------------------------------------------------
       // Rows k+1/k+2 prefetches may still be in flight: the elements other
@@ -316,34 +314,18 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
       const int npub = 2 + (has1 ? 2 : 0) + (has2 ? 2 : 0);
       const int total = G + npub;
-      int ok;
-      do {
-        ok = 1;
-        for (int id = tid; id < total; id += NT) {
-          int aux;
-          double val;
-          if (id < G) {
-            if (xget(ar + id, ex, &aux, &val))
-              xval[id] = val;
-            else
-              ok = 0;
-          } else {
-            const int pid = kPubCr + (id - G);  // 8, 9, (10, 11), (12, 13)
-            if (xget(pubs + pid, ex, &aux, &val))
-              pubv[pid] = val;
-            else
-              ok = 0;
-          }
+      double sg = 0.0;
+      for (int id = tid; id < total; id += NT) {
+        int aux;
+        if (id < G) {
+          const double o = xwait(ar + id, ex, &aux);
+          sg = (sg < o) ? o : sg;
+        } else {
+          const int pid = kPubCr + (id - G);  // 8, 9, (10, 11), (12, 13)
+          pubv[pid] = xwait(pubs + pid, ex, &aux);
         }
-        ok = __syncthreads_and(ok);
-        if (!ok) __builtin_amdgcn_s_sleep(1);
-      } while (!ok);
-      double sigma = 0.0;
-      for (int g2 = tid; g2 < G; g2 += NT) {
-        const double o = xval[g2];
-        sigma = (sigma < o) ? o : sigma;
       }
-      sigma = block_max_nan_ignoring(sigma, red_v);
+      const double sigma = block_max1(sg, red_v, par);  // also publishes pubv
       crr = pubv[kPubCr];
       ck = pubv[kPubCk];
       if (has1) {
@@ -531,10 +513,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
       size = 2;
     }
-    __syncthreads();
-    if (tid <= kk + 1) Lk[tid] = Lnext[tid];
+    __syncthreads();  // Lnext, dt/dc, Wl of this pivot complete
+    lp ^= 1;
     arow = arow_next;
-    __syncthreads();
     kk += size;
 #if defined(BBK_PANEL_PROFILE)
     BBK_PT(6);
