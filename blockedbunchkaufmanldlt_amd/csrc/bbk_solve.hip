@@ -72,6 +72,67 @@ __device__ __forceinline__ double poll_rows(const XRec* rec, int row0, int nrows
   return v;
 }
 
+// UB blocks at once: out[u] holds row row0[u] + (lane >> 2), rhs (lane & 3)
+// of block u (0 for q >= NR or rows >= nlim).  All UB loads of a lane are
+// in flight together and waited for in the same asm block.
+template <int UB>
+__device__ __forceinline__ void xload_multi(const XRec* const (&p)[UB], v4i (&d)[UB]);
+template <>
+__device__ __forceinline__ void xload_multi<2>(const XRec* const (&p)[2], v4i (&d)[2]) {
+  asm volatile(
+      "global_load_dwordx4 %0, %2, off sc1\n\t"
+      "global_load_dwordx4 %1, %3, off sc1\n\t"
+      "s_waitcnt vmcnt(0)"
+      : "=&v"(d[0]), "=&v"(d[1])
+      : "v"(p[0]), "v"(p[1])
+      : "memory");
+}
+template <>
+__device__ __forceinline__ void xload_multi<4>(const XRec* const (&p)[4], v4i (&d)[4]) {
+  asm volatile(
+      "global_load_dwordx4 %0, %4, off sc1\n\t"
+      "global_load_dwordx4 %1, %5, off sc1\n\t"
+      "global_load_dwordx4 %2, %6, off sc1\n\t"
+      "global_load_dwordx4 %3, %7, off sc1\n\t"
+      "s_waitcnt vmcnt(0)"
+      : "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]), "=&v"(d[3])
+      : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3])
+      : "memory");
+}
+
+template <int NR, int UB>
+__device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row0)[UB], int nlim,
+                                                int tag, double (&out)[UB]) {
+  const int lane = threadIdx.x & 63;
+  const int q = lane & 3, rr = lane >> 2;
+  const int qq = q < NR ? q : NR - 1;
+  const XRec* p[UB];
+  bool use[UB];
+#pragma unroll
+  for (int u = 0; u < UB; ++u) {
+    const int row = row0[u] + rr;
+    use[u] = q < NR && row < nlim;
+    p[u] = rec + (int64_t)(row < nlim ? row : nlim - 1) * kSolveMaxRhs + qq;
+    out[u] = 0.0;
+  }
+  for (;;) {
+    v4i d[UB];
+    xload_multi<UB>(p, d);
+    bool done = true;
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      if (!use[u]) continue;
+      if (d[u].x == tag)
+        out[u] = __longlong_as_double((static_cast<long long>(d[u].w) << 32) |
+                                      static_cast<unsigned int>(d[u].z));
+      else
+        done = false;
+    }
+    if (__ballot(!done) == 0) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 __device__ __forceinline__ double bcast(double v, int lane) { return readlane_d(v, lane); }
 
 // 16 consecutive rows i0 .. i0+15 of column j (forward: L(j, i) = F(i, j)).
@@ -129,21 +190,30 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     T acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = T(0);
-    T fc[16], fn[16];
-    if (J > 0) load_col16(fc, F, lda, w * 16, j, jok);
-    for (int I = 0; I < J; ++I) {
-      const int i0 = I * NB + w * 16;
-      if (I + 1 < J) load_col16(fn, F, lda, i0 + NB, j, jok);
-      // row i0 + 15 = j - 1 with j = j0: 2x2 coupling across the block edge
-      if (I == J - 1 && w == 3 && lane == 0 && piv[j0 - 1] == 2) fc[15] = T(0);
-      const double zv = poll_rows<NR>(rec, i0, 16, tag);
+    // UB earlier blocks per step: their F tiles and their records are in
+    // flight together (one wait per step)
+    constexpr int UB = sizeof(T) == 8 ? 2 : 4;
+    for (int I = 0; I < J; I += UB) {
+      T fc[UB][16];
+      int row0[UB];
 #pragma unroll
-      for (int ii = 0; ii < 16; ++ii) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] += fc[ii] * static_cast<T>(bcast(zv, ii * 4 + q));
+      for (int u = 0; u < UB; ++u) {
+        const int Iu = I + u;
+        load_col16(fc[u], F, lda, Iu * NB + w * 16, j, jok && Iu < J);
+        // row j0 - 1 with j = j0: 2x2 coupling across the block edge
+        if (Iu == J - 1 && w == 3 && lane == 0 && piv[j0 - 1] == 2) fc[u][15] = T(0);
+        row0[u] = min(Iu, J - 1) * NB + w * 16;  // past J - 1: fc is 0
       }
+      double zv[UB];
+      poll_rows_multi<NR, UB>(rec, row0, n, tag, zv);
 #pragma unroll
-      for (int ii = 0; ii < 16; ++ii) fc[ii] = fn[ii];
+      for (int u = 0; u < UB; ++u)
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            acc[q] += fc[u][ii] * static_cast<T>(bcast(zv[u], ii * 4 + q));
+        }
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) part[w][lane][q] = acc[q];
@@ -232,21 +302,29 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     T acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = T(0);
-    T fc[16], fn[16];
-    if (I + 1 < nblk) load_row16(fc, F, lda, i, (nblk - 1) * NB + w * 16, n, iok);
-    for (int J = nblk - 1; J > I; --J) {
-      const int c0 = J * NB + w * 16;
-      if (J - 1 > I) load_row16(fn, F, lda, i, c0 - NB, n, iok);
-      // column c0 = i + 1 with i = i0 + 63: 2x2 coupling across the block edge
-      if (J == I + 1 && w == 0 && lane == 63 && piv[i] == 2) fc[0] = T(0);
-      const double vv = poll_rows<NR>(rec, c0, min(16, n - c0), tag);
+    constexpr int UB = sizeof(T) == 8 ? 2 : 4;
+    for (int J = nblk - 1; J > I; J -= UB) {
+      T fc[UB][16];
+      int row0[UB];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] += fc[c] * static_cast<T>(bcast(vv, c * 4 + q));
+      for (int u = 0; u < UB; ++u) {
+        const int Ju = J - u;
+        const int c0 = Ju * NB + w * 16;
+        load_row16(fc[u], F, lda, i, c0, n, iok && Ju > I);
+        // column i + 1 with i = i0 + 63: 2x2 coupling across the block edge
+        if (Ju == I + 1 && w == 0 && lane == 63 && piv[i] == 2) fc[u][0] = T(0);
+        row0[u] = max(Ju, I + 1) * NB + w * 16;  // at or below I: fc is 0
       }
+      double vv[UB];
+      poll_rows_multi<NR, UB>(rec, row0, n, tag, vv);
 #pragma unroll
-      for (int c = 0; c < 16; ++c) fc[c] = fn[c];
+      for (int u = 0; u < UB; ++u)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            acc[q] += fc[u][c] * static_cast<T>(bcast(vv[u], c * 4 + q));
+        }
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) part[w][lane][q] = acc[q];
@@ -293,6 +371,8 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
     double* __restrict__ pcol, double* __restrict__ pabs_row,
     double* __restrict__ pabs_col, int ntile, int with_norm) {
   constexpr int TB = kSymvTile;
+  // the tile is read from HBM once, coalesced, and both passes run from LDS
+  __shared__ double tile[TB][TB + 1];
   __shared__ double xs_i[TB * kSolveMaxRhs];
   __shared__ double xs_j[TB * kSolveMaxRhs];
   __shared__ double red[4][TB * (kSolveMaxRhs + 1)];
@@ -303,6 +383,11 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
   const int I = idx - J * (J + 1) / 2;
   const int i0 = I * TB, j0 = J * TB;
   const int t = threadIdx.x;
+  for (int e = t; e < TB * TB; e += 256) {
+    const int r = e / TB, c = e % TB;
+    const int i = i0 + r, jj = j0 + c;
+    tile[r][c] = (i < n && jj < n && jj >= i) ? A[(int64_t)i * lda + jj] : 0.0;
+  }
   for (int e = t; e < TB * nr; e += 256) {
     const int r = e / nr, q = e % nr;
     xs_i[r * kSolveMaxRhs + q] = (i0 + r < n) ? X[(int64_t)(i0 + r) * ldx + q] : 0.0;
@@ -312,18 +397,12 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
   // row sums: thread (r = t & 63, part = t >> 6) covers 16 columns
   {
     const int r = t & 63, part = t >> 6;
-    const int i = i0 + r;
     double s[kSolveMaxRhs] = {0, 0, 0, 0};
     double sa = 0.0;
-    if (i < n) {
-      for (int c = part * 16; c < part * 16 + 16; ++c) {
-        const int j = j0 + c;
-        if (j < n && j >= i) {
-          const double a = A[(int64_t)i * lda + j];
-          for (int q = 0; q < nr; ++q) s[q] += a * xs_j[c * kSolveMaxRhs + q];
-          sa += fabs(a);
-        }
-      }
+    for (int c = part * 16; c < part * 16 + 16; ++c) {
+      const double a = tile[r][c];
+      for (int q = 0; q < nr; ++q) s[q] += a * xs_j[c * kSolveMaxRhs + q];
+      sa += fabs(a);
     }
     for (int q = 0; q < nr; ++q) red[part][r * (kSolveMaxRhs + 1) + q] = s[q];
     red[part][r * (kSolveMaxRhs + 1) + kSolveMaxRhs] = sa;
@@ -345,17 +424,14 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
   // column sums over the strictly upper part: thread (c = t & 63, part)
   {
     const int c = t & 63, part = t >> 6;
-    const int j = j0 + c;
+    const int jj = j0 + c;
     double s[kSolveMaxRhs] = {0, 0, 0, 0};
     double sa = 0.0;
-    if (j < n) {
-      for (int r = part * 16; r < part * 16 + 16; ++r) {
-        const int i = i0 + r;
-        if (i < n && i < j) {
-          const double a = A[(int64_t)i * lda + j];
-          for (int q = 0; q < nr; ++q) s[q] += a * xs_i[r * kSolveMaxRhs + q];
-          sa += fabs(a);
-        }
+    for (int r = part * 16; r < part * 16 + 16; ++r) {
+      if (i0 + r < jj) {
+        const double a = tile[r][c];
+        for (int q = 0; q < nr; ++q) s[q] += a * xs_i[r * kSolveMaxRhs + q];
+        sa += fabs(a);
       }
     }
     for (int q = 0; q < nr; ++q) red[part][c * (kSolveMaxRhs + 1) + q] = s[q];
@@ -384,7 +460,9 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
 }
 
 // y_i = sum_{J >= I(i)} prow[J][i] + sum_{I' <= J(i)} pcol[I'][i] in a fixed
-// order; R = B - Y (binary64); norms into norms[0..]:
+// order: 64 rows per workgroup, four wavefronts each summing every fourth
+// tile's partial, combined in wavefront order; R = B - Y (binary64); norms
+// into norms[0..]:
 //   norms[q] = max|r_q|, norms[4+q] = max|x_q|, norms[8+q] = max|b_q|,
 //   norms[12] = ||A||_inf (when with_norm).
 __global__ __launch_bounds__(256) void k_symv_reduce(
@@ -394,14 +472,32 @@ __global__ __launch_bounds__(256) void k_symv_reduce(
     int64_t ldb, const double* __restrict__ X, int64_t ldx,
     double* __restrict__ R, int64_t ldr, double* __restrict__ norms,
     int with_norm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  __shared__ double part[4][64][kSolveMaxRhs + 1];
+  const int li = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + li;
+  const bool ok = i < n;
   const int T = (n + kSymvTile - 1) / kSymvTile;
-  const int ti = i / kSymvTile;
+  const int ti = ok ? i / kSymvTile : 0;
+  double acc[kSolveMaxRhs + 1] = {0, 0, 0, 0, 0};
+  if (ok) {
+#pragma unroll 4
+    for (int J = ti + w; J < T; J += 4)
+      for (int q = 0; q < nr; ++q) acc[q] += prow[((int64_t)J * n + i) * nr + q];
+#pragma unroll 4
+    for (int I = w; I <= ti; I += 4)
+      for (int q = 0; q < nr; ++q) acc[q] += pcol[((int64_t)I * n + i) * nr + q];
+    if (with_norm) {
+#pragma unroll 4
+      for (int J = ti + w; J < T; J += 4) acc[kSolveMaxRhs] += pabs_row[(int64_t)J * n + i];
+#pragma unroll 4
+      for (int I = w; I <= ti; I += 4) acc[kSolveMaxRhs] += pabs_col[(int64_t)I * n + i];
+    }
+  }
+  for (int q = 0; q <= kSolveMaxRhs; ++q) part[w][li][q] = acc[q];
+  __syncthreads();
+  if (w != 0 || !ok) return;
   for (int q = 0; q < nr; ++q) {
-    double y = 0.0;
-    for (int J = ti; J < T; ++J) y += prow[((int64_t)J * n + i) * nr + q];
-    for (int I = 0; I <= ti; ++I) y += pcol[((int64_t)I * n + i) * nr + q];
+    const double y = ((part[0][li][q] + part[1][li][q]) + part[2][li][q]) + part[3][li][q];
     const double b = Bv[(int64_t)i * ldb + q];
     const double r = b - y;
     if (R) R[(int64_t)i * ldr + q] = r;
@@ -410,10 +506,9 @@ __global__ __launch_bounds__(256) void k_symv_reduce(
     atomic_max_nonneg(&norms[8 + q], fabs(b));
   }
   if (with_norm) {
-    double s = 0.0;
-    for (int J = ti; J < T; ++J) s += pabs_row[(int64_t)J * n + i];
-    for (int I = 0; I <= ti; ++I) s += pabs_col[(int64_t)I * n + i];
-    atomic_max_nonneg(&norms[12], s);
+    const double sabs = ((part[0][li][kSolveMaxRhs] + part[1][li][kSolveMaxRhs]) +
+                         part[2][li][kSolveMaxRhs]) + part[3][li][kSolveMaxRhs];
+    atomic_max_nonneg(&norms[12], sabs);
   }
 }
 
@@ -587,7 +682,7 @@ int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
     const int tiles = T * (T + 1) / 2;
     hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, h->stream, n, nr, A, lda,
                        X + q0, ldx, prow, pcol, parow, pacol, T, with_norm);
-    hipLaunchKernelGGL(k_symv_reduce, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, n, nr,
+    hipLaunchKernelGGL(k_symv_reduce, dim3(blocks_for(n, 64)), dim3(256), 0, h->stream, n, nr,
                        prow, pcol, parow, pacol, B + q0, ldb, X + q0, ldx,
                        R ? R + q0 : nullptr, ldr, norms, with_norm);
     hipLaunchKernelGGL(k_berr, dim3(1), dim3(32), 0, h->stream, nr, norms,
