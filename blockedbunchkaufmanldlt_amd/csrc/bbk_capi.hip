@@ -157,8 +157,11 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     h->epoch += 1024;
     pa.Wg = fs.Wg;
     pa.ldw = fs.ldw;
+    // Small panels run on one XCD: the grid is spread over the 8 XCDs and
+    // only every 8th workgroup works, when all of them fit co-resident.
+    pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
     h->timing_mark(Handle::kPanel, true);
-    e = launch_panel<T>(A, pa, groups, st);
+    e = launch_panel<T>(A, pa, groups * pa.xcd_stride, st);
     if (e != hipSuccess) return h->hip_fail(e, "panel launch");
     h->timing_mark(Handle::kPanel, false);
     // Rows above this panel: at most s_nom + panels (each earlier panel

@@ -18,6 +18,7 @@ struct PanelArgs {
   int epoch;        // record tags of this launch are epoch + 1, epoch + 2, ...
   void* Wg;         // (kPanelMaxWidth + 1) x ldw elements of T
   int64_t ldw;
+  int xcd_stride;   // 8: only workgroups blockIdx % 8 == 0 work (one XCD); 1: all
 };
 
 // Exchange records the panel kernel needs for a grid of `groups`.
