@@ -73,7 +73,6 @@ struct FactorScratch {
   size_t xrec_bytes;
   int* state;
   int* swaps;
-  int* plan;
   void* Wg;
   int64_t ldw;
 };
@@ -93,7 +92,6 @@ static size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
   f.xrec = reinterpret_cast<XRec*>(take(f.xrec_bytes));
   f.state = reinterpret_cast<int*>(take(sizeof(int) * PanelState::kInts));
   f.swaps = reinterpret_cast<int*>(take(sizeof(int) * 2 * (kPanelMaxWidth + 1)));
-  f.plan = reinterpret_cast<int*>(take(sizeof(int) * 4 * (kPanelMaxWidth + 1)));
   f.Wg = take(sizeof(T) * (kPanelMaxWidth + 1) * static_cast<size_t>(ldw));
   f.ldw = ldw;
   if (fs) *fs = f;
@@ -165,16 +163,12 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     if (e != hipSuccess) return h->hip_fail(e, "panel launch");
     h->timing_mark(Handle::kPanel, false);
     // Rows above this panel: at most s_nom + panels (each earlier panel
-    // can widen by one column).
-    const int rows_bound = std::min(n, s_nom + panels);
-    if (s_nom > 0) {
-      h->timing_mark(Handle::kSwap, true);
-      launch_swap_rows<T>(A, lda, fs.state, fs.swaps, fs.plan, rows_bound, st);
-      h->timing_mark(Handle::kSwap, false);
-    }
+    // can widen by one column); their interchanges run in the same launch
+    // as the trailing update.
+    const int rows_bound = s_nom > 0 ? std::min(n, s_nom + panels) : 0;
     h->timing_mark(Handle::kUpdate, true);
     launch_trailing_update<T>(A, lda, reinterpret_cast<const T*>(fs.Wg), fs.ldw,
-                              fs.state, n, n - s_nom - b, st);
+                              fs.state, fs.swaps, n, n - s_nom - b, rows_bound, st);
     h->timing_mark(Handle::kUpdate, false);
   }
   if (h->timing) h->panels += panels;

@@ -59,32 +59,20 @@ __device__ __forceinline__ void tri_tile(int idx, int* I, int* J) {
 // loads overlap the current chunk's MFMAs.  TM x TM output tile per
 // workgroup, four wavefronts of (TM/2) x (TM/2); MINB workgroups per CU;
 // with XCD, consecutive tiles (sharing U and W strips) go to one XCD.
-template <typename T, int TM, int MINB, int KC = 16, bool XCD = false>
-__global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
-    T* __restrict__ A, int64_t lda, const T* __restrict__ Wg, int64_t ldw,
-    const int* __restrict__ state, int n) {
+template <typename T, int TM, int KC>
+__device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
+                                            const T* __restrict__ Wg, int64_t ldw, int s,
+                                            int bw, int n, int idx, T (*Us)[KC * (TM + 4)],
+                                            T (*Ws)[KC * (TM + 4)]) {
   constexpr int LDS_LD = TM + 4;
   constexpr int PER = KC * TM / 256;  // staged elements per thread per operand
   constexpr int WT = TM / 2;          // wavefront tile
   constexpr int TI = WT / 16;         // 16 x 16 MFMA blocks per wavefront side
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T Us[2][KC * LDS_LD];
-  __shared__ T Ws[2][KC * LDS_LD];
-
-  const int s = state[PanelState::kStart];
-  const int bw = state[PanelState::kWidth];
   const int e = s + bw;
   const int m = n - e;
   if (m <= 0 || bw <= 0) return;
   const int tiles = (m + TM - 1) / TM;
-  int idx = blockIdx.x;
-  if (XCD) {
-    // workgroups are dealt to the 8 XCDs round-robin; give XCD x a
-    // contiguous run of tiles (neighbouring W and U strips share its L2)
-    const int nb = gridDim.x, per = (nb + 7) / 8, x = idx & 7, r = idx >> 3;
-    const int full = nb - 8 * (per - 1);  // XCDs that get `per` tiles
-    idx = x < full ? x * per + r : full * per + (x - full) * (per - 1) + r;
-  }
   int I, J;
   tri_tile(idx, &I, &J);
   if (J >= tiles) return;
@@ -163,33 +151,20 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
       }
 }
 
-// Tile shape (measured on MI355X at m = 16384, K = 64 / 128): 64 x 64
-// tiles with four or more workgroups resident per CU let one workgroup's C
-// traffic overlap another's MFMAs (binary64 15.7 -> 25.1 TFLOP/s, binary32
-// 40 -> 59 TFLOP/s against 128 x 128 tiles); K chunks of 8; XCD-contiguous
-// tile order.
-template <typename T>
-struct UpdateCfg {
-  static constexpr int kTM = 64;
-  static constexpr int kMinBlocks = 4;
-  static constexpr int kKC = 8;
-};
-
-// ---------------------------------------------------------------------
-// Deferred interchanges of the rows above the panel: the panel's
-// interchange sequence composed into one gather per row
-// (the reference swaps these immediately, src/block_ldlt_decomposition.cc:472-491).
-__global__ __launch_bounds__(256) void bk_swap_plan_kernel(
-    const int* __restrict__ state, const int* __restrict__ swaps,
-    int* __restrict__ plan) {
-  constexpr int NE = 2 * (kPanelMaxWidth + 1);
-  __shared__ int cols[NE];
-  __shared__ int slot[NE];
-  __shared__ int content[NE];
-  const int ns = state[PanelState::kNumSwaps];
+// Deferred interchanges of the rows above the panel (the reference swaps
+// them immediately, src/block_ldlt_decomposition.cc:472-491): the panel's
+// interchange sequence (a_q, r_q), q < ns, is composed into one gather per
+// row, row[dest_t] = row[src_t] over the ne = 2 ns touched columns, and
+// applied to rows [p0, p1).  Every workgroup of this role composes the
+// gather itself (ne <= 2 (kPanelMaxWidth + 1) entries).
+template <typename T, int RPG>
+__device__ __forceinline__ void swap_rows(T* __restrict__ A, int64_t lda,
+                                          const int* __restrict__ swaps, int ns, int p0,
+                                          int p1, int* cols, int* slot, int* content) {
   const int ne = 2 * ns;
   for (int t = threadIdx.x; t < ne; t += blockDim.x) cols[t] = swaps[t];
   __syncthreads();
+  // first occurrence of each column: its slot in the composed gather
   for (int t = threadIdx.x; t < ne; t += blockDim.x) {
     const int cval = cols[t];
     int f = t;
@@ -211,36 +186,68 @@ __global__ __launch_bounds__(256) void bk_swap_plan_kernel(
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
-    plan[t] = cols[t];                 // destination column
-    plan[NE + t] = content[slot[t]];   // source column
+  const int t0 = threadIdx.x, t1 = threadIdx.x + 256;
+  const int d0 = t0 < ne ? cols[t0] : 0, s0 = t0 < ne ? content[slot[t0]] : 0;
+  const int d1 = t1 < ne ? cols[t1] : 0, s1 = t1 < ne ? content[slot[t1]] : 0;
+  T v0[RPG], v1[RPG];
+#pragma unroll
+  for (int u = 0; u < RPG; ++u) {
+    const int row = p0 + u;
+    const T* R = A + (int64_t)row * lda;
+    v0[u] = (row < p1 && t0 < ne) ? R[s0] : T(0);
+    v1[u] = (row < p1 && t1 < ne) ? R[s1] : T(0);
+  }
+  __syncthreads();  // every read of the rows precedes every write
+#pragma unroll
+  for (int u = 0; u < RPG; ++u) {
+    const int row = p0 + u;
+    T* R = A + (int64_t)row * lda;
+    if (row < p1 && t0 < ne) R[d0] = v0[u];
+    if (row < p1 && t1 < ne) R[d1] = v1[u];
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void bk_swap_apply_kernel(
-    T* __restrict__ A, int64_t lda, const int* __restrict__ state,
-    const int* __restrict__ plan, int rows_per_group) {
+// One launch after each panel: workgroups [0, gemm_blocks) update trailing
+// tiles (XCD-contiguous order), the rest apply the panel's interchanges to
+// RPG rows each of the finished rows above the panel.  The two regions are
+// disjoint (rows >= e against rows < s), so both run concurrently.
+template <typename T, int TM, int MINB, int KC, int RPG>
+__global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
+    T* __restrict__ A, int64_t lda, const T* __restrict__ Wg, int64_t ldw,
+    const int* __restrict__ state, const int* __restrict__ swaps, int n, int gemm_blocks) {
   constexpr int NE = 2 * (kPanelMaxWidth + 1);
+  __shared__ T Us[2][KC * (TM + 4)];
+  __shared__ T Ws[2][KC * (TM + 4)];
+  __shared__ int cols[NE], slot[NE], content[NE];
   const int s = state[PanelState::kStart];
-  const int ne = 2 * state[PanelState::kNumSwaps];
-  if (ne == 0) return;
-  const int p0 = blockIdx.x * rows_per_group;
-  if (p0 >= s) return;
-  const int p1 = min(s, p0 + rows_per_group);
-  const int t0 = threadIdx.x, t1 = threadIdx.x + 256;
-  const int d0 = t0 < ne ? plan[t0] : 0, s0 = t0 < ne ? plan[NE + t0] : 0;
-  const int d1 = t1 < ne ? plan[t1] : 0, s1 = t1 < ne ? plan[NE + t1] : 0;
-  for (int row = p0; row < p1; ++row) {
-    T* R = A + (int64_t)row * lda;
-    const T v0 = t0 < ne ? R[s0] : T(0);
-    const T v1 = t1 < ne ? R[s1] : T(0);
-    __syncthreads();
-    if (t0 < ne) R[d0] = v0;
-    if (t1 < ne) R[d1] = v1;
-    __syncthreads();
+  const int bw = state[PanelState::kWidth];
+  int idx = blockIdx.x;
+  if (idx < gemm_blocks) {
+    // workgroups are dealt to the 8 XCDs round-robin; give XCD x a
+    // contiguous run of tiles (neighbouring W and U strips share its L2)
+    const int nb = gemm_blocks, per = (nb + 7) / 8, x = idx & 7, r = idx >> 3;
+    const int full = nb - 8 * (per - 1);  // XCDs that get `per` tiles
+    idx = x < full ? x * per + r : full * per + (x - full) * (per - 1) + r;
+    update_tile<T, TM, KC>(A, lda, Wg, ldw, s, bw, n, idx, Us, Ws);
+  } else {
+    const int ns = state[PanelState::kNumSwaps];
+    const int p0 = (idx - gemm_blocks) * RPG;
+    if (ns == 0 || p0 >= s) return;
+    swap_rows<T, RPG>(A, lda, swaps, ns, p0, min(s, p0 + RPG), cols, slot, content);
   }
 }
+
+// Tile shape (measured on MI355X at m = 16384, K = 64 / 128): 64 x 64
+// tiles with four or more workgroups resident per CU let one workgroup's C
+// traffic overlap another's MFMAs (binary64 15.7 -> 25.1 TFLOP/s, binary32
+// 40 -> 59 TFLOP/s against 128 x 128 tiles); K chunks of 8; XCD-contiguous
+// tile order.
+template <typename T>
+struct UpdateCfg {
+  static constexpr int kTM = 64;
+  static constexpr int kMinBlocks = 4;
+  static constexpr int kKC = 8;
+};
 
 __global__ void bk_init_kernel(int n, int* __restrict__ perm,
                                int* __restrict__ piv, int* __restrict__ state) {
@@ -261,27 +268,18 @@ __global__ void bk_zero_u32_kernel(unsigned* __restrict__ p, int count) {
 // Launchers.
 template <typename T>
 void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
-                            const int* state, int n, int m_bound,
-                            hipStream_t stream) {
-  if (m_bound <= 0) return;
+                            const int* state, const int* swaps, int n, int m_bound,
+                            int rows_bound, hipStream_t stream) {
   constexpr int TM = UpdateCfg<T>::kTM;
-  const int tiles = (m_bound + TM - 1) / TM;
-  const int blocks = tiles * (tiles + 1) / 2;
+  constexpr int RPG = 8;
+  const int tiles = m_bound > 0 ? (m_bound + TM - 1) / TM : 0;
+  const int gemm_blocks = tiles * (tiles + 1) / 2;
+  const int swap_blocks = rows_bound > 0 ? (rows_bound + RPG - 1) / RPG : 0;
+  if (gemm_blocks + swap_blocks == 0) return;
   hipLaunchKernelGGL((bk_trailing_update_kernel<T, TM, UpdateCfg<T>::kMinBlocks,
-                                                 UpdateCfg<T>::kKC, true>),
-                     dim3(blocks), dim3(256), 0, stream, A, lda, Wg, ldw, state, n);
-}
-
-template <typename T>
-void launch_swap_rows(T* A, int64_t lda, const int* state, const int* swaps,
-                      int* plan, int rows_bound, hipStream_t stream) {
-  if (rows_bound <= 0) return;
-  hipLaunchKernelGGL(bk_swap_plan_kernel, dim3(1), dim3(256), 0, stream, state,
-                     swaps, plan);
-  const int rpg = 4;
-  const int groups = (rows_bound + rpg - 1) / rpg;
-  hipLaunchKernelGGL(bk_swap_apply_kernel<T>, dim3(groups), dim3(256), 0,
-                     stream, A, lda, state, plan, rpg);
+                                                 UpdateCfg<T>::kKC, RPG>),
+                     dim3(gemm_blocks + swap_blocks), dim3(256), 0, stream, A, lda, Wg, ldw,
+                     state, swaps, n, gemm_blocks);
 }
 
 void launch_init(int n, int* perm, int* piv, int* state, hipStream_t stream) {
@@ -295,15 +293,11 @@ void launch_zero_u32(unsigned* p, int count, hipStream_t stream) {
                      0, stream, p, count);
 }
 
-template void launch_trailing_update<double>(double*, int64_t, const double*,
-                                             int64_t, const int*, int, int,
+template void launch_trailing_update<double>(double*, int64_t, const double*, int64_t,
+                                             const int*, const int*, int, int, int,
                                              hipStream_t);
-template void launch_trailing_update<float>(float*, int64_t, const float*,
-                                            int64_t, const int*, int, int,
+template void launch_trailing_update<float>(float*, int64_t, const float*, int64_t,
+                                            const int*, const int*, int, int, int,
                                             hipStream_t);
-template void launch_swap_rows<double>(double*, int64_t, const int*,
-                                       const int*, int*, int, hipStream_t);
-template void launch_swap_rows<float>(float*, int64_t, const int*, const int*,
-                                      int*, int, hipStream_t);
 
 }  // namespace bbk

@@ -29,13 +29,12 @@ hipError_t launch_panel(T* A, const PanelArgs& args, int groups,
                         hipStream_t stream);
 template <typename T>
 hipError_t panel_max_groups(int* groups);
+// Trailing update of the panel just factored plus the interchange of the
+// rows above it (rows_bound: upper bound of the panel start; 0 = none).
 template <typename T>
 void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
-                            const int* state, int n, int m_bound,
-                            hipStream_t stream);
-template <typename T>
-void launch_swap_rows(T* A, int64_t lda, const int* state, const int* swaps,
-                      int* plan, int rows_bound, hipStream_t stream);
+                            const int* state, const int* swaps, int n, int m_bound,
+                            int rows_bound, hipStream_t stream);
 void launch_init(int n, int* perm, int* piv, int* state, hipStream_t stream);
 void launch_zero_u32(unsigned* p, int count, hipStream_t stream);
 
