@@ -27,12 +27,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "../../include/bbk_ldlt.h"
 #include "bbk_batched.hpp"
 #include "bbk_handle.hpp"
+#include "bbk_update.hpp"
 
 namespace bbk {
 
@@ -110,214 +112,231 @@ __device__ __forceinline__ T ucol(const ScaleTab& st, const T* W, int n, int p, 
                         st.c1[p] * w);
 }
 
-// One matrix, whole team of NT threads.  F: n x lda row-major upper
-// triangle; W: (bw + 2) rows of n elements (rows 0..bw = panel rows, last =
-// the candidate row C).
+// Panel P of one matrix, whole team of NT threads, starting at column s
+// (one column early when the previous panel deferred a 2x2): Bunch-Kaufman
+// steps with the panel rows W (row stride n; rows 0..bw = panel rows, row
+// bw + 1 = the candidate row C).  Returns the panel's end e (the nominal
+// end, or the column of a deferred 2x2).
+template <typename T, int NT>
+__device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
+                            int* __restrict__ piv, int bw, int P, int s, T* W, T* Lk, T* Lr,
+                            ScaleTab& st, double* rv, int* ri) {
+  const int tid = threadIdx.x;
+  T* C = W + (int64_t)(bw + 1) * n;
+  int e = min((P + 1) * bw, n);  // nominal end; s may sit one column early
+  int kk = 0;
+  while (s + kk < e) {
+    const int k = s + kk;
+    if (tid < kk) Lk[tid] = ucol<T>(st, W, n, tid, k);  // L(k, s + tid)
+    __syncthreads();
+    // W(k, j) = F(k, j) - sum_p L(k, p) W(p, j), j >= k  (:122-162)
+    double best = 0.0;
+    int arg = n;
+    T* Wk = W + (int64_t)kk * n;
+    for (int j = k + tid; j < n; j += NT) {
+      T acc = T(0);
+      for (int p = 0; p < kk; ++p) acc += Lk[p] * W[(int64_t)p * n + j];
+      const T w = F[(int64_t)k * lda + j] - acc;
+      Wk[j] = w;
+      const double a = fabs(static_cast<double>(w));
+      if (j > k && a > best) {
+        best = a;
+        arg = j;
+      }
+    }
+    team_argmax<NT>(best, arg, rv, ri);  // (:164-182) first maximum
+    __syncthreads();
+    const double omega = best;
+    const int r = arg;
+    const double akk = fabs(static_cast<double>(Wk[k]));
+    int kind = 1;
+    if (!(akk >= kAlpha * omega || omega == 0.0)) {
+      // candidate row r: C(i) = A(r, i) - sum_p L(r, p) W(p, i), i >= k
+      // (:184-352), sigma over i != r (:354-367)
+      if (tid < kk) Lr[tid] = ucol<T>(st, W, n, tid, r);  // L(r, s + tid)
+      __syncthreads();
+      double sig = 0.0;
+      for (int i = k + tid; i < n; i += NT) {
+        const T base = i < r ? F[(int64_t)i * lda + r] : F[(int64_t)r * lda + i];
+        T acc = T(0);
+        for (int p = 0; p < kk; ++p) acc += Lr[p] * W[(int64_t)p * n + i];
+        const T cv = base - acc;
+        C[i] = cv;
+        const double a = fabs(static_cast<double>(cv));
+        if (i != r) sig = (sig < a) ? a : sig;
+      }
+      const double sigma = team_max<NT>(sig, rv);
+      __syncthreads();
+      const double arr = fabs(static_cast<double>(C[r]));
+      kind = select_pivot_nc(akk, omega, sigma, arr);  // 3, -1 or 2
+      if (kind == 2 && kk == e - s - 1 && e < n) {
+        // 2x2 across the panel edge: this panel ends before k
+        e = k;
+        __syncthreads();
+        break;
+      }
+      // the candidate's value at k replaces W(k, r) (column part of the
+      // candidate row written over row k's entry)
+      if (tid == 0) Wk[r] = C[k];
+      __syncthreads();
+      if (kind == -1) {
+        // interchange k <-> r (:414-498); row k of W becomes the candidate
+        for (int j = r + 1 + tid; j < n; j += NT)
+          swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)r * lda + j]);
+        for (int j = k + 1 + tid; j < r; j += NT)
+          swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)j * lda + r]);
+        for (int i = tid; i < k; i += NT)
+          swap_elems(&F[(int64_t)i * lda + k], &F[(int64_t)i * lda + r]);
+        for (int p = tid; p < kk; p += NT)
+          swap_elems(&W[(int64_t)p * n + k], &W[(int64_t)p * n + r]);
+        for (int j = k + 1 + tid; j < n; j += NT)
+          if (j != r) Wk[j] = C[j];
+        if (tid == 0) {
+          swap_elems(&F[(int64_t)k * lda + k], &F[(int64_t)r * lda + r]);
+          Wk[k] = C[r];
+          const int t = perm[k];
+          perm[k] = perm[r];
+          perm[r] = t;
+        }
+      } else if (kind == 2) {
+        // interchange k+1 <-> r (:500-587); row k+1 of W becomes the candidate
+        const int k1 = k + 1;
+        T* Wk1 = W + (int64_t)(kk + 1) * n;
+        if (r != k1) {
+          for (int j = r + 1 + tid; j < n; j += NT)
+            swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)r * lda + j]);
+          for (int j = k + 2 + tid; j < r; j += NT)
+            swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)j * lda + r]);
+          for (int i = tid; i < k; i += NT)
+            swap_elems(&F[(int64_t)i * lda + k1], &F[(int64_t)i * lda + r]);
+          for (int p = tid; p < kk; p += NT)
+            swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
+        }
+        for (int j = k + 2 + tid; j < n; j += NT)
+          if (j != r) Wk1[j] = C[j];
+        if (tid == 0) {
+          if (r != k1) {
+            swap_elems(&F[(int64_t)k1 * lda + k1], &F[(int64_t)r * lda + r]);
+            swap_elems(&F[(int64_t)k * lda + k1], &F[(int64_t)k * lda + r]);
+            swap_elems(&Wk[k1], &Wk[r]);
+            Wk1[r] = C[k1];
+            const int t = perm[k1];
+            perm[k1] = perm[r];
+            perm[r] = t;
+          }
+          Wk1[k1] = C[r];
+        }
+      }
+      __syncthreads();
+    }
+    if (kind != 2) {
+      // ScaleOneByOnePivot (:633-656)
+      const double d = static_cast<double>(Wk[k]);
+      const double den = safe_den(d);
+      for (int j = k + 1 + tid; j < n; j += NT)
+        F[(int64_t)k * lda + j] = static_cast<T>(div_or_zero(static_cast<double>(Wk[j]), den));
+      if (tid == 0) {
+        F[(int64_t)k * lda + k] = Wk[k];
+        piv[k] = 1;
+        st.ty[kk] = 1;
+        st.c0[kk] = den;
+      }
+      kk += 1;
+    } else {
+      // ScaleTwoByTwoPivot (:658-704), binary64 without contraction
+      const int k1 = k + 1;
+      const T* Wk1 = W + (int64_t)(kk + 1) * n;
+      const double a = static_cast<double>(Wk[k]);
+      const double b = static_cast<double>(Wk[k1]);
+      const double c = static_cast<double>(Wk1[k1]);
+      double itl, ibr, ioff;
+      inverse_2x2_nc(a, b, c, &itl, &ibr, &ioff);
+      for (int j = k + 2 + tid; j < n; j += NT) {
+        double u0, u1;
+        apply_2x2_nc(itl, ibr, ioff, static_cast<double>(Wk[j]), static_cast<double>(Wk1[j]),
+                     &u0, &u1);
+        F[(int64_t)k * lda + j] = static_cast<T>(u0);
+        F[(int64_t)k1 * lda + j] = static_cast<T>(u1);
+      }
+      if (tid == 0) {
+        F[(int64_t)k * lda + k] = Wk[k];
+        F[(int64_t)k * lda + k1] = Wk[k1];
+        F[(int64_t)k1 * lda + k1] = Wk1[k1];
+        piv[k] = 2;
+        piv[k1] = -2;
+        st.ty[kk] = 2;
+        st.c0[kk] = itl;
+        st.c1[kk] = ioff;
+        st.ty[kk + 1] = -2;
+        st.c0[kk + 1] = ioff;
+        st.c1[kk + 1] = ibr;
+      }
+      kk += 2;
+    }
+    __syncthreads();
+  }
+  return e;
+}
+
+// Trailing update F(i, j) -= sum_p F(s+p, i) W(p, j), e <= i <= j < n
+// (:706-744, src/kernels.cc:67-91), 4 x 4 register tiles per thread.
+template <typename T, int NT>
+__device__ void simt_update(T* __restrict__ F, int64_t lda, int n, int s, int e, const T* W) {
+  const int tid = threadIdx.x;
+  const int m = n - e, kp = e - s;
+  if (m > 0 && kp > 0) {
+    const int nt = (m + 3) >> 2;
+    const int tiles = nt * (nt + 1) / 2;
+    for (int t = tid; t < tiles; t += NT) {
+      int tj = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+      while (tj * (tj + 1) / 2 > t) --tj;
+      while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
+      const int ti = t - tj * (tj + 1) / 2;
+      const int i0 = e + 4 * ti, j0 = e + 4 * tj;
+      T acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+      for (int p = 0; p < kp; ++p) {
+        const T* urow = F + (int64_t)(s + p) * lda;
+        const T* wrow = W + (int64_t)p * n;
+        T u[4], w[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          u[a] = i0 + a < n ? urow[i0 + a] : T(0);
+          w[a] = j0 + a < n ? wrow[j0 + a] : T(0);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] += u[a] * w[b];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int i = i0 + a, j = j0 + b;
+          if (i <= j && j < n) F[(int64_t)i * lda + j] -= acc[a][b];
+        }
+    }
+  }
+}
+
+// One matrix, whole team: every panel, then its trailing update.
 template <typename T, int NT>
 __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
                            int* __restrict__ piv, int bw, T* W, T* Lk, T* Lr, ScaleTab& st,
                            double* rv, int* ri) {
-  const int tid = threadIdx.x;
-  T* C = W + (int64_t)(bw + 1) * n;
-  for (int i = tid; i < n; i += NT) {
+  for (int i = threadIdx.x; i < n; i += NT) {
     perm[i] = i;
     piv[i] = 1;
   }
   __syncthreads();
   int s = 0;
   for (int P = 0; s < n; ++P) {
-    int e = min((P + 1) * bw, n);  // nominal end; s may sit one column early
-    int kk = 0;
-    while (s + kk < e) {
-      const int k = s + kk;
-      if (tid < kk) Lk[tid] = ucol<T>(st, W, n, tid, k);  // L(k, s + tid)
-      __syncthreads();
-      // W(k, j) = F(k, j) - sum_p L(k, p) W(p, j), j >= k  (:122-162)
-      double best = 0.0;
-      int arg = n;
-      T* Wk = W + (int64_t)kk * n;
-      for (int j = k + tid; j < n; j += NT) {
-        T acc = T(0);
-        for (int p = 0; p < kk; ++p) acc += Lk[p] * W[(int64_t)p * n + j];
-        const T w = F[(int64_t)k * lda + j] - acc;
-        Wk[j] = w;
-        const double a = fabs(static_cast<double>(w));
-        if (j > k && a > best) {
-          best = a;
-          arg = j;
-        }
-      }
-      team_argmax<NT>(best, arg, rv, ri);  // (:164-182) first maximum
-      __syncthreads();
-      const double omega = best;
-      const int r = arg;
-      const double akk = fabs(static_cast<double>(Wk[k]));
-      int kind = 1;
-      if (!(akk >= kAlpha * omega || omega == 0.0)) {
-        // candidate row r: C(i) = A(r, i) - sum_p L(r, p) W(p, i), i >= k
-        // (:184-352), sigma over i != r (:354-367)
-        if (tid < kk) Lr[tid] = ucol<T>(st, W, n, tid, r);  // L(r, s + tid)
-        __syncthreads();
-        double sig = 0.0;
-        for (int i = k + tid; i < n; i += NT) {
-          const T base = i < r ? F[(int64_t)i * lda + r] : F[(int64_t)r * lda + i];
-          T acc = T(0);
-          for (int p = 0; p < kk; ++p) acc += Lr[p] * W[(int64_t)p * n + i];
-          const T cv = base - acc;
-          C[i] = cv;
-          const double a = fabs(static_cast<double>(cv));
-          if (i != r) sig = (sig < a) ? a : sig;
-        }
-        const double sigma = team_max<NT>(sig, rv);
-        __syncthreads();
-        const double arr = fabs(static_cast<double>(C[r]));
-        kind = select_pivot_nc(akk, omega, sigma, arr);  // 3, -1 or 2
-        if (kind == 2 && kk == e - s - 1 && e < n) {
-          // 2x2 across the panel edge: this panel ends before k
-          e = k;
-          __syncthreads();
-          break;
-        }
-        // the candidate's value at k replaces W(k, r) (column part of the
-        // candidate row written over row k's entry)
-        if (tid == 0) Wk[r] = C[k];
-        __syncthreads();
-        if (kind == -1) {
-          // interchange k <-> r (:414-498); row k of W becomes the candidate
-          for (int j = r + 1 + tid; j < n; j += NT)
-            swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)r * lda + j]);
-          for (int j = k + 1 + tid; j < r; j += NT)
-            swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)j * lda + r]);
-          for (int i = tid; i < k; i += NT)
-            swap_elems(&F[(int64_t)i * lda + k], &F[(int64_t)i * lda + r]);
-          for (int p = tid; p < kk; p += NT)
-            swap_elems(&W[(int64_t)p * n + k], &W[(int64_t)p * n + r]);
-          for (int j = k + 1 + tid; j < n; j += NT)
-            if (j != r) Wk[j] = C[j];
-          if (tid == 0) {
-            swap_elems(&F[(int64_t)k * lda + k], &F[(int64_t)r * lda + r]);
-            Wk[k] = C[r];
-            const int t = perm[k];
-            perm[k] = perm[r];
-            perm[r] = t;
-          }
-        } else if (kind == 2) {
-          // interchange k+1 <-> r (:500-587); row k+1 of W becomes the candidate
-          const int k1 = k + 1;
-          T* Wk1 = W + (int64_t)(kk + 1) * n;
-          if (r != k1) {
-            for (int j = r + 1 + tid; j < n; j += NT)
-              swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)r * lda + j]);
-            for (int j = k + 2 + tid; j < r; j += NT)
-              swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)j * lda + r]);
-            for (int i = tid; i < k; i += NT)
-              swap_elems(&F[(int64_t)i * lda + k1], &F[(int64_t)i * lda + r]);
-            for (int p = tid; p < kk; p += NT)
-              swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
-          }
-          for (int j = k + 2 + tid; j < n; j += NT)
-            if (j != r) Wk1[j] = C[j];
-          if (tid == 0) {
-            if (r != k1) {
-              swap_elems(&F[(int64_t)k1 * lda + k1], &F[(int64_t)r * lda + r]);
-              swap_elems(&F[(int64_t)k * lda + k1], &F[(int64_t)k * lda + r]);
-              swap_elems(&Wk[k1], &Wk[r]);
-              Wk1[r] = C[k1];
-              const int t = perm[k1];
-              perm[k1] = perm[r];
-              perm[r] = t;
-            }
-            Wk1[k1] = C[r];
-          }
-        }
-        __syncthreads();
-      }
-      if (kind != 2) {
-        // ScaleOneByOnePivot (:633-656)
-        const double d = static_cast<double>(Wk[k]);
-        const double den = safe_den(d);
-        for (int j = k + 1 + tid; j < n; j += NT)
-          F[(int64_t)k * lda + j] = static_cast<T>(div_or_zero(static_cast<double>(Wk[j]), den));
-        if (tid == 0) {
-          F[(int64_t)k * lda + k] = Wk[k];
-          piv[k] = 1;
-          st.ty[kk] = 1;
-          st.c0[kk] = den;
-        }
-        kk += 1;
-      } else {
-        // ScaleTwoByTwoPivot (:658-704), binary64 without contraction
-        const int k1 = k + 1;
-        const T* Wk1 = W + (int64_t)(kk + 1) * n;
-        const double a = static_cast<double>(Wk[k]);
-        const double b = static_cast<double>(Wk[k1]);
-        const double c = static_cast<double>(Wk1[k1]);
-        double itl, ibr, ioff;
-        inverse_2x2_nc(a, b, c, &itl, &ibr, &ioff);
-        for (int j = k + 2 + tid; j < n; j += NT) {
-          double u0, u1;
-          apply_2x2_nc(itl, ibr, ioff, static_cast<double>(Wk[j]), static_cast<double>(Wk1[j]),
-                       &u0, &u1);
-          F[(int64_t)k * lda + j] = static_cast<T>(u0);
-          F[(int64_t)k1 * lda + j] = static_cast<T>(u1);
-        }
-        if (tid == 0) {
-          F[(int64_t)k * lda + k] = Wk[k];
-          F[(int64_t)k * lda + k1] = Wk[k1];
-          F[(int64_t)k1 * lda + k1] = Wk1[k1];
-          piv[k] = 2;
-          piv[k1] = -2;
-          st.ty[kk] = 2;
-          st.c0[kk] = itl;
-          st.c1[kk] = ioff;
-          st.ty[kk + 1] = -2;
-          st.c0[kk + 1] = ioff;
-          st.c1[kk + 1] = ibr;
-        }
-        kk += 2;
-      }
-      __syncthreads();
-    }
-    // trailing update F(i, j) -= sum_p F(s+p, i) W(p, j), e <= i <= j < n
-    // (:706-744, src/kernels.cc:67-91), 4 x 4 register tiles
-    const int m = n - e, kp = e - s;
-    if (m > 0 && kp > 0) {
-      const int nt = (m + 3) >> 2;
-      const int tiles = nt * (nt + 1) / 2;
-      for (int t = tid; t < tiles; t += NT) {
-        int tj = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-        while (tj * (tj + 1) / 2 > t) --tj;
-        while ((tj + 1) * (tj + 2) / 2 <= t) ++tj;
-        const int ti = t - tj * (tj + 1) / 2;
-        const int i0 = e + 4 * ti, j0 = e + 4 * tj;
-        T acc[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
-        for (int p = 0; p < kp; ++p) {
-          const T* urow = F + (int64_t)(s + p) * lda;
-          const T* wrow = W + (int64_t)p * n;
-          T u[4], w[4];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            u[a] = i0 + a < n ? urow[i0 + a] : T(0);
-            w[a] = j0 + a < n ? wrow[j0 + a] : T(0);
-          }
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] += u[a] * w[b];
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) {
-            const int i = i0 + a, j = j0 + b;
-            if (i <= j && j < n) F[(int64_t)i * lda + j] -= acc[a][b];
-          }
-      }
-    }
+    const int e = factor_panel<T, NT>(F, lda, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri);
+    simt_update<T, NT>(F, lda, n, s, e, W);
     __syncthreads();
     s = e;
   }
@@ -346,6 +365,89 @@ __global__ __launch_bounds__(NT) void k_batched_sytrf(BatchArgs<T> a) {
     T* W = need <= a.lds_elems ? lds : a.ws + (int64_t)blockIdx.x * a.ws_stride;
     factor_one<T, NT>(a.A[m], a.lda[m], n, a.perm[m], a.piv[m], bw, W, Lk, Lr, st, rv, ri);
   }
+}
+
+// ---- lockstep mode (batches of large matrices) -----------------------------
+// Per-matrix state (4 ints): next panel start, this panel's start, this
+// panel's rows (0: nothing to update), unused.
+template <typename T>
+struct StepArgs {
+  int count;
+  int step;
+  const int* ns;
+  T* const* A;
+  const int64_t* lda;
+  int* const* perm;
+  int* const* piv;
+  T* ws;                 // W slices, (width + 2) x n elements per matrix
+  const int64_t* wsoff;  // matrix -> first element of its slice
+  int* st;
+  const int* prefix;     // this step's first tile per matrix (count + 1 entries)
+  BatchWidths widths;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
+  __shared__ T Lk[kBatchMaxWidth + 2];
+  __shared__ T Lr[kBatchMaxWidth + 2];
+  __shared__ ScaleTab stab;
+  __shared__ double rv[4];
+  __shared__ int ri[4];
+  const int m = blockIdx.x;
+  const int n = a.ns[m];
+  int* S = a.st + 4 * m;
+  if (n <= 0) {
+    if (threadIdx.x == 0) S[2] = 0;
+    return;
+  }
+  const int bw = a.widths.w[batch_class(n)];
+  const int P = a.step;
+  if (P >= (n + bw - 1) / bw) {
+    if (threadIdx.x == 0) S[2] = 0;
+    return;
+  }
+  int* perm = a.perm[m];
+  int* piv = a.piv[m];
+  if (P == 0) {
+    for (int i = threadIdx.x; i < n; i += 256) {
+      perm[i] = i;
+      piv[i] = 1;
+    }
+    __syncthreads();
+  }
+  const int s = P == 0 ? 0 : S[0];
+  T* W = a.ws + a.wsoff[m];
+  const int e = factor_panel<T, 256>(a.A[m], a.lda[m], n, perm, piv, bw, P, s, W, Lk, Lr, stab,
+                                     rv, ri);
+  if (threadIdx.x == 0) {
+    S[0] = e;
+    S[1] = s;
+    S[2] = e - s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 4) void k_step_update(StepArgs<T> a) {
+  constexpr int TM = 64, KC = 8;
+  __shared__ T Us[2][KC * (TM + 4)];
+  __shared__ T Ws[2][KC * (TM + 4)];
+  const int b = blockIdx.x;
+  // matrix m with prefix[m] <= b < prefix[m + 1]
+  int lo = 0, hi = a.count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.prefix[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int m = lo;
+  const int* S = a.st + 4 * m;
+  const int kp = S[2];
+  if (kp <= 0) return;
+  const int n = a.ns[m];
+  update_tile<T, TM, KC>(a.A[m], a.lda[m], a.ws + a.wsoff[m], n, S[1], kp, n, b - a.prefix[m],
+                         Us, Ws);
 }
 
 __global__ void k_batch_fill(int* p, int count, int v) {
@@ -409,6 +511,68 @@ static int batched_grid(Handle* h, int count, int elem_bytes, int nt) {
   return nt == 64 ? batched_grid_nt<float, 64>(h, count) : batched_grid_nt<float, 256>(h, count);
 }
 
+// Lockstep schedule: per matrix a W slice and 4 state ints; per panel step
+// the prefix sums of the matrices' trailing-tile counts (a bound from the
+// nominal panel end: a deferred 2x2 ends a panel at most one column early).
+static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes,
+                            const BatchWidths& wt) {
+  BatchPlan& p = h->batch;
+  int steps = 0;
+  std::vector<int64_t> wsoff(count);
+  int64_t wel = 0;
+  for (int i = 0; i < count; ++i) {
+    const int n = h_n[i], bw = wt.w[batch_class(n)];
+    if (n > 0) steps = std::max(steps, (n + bw - 1) / bw);
+    wsoff[i] = wel;
+    wel += ((int64_t)(bw + 2) * n + 63) / 64 * 64;
+  }
+  const int64_t sstride = count + 1;
+  std::vector<int> prefix((size_t)steps * sstride, 0);
+  for (int P = 0; P < steps; ++P) {
+    int* pr = prefix.data() + (size_t)P * sstride;
+    int acc = 0;
+    for (int i = 0; i < count; ++i) {
+      pr[i] = acc;
+      const int n = h_n[i], bw = wt.w[batch_class(n)];
+      if (n > 0 && P < (n + bw - 1) / bw) {
+        const int mb = std::max(0, n - std::min((P + 1) * bw, n) + 1);
+        const int T = (mb + 63) / 64;
+        acc += T * (T + 1) / 2;
+      }
+    }
+    pr[count] = acc;
+  }
+  // buf: n[count], state[4 count], prefix[steps * (count + 1)], wsoff (int64)
+  const size_t ints = count + 4 * (size_t)count + prefix.size() + 64;
+  const size_t bytes = ints * sizeof(int) + count * sizeof(int64_t) + 256;
+  int rc = h->reserve_buf(&p.buf, &p.buf_bytes, bytes, "batch schedule");
+  if (rc != BBK_OK) return rc;
+  rc = h->reserve_buf(&p.ws, &p.ws_bytes, std::max<size_t>(1, wel) * elem_bytes, "batch workspace");
+  if (rc != BBK_OK) return rc;
+  std::vector<char> host(bytes, 0);
+  int* hi = reinterpret_cast<int*>(host.data());
+  std::copy(h_n, h_n + count, hi);
+  std::copy(prefix.begin(), prefix.end(), hi + 5 * (size_t)count);
+  const size_t off64 = ((ints * sizeof(int)) + 255) / 256 * 256;
+  std::memcpy(host.data() + off64, wsoff.data(), count * sizeof(int64_t));
+  hipError_t e = hipMemcpyAsync(p.buf, host.data(), bytes, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "upload batch schedule");
+  p.count = count;
+  p.elem = elem_bytes;
+  p.sizes.assign(h_n, h_n + count);
+  p.widths = wt;
+  p.lockstep = true;
+  p.steps = steps;
+  p.step_stride = sstride;
+  p.ws_elems = wel;
+  p.step_tiles.resize(steps);
+  for (int P = 0; P < steps; ++P) p.step_tiles[P] = prefix[(size_t)P * sstride + count];
+  p.grid = count;
+  p.nt = 256;
+  return BBK_OK;
+}
+
 // Sort largest first, upload sizes and queue order, size the global W
 // slices for matrices whose panel rows do not fit LDS.
 int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
@@ -424,6 +588,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return h_n[a] > h_n[b]; });
   int nmax = 0;
   for (int i = 0; i < count; ++i) nmax = std::max(nmax, h_n[i]);
+  if (nmax > kBatchLockstepN) return prepare_lockstep(h, count, h_n, elem_bytes, wt);
   const int nt = wt.nt[batch_class(nmax)] == 64 ? 64 : 256;
   const int lds_elems = batch_lds_bytes(nt) / elem_bytes;
   int64_t stride = 0;
@@ -457,6 +622,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   p.ws_stride = stride;
   p.grid = grid;
   p.nt = nt;
+  p.lockstep = false;
   return BBK_OK;
 }
 
@@ -470,10 +636,43 @@ static bool plan_matches(const BatchPlan& p, int count, const int* h_n, int elem
 
 // Queue reset (fill kernel: graph-capture safe) + the persistent launch.
 template <typename T>
+static int lockstep_launch(Handle* h, T* const* A, const int64_t* lda, int* const* perm,
+                           int* const* piv) {
+  const BatchPlan& p = h->batch;
+  int* base = reinterpret_cast<int*>(p.buf);
+  const size_t ints = p.count + 4 * (size_t)p.count + (size_t)p.steps * p.step_stride + 64;
+  StepArgs<T> a;
+  a.count = p.count;
+  a.ns = base;
+  a.A = A;
+  a.lda = lda;
+  a.perm = perm;
+  a.piv = piv;
+  a.ws = reinterpret_cast<T*>(p.ws);
+  a.wsoff = reinterpret_cast<const int64_t*>(reinterpret_cast<char*>(p.buf) +
+                                             ((ints * sizeof(int)) + 255) / 256 * 256);
+  a.st = base + p.count;
+  a.widths = p.widths;
+  const int* prefix_all = base + 5 * (size_t)p.count;
+  for (int P = 0; P < p.steps; ++P) {
+    a.step = P;
+    a.prefix = prefix_all + (size_t)P * p.step_stride;
+    hipLaunchKernelGGL(k_step_panel<T>, dim3(p.count), dim3(256), 0, h->stream, a);
+    const int tiles = p.step_tiles[P];
+    if (tiles > 0)
+      hipLaunchKernelGGL(k_step_update<T>, dim3(tiles), dim3(256), 0, h->stream, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h->hip_fail(e, "lockstep batched launch");
+  return BBK_OK;
+}
+
+template <typename T>
 int batched_launch(Handle* h, T* const* A, const int64_t* lda, int* const* perm,
                    int* const* piv) {
   const BatchPlan& p = h->batch;
   if (p.count == 0) return BBK_OK;
+  if (p.lockstep) return lockstep_launch<T>(h, A, lda, perm, piv);
   int* base = reinterpret_cast<int*>(p.buf);
   BatchArgs<T> a;
   a.count = p.count;

@@ -16,6 +16,10 @@ constexpr int kBatchMaxWidth = 64;  // widest panel a batched matrix may use
 // two four-wavefront teams fit a CU's 160 KB.
 constexpr int kBatchLdsBytesSmall = 24 * 1024;
 constexpr int kBatchLdsBytes = 76 * 1024;
+// Batches whose largest matrix exceeds this run in lockstep: per panel step
+// one launch factors every matrix's panel (one workgroup each), a second
+// applies all trailing updates as MFMA tiles spread over the whole GPU.
+constexpr int kBatchLockstepN = 512;
 __host__ __device__ constexpr int batch_lds_bytes(int nt) {
   return nt == 64 ? kBatchLdsBytesSmall : kBatchLdsBytes;
 }
@@ -51,6 +55,11 @@ struct BatchPlan {
   int64_t ws_stride = 0;
   int grid = 0;
   int nt = 64;          // threads per matrix team
+  bool lockstep = false;
+  int steps = 0;        // lockstep: panel steps (largest ceil(n / width))
+  int64_t step_stride = 0;  // lockstep: ints per step in the tile prefix table
+  int64_t ws_elems = 0;
+  std::vector<int> step_tiles;  // lockstep: trailing tiles per step
 };
 
 }  // namespace bbk

@@ -11,146 +11,11 @@
 #include <climits>
 
 #include "bbk_kernels.hpp"
+#include "bbk_update.hpp"
 
 namespace bbk {
 
 // ---------------------------------------------------------------------
-// Trailing update C(i,j) -= sum_q U(s+q, i) W(q, j) over the upper
-// triangle of [e, n)^2, e = s + width (UpdateTrailingBlocks +
-// kernels::UpdateTrailingBlock, src/block_ldlt_decomposition.cc:706-744,
-// src/kernels.cc:67-91).  Square output tiles per workgroup, four
-// wavefronts, 16x16x4 MFMA, K staged through LDS in chunks.
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef double acc_t __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // output element v of a lane sits in row (lane / 16) + 4 v (measured)
-  static __device__ __forceinline__ int row(int lane, int v) {
-    return (lane >> 4) + 4 * v;
-  }
-};
-template <>
-struct Mfma<float> {
-  typedef float acc_t __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  // output element v of a lane sits in row 4 (lane / 16) + v (measured)
-  static __device__ __forceinline__ int row(int lane, int v) {
-    return 4 * (lane >> 4) + v;
-  }
-};
-
-__device__ __forceinline__ void tri_tile(int idx, int* I, int* J) {
-  int jj = static_cast<int>((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-  while (jj * (jj + 1) / 2 > idx) --jj;
-  while ((jj + 1) * (jj + 2) / 2 <= idx) ++jj;
-  *J = jj;
-  *I = idx - jj * (jj + 1) / 2;
-}
-
-// The accumulators start from the C tile itself and the U operand is
-// negated while it is staged, so the MFMA chain computes C - U^T W in
-// place; K is staged through two LDS buffers so the next chunk's global
-// loads overlap the current chunk's MFMAs.  TM x TM output tile per
-// workgroup, four wavefronts of (TM/2) x (TM/2); MINB workgroups per CU;
-// with XCD, consecutive tiles (sharing U and W strips) go to one XCD.
-template <typename T, int TM, int KC>
-__device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
-                                            const T* __restrict__ Wg, int64_t ldw, int s,
-                                            int bw, int n, int idx, T (*Us)[KC * (TM + 4)],
-                                            T (*Ws)[KC * (TM + 4)]) {
-  constexpr int LDS_LD = TM + 4;
-  constexpr int PER = KC * TM / 256;  // staged elements per thread per operand
-  constexpr int WT = TM / 2;          // wavefront tile
-  constexpr int TI = WT / 16;         // 16 x 16 MFMA blocks per wavefront side
-  typedef typename Mfma<T>::acc_t acc_t;
-  const int e = s + bw;
-  const int m = n - e;
-  if (m <= 0 || bw <= 0) return;
-  const int tiles = (m + TM - 1) / TM;
-  int I, J;
-  tri_tile(idx, &I, &J);
-  if (J >= tiles) return;
-  const int i0 = e + I * TM, j0 = e + J * TM;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-
-  acc_t acc[TI][TI];
-#pragma unroll
-  for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < TI; ++tj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * WT + ti * 16 + Mfma<T>::row(lane, v);
-        const int jj = j0 + wc * WT + tj * 16 + (lane & 15);
-        acc[ti][tj][v] = (i < n && jj < n && jj >= i) ? A[(int64_t)i * lda + jj] : T(0);
-      }
-
-  T ru[PER], rw[PER];
-  auto fetch = [&](int k0) {
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int t = tid + 256 * u;
-      const int q = t / TM, cidx = t % TM;
-      const int kq = k0 + q;
-      ru[u] = (kq < bw && i0 + cidx < n) ? -A[(int64_t)(s + kq) * lda + i0 + cidx] : T(0);
-      rw[u] = (kq < bw && j0 + cidx < n) ? Wg[(int64_t)kq * ldw + j0 + cidx] : T(0);
-    }
-  };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int t = tid + 256 * u;
-      const int q = t / TM, cidx = t % TM;
-      Us[buf][q * LDS_LD + cidx] = ru[u];
-      Ws[buf][q * LDS_LD + cidx] = rw[u];
-    }
-  };
-
-  const int nk = (bw + KC - 1) / KC;
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int kc = 0; kc < nk; ++kc) {
-    const int buf = kc & 1;
-    if (kc + 1 < nk) fetch((kc + 1) * KC);
-#pragma unroll
-    for (int k4 = 0; k4 < KC; k4 += 4) {
-      T a[TI], b[TI];
-      const int kr = k4 + (lane >> 4);
-#pragma unroll
-      for (int t = 0; t < TI; ++t) {
-        a[t] = Us[buf][kr * LDS_LD + wr * WT + t * 16 + (lane & 15)];
-        b[t] = Ws[buf][kr * LDS_LD + wc * WT + t * 16 + (lane & 15)];
-      }
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < TI; ++tj)
-          acc[ti][tj] = Mfma<T>::mma(a[ti], b[tj], acc[ti][tj]);
-    }
-    if (kc + 1 < nk) stash(buf ^ 1);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < TI; ++tj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * WT + ti * 16 + Mfma<T>::row(lane, v);
-        const int jj = j0 + wc * WT + tj * 16 + (lane & 15);
-        if (i < n && jj < n && jj >= i) A[(int64_t)i * lda + jj] = acc[ti][tj][v];
-      }
-}
-
 // Deferred interchanges of the rows above the panel (the reference swaps
 // them immediately, src/block_ldlt_decomposition.cc:472-491): the panel's
 // interchange sequence (a_q, r_q), q < ns, is composed into one gather per

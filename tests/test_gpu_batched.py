@@ -213,3 +213,51 @@ def test_batched_autotuner(oracle, handle):
     finally:
         handle.set_batched_widths(torch.float64, saved)
         handle.set_batched_team_sizes(torch.float64, saved_nt)
+
+
+@pytest.mark.parametrize("bw", [3, 32])
+def test_batched_lockstep_matches_oracle(oracle, handle, bw):
+    """Batches with a matrix above 512 run in lockstep (one panel launch +
+    one batched MFMA update launch per panel step); tiny matrices in the same
+    batch finish early; narrow panels defer many 2x2 pivots."""
+    import torch
+    saved = handle.batched_widths(torch.float64)
+    handle.set_batched_widths(torch.float64, [bw] * 8)
+    try:
+        ns = [600, 1, 2, 17, 64, 300, 533]
+        mats = handle.gen_synth_batched(ns, seed=777 + bw, scale=1.0, zero_diag_stride=7,
+                                        dtype=torch.float64, pad=5)
+        hosts = [host_matrix(m) for m in mats]
+        outs = factor_batch(handle, oracle, mats, torch.float64)
+        for A, g in zip(hosts, outs):
+            check(oracle, A, g, bw)
+        assert any((g.piv == 2).any() for g in outs)
+    finally:
+        handle.set_batched_widths(torch.float64, saved)
+
+
+def test_batched_lockstep_fp32_and_graph(oracle, handle):
+    import torch
+    ns = [700, 520, 90, 256]
+    src = handle.gen_synth_batched(ns, seed=99, zero_diag_stride=7, dtype=torch.float32)
+    eager = factor_batch(handle, oracle, [s.clone() for s in src], torch.float32)
+    widths = handle.batched_widths(torch.float32)
+    for s_, e in zip(src, eager):
+        check(oracle, host_matrix(s_), e, width_for(widths, s_.shape[0]), fp32=True)
+    work = [s.clone() for s in src]
+    perms = [torch.empty(n, dtype=torch.int32, device="cuda") for n in ns]
+    pivs = [torch.empty(n, dtype=torch.int32, device="cuda") for n in ns]
+    g = handle.graph_sytrf_batched(work, perms, pivs)
+    try:
+        for rep in range(2):
+            for w, s_ in zip(work, src):
+                w.copy_(s_)
+            torch.cuda.synchronize()
+            g.launch()
+            handle.synchronize()
+            for w, p, v, e in zip(work, perms, pivs, eager):
+                n = w.shape[0]
+                assert np.array_equal(np.triu(w[:, :n].double().cpu().numpy()), e.compact), rep
+                assert np.array_equal(v.cpu().numpy(), e.piv)
+    finally:
+        g.close()
