@@ -112,11 +112,14 @@ __device__ void solve32(int n, const float* __restrict__ F, int64_t ldf,
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = 0.0f;
       const int jend = (cut && i == j0 + jn) ? jn - 1 : jn;
-#pragma unroll 8
-      for (int jj = 0; jj < jend; ++jj) {
-        const float l = F[(int64_t)(j0 + jj) * ldf + i];
+      // all of the row's loads in flight before the multiply-adds
+      float lv[kRB];
 #pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] += l * z[(j0 + jj) * NR + q];
+      for (int jj = 0; jj < kRB; ++jj) lv[jj] = jj < jend ? F[(int64_t)(j0 + jj) * ldf + i] : 0.0f;
+#pragma unroll
+      for (int jj = 0; jj < kRB; ++jj) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] += lv[jj] * z[(j0 + jj) * NR + q];
       }
 #pragma unroll
       for (int q = 0; q < NR; ++q) z[i * NR + q] -= acc[q];
@@ -161,6 +164,7 @@ __device__ void solve32(int n, const float* __restrict__ F, int64_t ldf,
       double acc[NR];
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] = 0.0;
+#pragma unroll 4
       for (int j = i0 + in + lane; j < n; j += 64) {
         const float l = j == jskip ? 0.0f : F[(int64_t)i * ldf + j];
 #pragma unroll
@@ -200,6 +204,64 @@ __device__ void solve32(int n, const float* __restrict__ F, int64_t ldf,
         for (int q = 0; q < NR; ++q) z[i * NR + q] = v[q];
     }
     __syncthreads();
+  }
+}
+
+// y[i NR + q] += sum_j A(i, j) x_j[q] for the symmetric A stored in the
+// upper triangle (ABS: |A(i, j)| into y[i NR], x ignored).  Wavefront w
+// takes row blocks of RB rows; per 64-column chunk each lane (one column)
+// accumulates the block's row-part products in registers and its column
+// part for the chunk, so LDS atomics happen once per RB rows per column and
+// once per row at the end of the block (not per element).
+template <int NR, bool ABS>
+__device__ __forceinline__ void team_symv(int n, const double* __restrict__ A, int64_t lda,
+                                          const double* __restrict__ X, int64_t ldx,
+                                          double* y) {
+  constexpr int QN = ABS ? 1 : NR;
+  constexpr int RB = QN <= 2 ? 16 : 8;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int jb = w * RB; jb < n; jb += (kRT / 64) * RB) {
+    double xj[RB][QN], racc[RB][QN];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        racc[r][q] = 0.0;
+        xj[r][q] = ABS ? 1.0 : (jb + r < n ? X[(int64_t)(jb + r) * ldx + q] : 0.0);
+      }
+    for (int ib = jb; ib < n; ib += 64) {
+      const int i = ib + lane;
+      const bool vi = i < n;
+      double xi[QN], cacc[QN];
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        xi[q] = ABS ? 1.0 : (vi ? X[(int64_t)i * ldx + q] : 0.0);
+        cacc[q] = 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int j = jb + r;
+        double a = (vi && i >= j) ? A[(int64_t)j * lda + i] : 0.0;  // i >= j implies j < n
+        if (ABS) a = fabs(a);
+#pragma unroll
+        for (int q = 0; q < QN; ++q) racc[r][q] += a * xi[q];
+        if (i > j) {
+#pragma unroll
+          for (int q = 0; q < QN; ++q) cacc[q] += a * xj[r][q];
+        }
+      }
+      if (vi) {
+#pragma unroll
+        for (int q = 0; q < QN; ++q) atomicAdd(&y[i * NR + q], cacc[q]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const double sr = wave_sum(racc[r][q]);
+        if (lane == 0 && jb + r < n) atomicAdd(&y[(jb + r) * NR + q], sr);
+      }
   }
 }
 
@@ -256,16 +318,7 @@ __global__ __launch_bounds__(kRT) void k_batched_refine(RefineArgs a) {
         if (want_norm) {
           for (int i = t; i < n; i += kRT) y[i * NR] = 0.0;
           __syncthreads();
-          for (int j = w; j < n; j += 4) {
-            double ra = 0.0;
-            for (int i = j + lane; i < n; i += 64) {
-              const double v = fabs(A[(int64_t)j * lda + i]);
-              ra += v;
-              if (i > j) atomicAdd(&y[i * NR], v);
-            }
-            ra = wave_sum(ra);
-            if (lane == 0) atomicAdd(&y[j * NR], ra);
-          }
+          team_symv<NR, true>(n, A, lda, nullptr, 0, y);
           __syncthreads();
           double mx = 0.0;
           for (int i = t; i < n; i += kRT) mx = (mx < y[i * NR]) ? y[i * NR] : mx;
@@ -275,27 +328,7 @@ __global__ __launch_bounds__(kRT) void k_batched_refine(RefineArgs a) {
         }
         for (int i = t; i < n * NR; i += kRT) y[i] = 0.0;
         __syncthreads();
-        for (int j = w; j < n; j += 4) {
-          double xj[NR], acc[NR];
-  #pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            xj[q] = Xm[(int64_t)j * ldx + q];
-            acc[q] = 0.0;
-          }
-          for (int i = j + lane; i < n; i += 64) {
-            const double v = A[(int64_t)j * lda + i];
-  #pragma unroll
-            for (int q = 0; q < NR; ++q) {
-              acc[q] += v * Xm[(int64_t)i * ldx + q];
-              if (i > j) atomicAdd(&y[i * NR + q], v * xj[q]);
-            }
-          }
-  #pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            const double s = wave_sum(acc[q]);
-            if (lane == 0) atomicAdd(&y[j * NR + q], s);
-          }
-        }
+        team_symv<NR, false>(n, A, lda, Xm, ldx, y);
         __syncthreads();
         // r = b - y (kept in y), eta
         double rmax[NR], xmax[NR], bmax[NR];
