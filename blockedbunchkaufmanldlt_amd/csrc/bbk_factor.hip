@@ -76,7 +76,7 @@ __device__ __forceinline__ void swap_rows(T* __restrict__ A, int64_t lda,
 // tiles (XCD-contiguous order), the rest apply the panel's interchanges to
 // RPG rows each of the finished rows above the panel.  The two regions are
 // disjoint (rows >= e against rows < s), so both run concurrently.
-template <typename T, int TM, int MINB, int KC, int RPG>
+template <typename T, int TM, int MINB, int KC, int RPG, bool SUPER = true>
 __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
     T* __restrict__ A, int64_t lda, const T* __restrict__ Wg, int64_t ldw,
     const int* __restrict__ state, const int* __restrict__ swaps, int n, int gemm_blocks) {
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
     const int nb = gemm_blocks, per = (nb + 7) / 8, x = idx & 7, r = idx >> 3;
     const int full = nb - 8 * (per - 1);  // XCDs that get `per` tiles
     idx = x < full ? x * per + r : full * per + (x - full) * (per - 1) + r;
-    update_tile<T, TM, KC>(A, lda, Wg, ldw, s, bw, n, idx, Us, Ws);
+    update_tile<T, TM, KC, SUPER>(A, lda, Wg, ldw, s, bw, n, idx, Us, Ws);
   } else {
     const int ns = state[PanelState::kNumSwaps];
     const int p0 = (idx - gemm_blocks) * RPG;

@@ -51,7 +51,36 @@ __device__ __forceinline__ void tri_tile(int idx, int* I, int* J) {
 // loads overlap the current chunk's MFMAs.  TM x TM output tile per
 // workgroup, four wavefronts of (TM/2) x (TM/2); MINB workgroups per CU;
 // with XCD, consecutive tiles (sharing U and W strips) go to one XCD.
-template <typename T, int TM, int KC>
+// Tile idx -> (I, J), I <= J < T, in 8 x 8 supertiles: supertile columns
+// in order, within a column the full supertiles top to bottom, then the
+// diagonal one; column-major inside a supertile.  A run of consecutive
+// indices (one XCD's share) then reuses a few U and W strip pieces from L2
+// instead of streaming whole strips.
+__device__ __forceinline__ void supertile_of(int idx, int T, int* I, int* J) {
+  constexpr int S = 8;
+  int bj = 0, base = 0, C = 0;
+  for (;;) {
+    C = min(S, T - bj * S);
+    const int col = bj * S * C + C * (C + 1) / 2;
+    if (idx < base + col) break;
+    base += col;
+    ++bj;
+  }
+  const int l = idx - base;
+  const int full = bj * S * C;
+  if (l < full) {
+    const int bi = l / (S * C), ll = l % (S * C);
+    *I = bi * S + ll % S;
+    *J = bj * S + ll / S;
+  } else {
+    int li, lj;
+    tri_tile(l - full, &li, &lj);
+    *I = bj * S + li;
+    *J = bj * S + lj;
+  }
+}
+
+template <typename T, int TM, int KC, bool SUPER = true>
 __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
                                             const T* __restrict__ Wg, int64_t ldw, int s,
                                             int bw, int n, int idx, T (*Us)[KC * (TM + 4)],
@@ -66,8 +95,13 @@ __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
   if (m <= 0 || bw <= 0) return;
   const int tiles = (m + TM - 1) / TM;
   int I, J;
-  tri_tile(idx, &I, &J);
-  if (J >= tiles) return;
+  if (SUPER) {
+    if (idx >= tiles * (tiles + 1) / 2) return;  // the grid covers a size bound
+    supertile_of(idx, tiles, &I, &J);
+  } else {
+    tri_tile(idx, &I, &J);
+    if (J >= tiles) return;
+  }
   const int i0 = e + I * TM, j0 = e + J * TM;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
