@@ -99,8 +99,9 @@ static size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
 }
 
 int default_panel_width(int n, int elem_bytes) {
+  (void)elem_bytes;
   if (n <= 256) return 32;
-  if (n <= 1024 || elem_bytes == 8) return 64;
+  if (n <= 1024) return 64;
   return 128;
 }
 

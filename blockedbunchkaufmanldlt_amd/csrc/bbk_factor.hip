@@ -29,28 +29,7 @@ __device__ __forceinline__ void swap_rows(T* __restrict__ A, int64_t lda,
   const int ne = 2 * ns;
   for (int t = threadIdx.x; t < ne; t += blockDim.x) cols[t] = swaps[t];
   __syncthreads();
-  // first occurrence of each column: its slot in the composed gather
-  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
-    const int cval = cols[t];
-    int f = t;
-    for (int u = 0; u < t; ++u)
-      if (cols[u] == cval) {
-        f = u;
-        break;
-      }
-    slot[t] = f;
-    content[t] = cval;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < ns; ++q) {
-      const int sa = slot[2 * q], sb = slot[2 * q + 1];
-      const int tmp = content[sa];
-      content[sa] = content[sb];
-      content[sb] = tmp;
-    }
-  }
-  __syncthreads();
+  compose_swaps(cols, ns, slot, content);
   const int t0 = threadIdx.x, t1 = threadIdx.x + 256;
   const int d0 = t0 < ne ? cols[t0] : 0, s0 = t0 < ne ? content[slot[t0]] : 0;
   const int d1 = t1 < ne ? cols[t1] : 0, s1 = t1 < ne ? content[slot[t1]] : 0;
@@ -62,6 +41,7 @@ __device__ __forceinline__ void swap_rows(T* __restrict__ A, int64_t lda,
     v0[u] = (row < p1 && t0 < ne) ? R[s0] : T(0);
     v1[u] = (row < p1 && t1 < ne) ? R[s1] : T(0);
   }
+  wait_vm();
   __syncthreads();  // every read of the rows precedes every write
 #pragma unroll
   for (int u = 0; u < RPG; ++u) {

@@ -28,16 +28,24 @@ struct __attribute__((aligned(16))) XRec {
   double v;
 };
 
-// Widest panel whose W rows (width + 1) x 256 fit the 160 KB LDS.
+// Panel width per type.  The panel's W rows live in LDS, (kLdsRows + 1) x
+// 256 elements (the 160 KB LDS bound); a binary64 panel is twice as wide:
+// once it passes row kRegRows, rows [0, kRegRows) move to registers (one
+// column per thread, 2 kRegRows VGPRs) and the LDS rows are reused for the
+// rest, so the trailing update runs at rank 128 in both types.
 template <typename T>
 struct PanelCfg;
 template <>
 struct PanelCfg<double> {
-  static constexpr int kMaxWidth = 64;   // 65 x 256 x 8 B = 130 KB
+  static constexpr int kLdsRows = 64;  // 65 x 256 x 8 B = 130 KB
+  static constexpr int kRegRows = 64;
+  static constexpr int kMaxWidth = kLdsRows + kRegRows;
 };
 template <>
 struct PanelCfg<float> {
-  static constexpr int kMaxWidth = 128;  // 129 x 256 x 4 B = 129 KB
+  static constexpr int kLdsRows = 128;  // 129 x 256 x 4 B = 129 KB
+  static constexpr int kRegRows = 0;
+  static constexpr int kMaxWidth = kLdsRows;
 };
 
 // Per-panel device state (ints).
@@ -81,6 +89,37 @@ __device__ __forceinline__ double xwait(const XRec* p, int tag, int* aux) {
   double v;
   while (!xget(p, tag, aux, &v)) __builtin_amdgcn_s_sleep(1);
   return v;
+}
+
+// An interchange sequence (cols[2q], cols[2q+1]), q < ns, held in LDS,
+// composed into one gather over its ne = 2 ns entries: afterwards position
+// cols[t] takes the value found before the sequence at position
+// content[slot[t]] (slot[t] is the first entry naming the same position).
+// Called by every thread of the workgroup.
+__device__ __forceinline__ void compose_swaps(const int* cols, int ns, int* slot,
+                                              int* content) {
+  const int ne = 2 * ns;
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int cval = cols[t];
+    int f = t;
+    for (int u = 0; u < t; ++u)
+      if (cols[u] == cval) {
+        f = u;
+        break;
+      }
+    slot[t] = f;
+    content[t] = cval;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < ns; ++q) {
+      const int sa = slot[2 * q], sb = slot[2 * q + 1];
+      const int tmp = content[sa];
+      content[sa] = content[sb];
+      content[sb] = tmp;
+    }
+  }
+  __syncthreads();
 }
 
 // Wait until this wave's outstanding vector memory operations completed.

@@ -3,12 +3,13 @@
 // One cooperative launch factors one panel of up to b columns starting at
 // s (read from device state).  Workgroup g owns the active columns
 // [s + 256 g, s + 256 g + 256) of rows [s, n); thread t owns one column and
-// keeps that column of the panel's unscaled rows W = D L^T in LDS.
+// keeps that column of the panel's unscaled rows W = D L^T in LDS (binary64:
+// the first 64 rows move to registers once the panel passes them).
 //
 // Per pivot k (local index kk) the workgroups meet at most twice:
 //   exchange 1  after W(k, :) = A(k, :) - L(k, panel) W(panel, :) and the
-//               local argmax; carries (omega_g, r_g), W(panel, r_g), the
-//               W columns k, k+1, k+2 and a few scalars;
+//               local argmax; carries (omega_g, r_g), the W columns k, k+1,
+//               k+2 and a few scalars;
 //   exchange 2  (only when the Bunch-Kaufman test needs the candidate row r)
 //               after the candidate row and its local max sigma_g; carries
 //               sigma_g and the candidate values at k, k+1, k+2, r.
@@ -18,7 +19,8 @@
 // one store per writer and one polling read per reader, with no atomics.
 // Two record areas alternate by exchange parity.  Everything the next
 // pivot needs (its L column, its A row) is rebuilt locally from published
-// values, so no third synchronization per pivot is needed.
+// values, so no third synchronization per pivot is needed; the candidate's
+// L row is column r of the finished U rows, read back from A.
 //
 // Arithmetic: pivot tests, 2x2 inverse and L = D^-1 W in binary64 without
 // contraction (src/block_ldlt_decomposition.cc:21-26,369-390,633-704).
@@ -66,20 +68,36 @@ __device__ __forceinline__ void stc(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// sum_q L[q] * W(q, own column), q < kk; four accumulators keep several
-// LDS reads in flight.
-template <typename T, int NT>
-__device__ __forceinline__ T panel_dot(const T* L, const T* Wl, int tid, int kk) {
+// sum_q L[q] * W(q, own column), q < kk.  Rows [0, RR) come from the
+// registers wr once base == RR; rows [base, kk) from LDS, row q at
+// (q - base) * WS.  Four accumulators keep several loads in flight.
+template <typename T, int WS, int RR, int NR>
+__device__ __forceinline__ T panel_dot(const T* L, const T* Wl, const T (&wr)[NR], int tid,
+                                       int kk, int base) {
   T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  if constexpr (RR > 0) {
+    if (base > 0) {
+#pragma unroll
+      for (int q = 0; q < RR; q += 4) {
+        a0 += L[q] * wr[q];
+        a1 += L[q + 1] * wr[q + 1];
+        a2 += L[q + 2] * wr[q + 2];
+        a3 += L[q + 3] * wr[q + 3];
+      }
+    }
+  }
+  const T* Wc = Wl + tid;
+  const T* Lb = L + base;
+  const int cnt = kk - base;
   int q = 0;
 #pragma unroll 2
-  for (; q + 4 <= kk; q += 4) {
-    a0 += L[q] * Wl[q * NT + tid];
-    a1 += L[q + 1] * Wl[(q + 1) * NT + tid];
-    a2 += L[q + 2] * Wl[(q + 2) * NT + tid];
-    a3 += L[q + 3] * Wl[(q + 3) * NT + tid];
+  for (; q + 4 <= cnt; q += 4) {
+    a0 += Lb[q] * Wc[q * WS];
+    a1 += Lb[q + 1] * Wc[(q + 1) * WS];
+    a2 += Lb[q + 2] * Wc[(q + 2) * WS];
+    a3 += Lb[q + 3] * Wc[(q + 3) * WS];
   }
-  for (; q < kk; ++q) a0 += L[q] * Wl[q * NT + tid];
+  for (; q < cnt; ++q) a0 += Lb[q] * Wc[q * WS];
   return (a0 + a1) + (a2 + a3);
 }
 
@@ -106,35 +124,28 @@ enum PubId {
   kNumPub = 16
 };
 
-// L(r, s + q) from W(q, r) (the candidate's column) and row q's scaling
-// record; `partner` is W(q + 1, r) for a 2x2 start, W(q - 1, r) for its
-// second row.  Same operations as ucol().
-__device__ __forceinline__ double ucol_pair(int ty, double c0, double c1, double self,
-                                            double partner) {
-  if (ty == 1) return div_or_zero(self, c0);
-  double u0, u1;
-  if (ty == 2) {
-    apply_2x2_nc(c0, 0.0, c1, self, partner, &u0, &u1);
-    return u0;
-  }
-  apply_2x2_nc(0.0, c1, c0, partner, self, &u0, &u1);
-  return u1;
-}
-
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__ A, PanelArgs p) {
   constexpr int NT = kPanelThreads;
   constexpr int BM = PanelCfg<T>::kMaxWidth;
-  __shared__ T Wl[(BM + 1) * NT];
+  constexpr int RR = PanelCfg<T>::kRegRows;
+  constexpr int LR = PanelCfg<T>::kLdsRows;
+  constexpr int NR = RR > 0 ? RR : 1;
+  constexpr int WS = NT;  // LDS row stride of W
+  // W(q, own column): q < base (= 0 or RR) in registers wr[q], the rest in
+  // LDS row q - base
+  __shared__ T Wl[(LR + 1) * WS];
   __shared__ T Lbuf[2][BM + 2];  // L(k, panel) of this pivot / of the next
   __shared__ T Lr[BM + 2];
-  __shared__ double colR[BM + 2], colK[BM + 2], colK1[BM + 2], colK2[BM + 2];
+  __shared__ double colK[BM + 2], colK1[BM + 2], colK2[BM + 2];
   __shared__ int dt[BM + 2];
   __shared__ double dc0[BM + 2], dc1[BM + 2];
   __shared__ double xval[kMaxPanelGroupsLds];
   __shared__ double pubv[kNumPub];
   __shared__ double red_v[2 * (NT / 64)];
   __shared__ int red_i[2 * (NT / 64)];
+  // this panel's interchanges (workgroup 0), applied to perm at the end
+  __shared__ int swl[2 * (BM + 1)], sw_slot[2 * (BM + 1)], sw_content[2 * (BM + 1)];
 
   const int n = p.n;
   const int64_t lda = p.lda;
@@ -160,8 +171,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   const bool own = j < n;
   const int bt = min(p.b, n - s);
   T* Wg = reinterpret_cast<T*>(p.Wg);
-  // record areas: [slots G][pubs 16][pubcols 3 BM][colR G BM], two parities
-  const int PS = G + kNumPub + 3 * BM + G * BM;
+  // record areas: [slots G][pubs 16][pubcols 3 BM], two parities
+  const int PS = G + kNumPub + 3 * BM;
   XRec* const xbase = p.xrec;
   int ex = p.epoch;  // exchange number = record tag
   int par = 0;       // reduction scratch half
@@ -202,6 +213,34 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
 
   T arow = own ? ldc(&A[(int64_t)s * lda + j]) : T(0);
   int kk = 0, nswap = 0;
+  T wr[NR];
+  int base = 0;
+  // Once base == RR, workgroup 0 keeps a copy of the register rows of the
+  // panel's own columns s + RR + c, c < LR + 1 (the only columns whose W
+  // entries are published: k, k+1, k+2) in its LDS columns [0, RR), which
+  // belong to the finished columns [s, s + RR): entry (q, c) in LDS row c,
+  // column q.
+  auto pb = [&](int q, int c) -> T& { return Wl[c * WS + q]; };
+  // W(q, col), q < kk, for a column of this group (col is a panel column
+  // when q < base)
+  auto wcol = [&](int q, int col) -> T {
+    if (RR > 0 && base > 0 && q < RR) return pb(q, col - s - RR);
+    return Wl[(q - base) * WS + (col - c0)];
+  };
+  // a column exchange of W: own column takes src[q], q < min(kk, base)
+  auto take_col = [&](const double* src) {
+    if constexpr (RR > 0) {
+      if (base > 0) {
+        const int c = j - s - RR;
+        const bool panel_col = gi == 0 && c >= 0 && c < LR + 1;
+#pragma unroll
+        for (int q = 0; q < RR; ++q) {
+          wr[q] = static_cast<T>(src[q]);
+          if (panel_col) pb(q, c) = wr[q];
+        }
+      }
+    }
+  };
 #if defined(BBK_PANEL_PROFILE)
   // Phase timing with the 100 MHz device clock (build-time option only).
   unsigned long long tp[8] = {0}, pacc[8] = {0};
@@ -220,14 +259,33 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     BBK_PT(0);
     const int k = s + kk;
     const bool has1 = k + 1 < n, has2 = k + 2 < n;
+    // column k+2's W entries are published only when a pivot at k+2 can
+    // follow in this panel
+    const bool use2 = has2 && kk + 2 < bt;
+    if constexpr (RR > 0) {
+      if (base == 0 && kk >= RR) {
+        // rows [0, RR) to registers; a row RR (2x2 across the boundary)
+        // moves to LDS row 0
+#pragma unroll
+        for (int q = 0; q < RR; ++q) wr[q] = Wl[q * WS + tid];
+        for (int q = RR; q < kk; ++q) Wl[(q - RR) * WS + tid] = Wl[q * WS + tid];
+        base = RR;
+        __syncthreads();
+        if (gi == 0 && tid >= RR && tid < RR + LR + 1) {
+#pragma unroll
+          for (int q = 0; q < RR; ++q) pb(q, tid - RR) = wr[q];
+        }
+        __syncthreads();
+      }
+    }
     T* const Lk = Lbuf[lp];
     T* const Lnext = Lbuf[lp ^ 1];
     // ---- phase A: W(k, j) for own j >= k, local argmax over j > k -------
     const bool active = own && j >= k;
     T w = T(0);
     if (active) {
-      w = arow - panel_dot<T, NT>(Lk, Wl, tid, kk);
-      Wl[kk * NT + tid] = w;
+      w = arow - panel_dot<T, WS, RR>(Lk, Wl, wr, tid, kk, base);
+      Wl[(kk - base) * WS + tid] = w;
     }
     double v = 0.0;
     int vi = INT_MAX;
@@ -250,7 +308,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       // slot: local argmax column and its signed value W(k, r_g)
       if (tid == 0)
         post(ar + gi, ex, vi,
-             vi != INT_MAX ? static_cast<double>(Wl[kk * NT + (vi - c0)]) : 0.0);
+             vi != INT_MAX ? static_cast<double>(Wl[(kk - base) * WS + (vi - c0)]) : 0.0);
       XRec* pubs = ar + G;
       XRec* pcols = ar + G + kNumPub;
       if (j == k) {
@@ -264,14 +322,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k + 2) post(pubs + kPubWk2, ex, 0, static_cast<double>(w));
       if (tid < kk) {
         const int q = tid;
-        if (is_mine(k)) post(pcols + q, ex, q, static_cast<double>(Wl[q * NT + (k - c0)]));
+        if (is_mine(k)) post(pcols + q, ex, q, static_cast<double>(wcol(q, k)));
         if (has1 && is_mine(k + 1))
-          post(pcols + BM + q, ex, q, static_cast<double>(Wl[q * NT + (k + 1 - c0)]));
-        if (has2 && is_mine(k + 2))
-          post(pcols + 2 * BM + q, ex, q, static_cast<double>(Wl[q * NT + (k + 2 - c0)]));
-        if (vi != INT_MAX)
-          post(ar + G + kNumPub + 3 * BM + gi * BM + q, ex, q,
-               static_cast<double>(Wl[q * NT + (vi - c0)]));
+          post(pcols + BM + q, ex, q, static_cast<double>(wcol(q, k + 1)));
+        if (use2 && is_mine(k + 2))
+          post(pcols + 2 * BM + q, ex, q, static_cast<double>(wcol(q, k + 2)));
       }
     }
     BBK_PT(2);
@@ -281,7 +336,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     {
       const XRec* ar = area(ex);
       const int npub = 2 + (has1 ? 2 : 0) + (has2 ? 1 : 0);
-      const int ncol = kk * (1 + (has1 ? 1 : 0) + (has2 ? 1 : 0));
+      const int ncol = kk * (1 + (has1 ? 1 : 0) + (use2 ? 1 : 0));
       const int total = G + npub + ncol;
       for (int id = tid; id < total; id += NT) {
         int aux;
@@ -322,27 +377,14 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       kind = 1;  // src/block_ldlt_decomposition.cc:92
     } else {
       // ---- phase B: candidate row r ----------------------------------------
-      const XRec* ar1 = area(ex);
-      const int gstar = (r - s) / NT;
       T a = T(0);
       if (active) a = (j < r) ? ldc(&A[(int64_t)j * lda + r]) : ldc(&A[(int64_t)r * lda + j]);
-      if (tid < kk) {
-        // W(q, r) from r's owner; L(r, s+q) needs the 2x2 partner's value too
-        const XRec* cr = ar1 + G + kNumPub + 3 * BM + gstar * BM;
-        int aux;
-        const double cq = xwait(cr + tid, ex, &aux);
-        const int ty = dt[tid];
-        double partner = 0.0;
-        if (ty == 2)
-          partner = xwait(cr + tid + 1, ex, &aux);
-        else if (ty == -2)
-          partner = xwait(cr + tid - 1, ex, &aux);
-        colR[tid] = cq;
-        Lr[tid] = static_cast<T>(ucol_pair(ty, dc0[tid], dc1[tid], cq, partner));
-      }
+      // L(r, s+q) = U(q, r), q < kk: column r of the finished rows as stored
+      // (the same operations on W(q, r) that ucol() performs)
+      if (tid < kk) Lr[tid] = ldc(&A[(int64_t)(s + tid) * lda + r]);
       __syncthreads();
       if (active) {
-        c = a - panel_dot<T, NT>(Lr, Wl, tid, kk);
+        c = a - panel_dot<T, WS, RR>(Lr, Wl, wr, tid, kk, base);
       }
       double sv = 0.0;
       if (active && j != r) {
@@ -424,24 +466,24 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       size = 1;
     } else if (kind == -1) {
       // symmetric interchange k <-> r (src/block_ldlt_decomposition.cc:414-498)
-      // columns k and r of the finished panel rows, one row per thread
+      // columns k and r of the finished panel rows, one row per thread; W
+      // column r takes column k's (column k is finished: its W entries are
+      // never read again)
       if (tid < kk) {
         const int q = tid;
-        if (is_mine(k)) {
-          Wl[q * NT + (k - c0)] = static_cast<T>(colR[q]);
-          sta(&A[(int64_t)(s + q) * lda + k], Lr[q]);
-        }
+        if (is_mine(k)) sta(&A[(int64_t)(s + q) * lda + k], Lr[q]);
         if (is_mine(r)) {
-          Wl[q * NT + (r - c0)] = static_cast<T>(colK[q]);
+          if (q >= base) Wl[(q - base) * WS + (r - c0)] = static_cast<T>(colK[q]);
           sta(&A[(int64_t)(s + q) * lda + r], Lk[q]);
         }
       }
+      if (j == r) take_col(colK);
       T nw = c;
       if (j == k)
         nw = static_cast<T>(crr);
       else if (j == r)
         nw = static_cast<T>(ck);
-      if (active) Wl[kk * NT + tid] = nw;
+      if (active) Wl[(kk - base) * WS + tid] = nw;
       if (own && j > r)
         sta(&A[(int64_t)r * lda + j], arow);
       else if (own && j > k && j < r)
@@ -455,9 +497,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         p.piv[k] = 1;
       }
       if (gi == 0 && tid == 0) {
-        const int t = p.perm[k];
-        p.perm[k] = p.perm[r];
-        p.perm[r] = t;
+        swl[2 * nswap] = k;
+        swl[2 * nswap + 1] = r;
         p.swaps[2 * nswap] = k;
         p.swaps[2 * nswap + 1] = r;
       }
@@ -487,15 +528,13 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         // columns k+1 and r of the finished panel rows, one row per thread
         if (tid < kk) {
           const int q = tid;
-          if (is_mine(k1)) {
-            Wl[q * NT + (k1 - c0)] = static_cast<T>(colR[q]);
-            sta(&A[(int64_t)(s + q) * lda + k1], Lr[q]);
-          }
+          if (is_mine(k1)) sta(&A[(int64_t)(s + q) * lda + k1], Lr[q]);
           if (is_mine(r)) {
-            Wl[q * NT + (r - c0)] = static_cast<T>(colK1[q]);
+            if (q >= base) Wl[(q - base) * WS + (r - c0)] = static_cast<T>(colK1[q]);
             sta(&A[(int64_t)(s + q) * lda + r], static_cast<T>(ucol(dt, dc0, dc1, colK1, q)));
           }
         }
+        if (j == r) take_col(colK1);
         // trailing matrix: new row/column r take the old row k+1 (= P1)
         if (own && j > r)
           sta(&A[(int64_t)r * lda + j], P1);
@@ -503,9 +542,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
           sta(&A[(int64_t)j * lda + r], P1);
         if (j == r) sta(&A[(int64_t)r * lda + r], static_cast<T>(a11));
         if (gi == 0 && tid == 0) {
-          const int t = p.perm[k1];
-          p.perm[k1] = p.perm[r];
-          p.perm[r] = t;
+          swl[2 * nswap] = k1;
+          swl[2 * nswap + 1] = r;
           p.swaps[2 * nswap] = k1;
           p.swaps[2 * nswap + 1] = r;
         }
@@ -513,7 +551,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
       // new W rows kk (current row) and kk+1 (candidate row), columns
       // k+1 and r exchanged
-      T w0 = active ? Wl[kk * NT + tid] : T(0);
+      T w0 = active ? Wl[(kk - base) * WS + tid] : T(0);
       T w1 = c;
       if (j == k1) {
         w0 = static_cast<T>(wkr);
@@ -523,8 +561,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         w1 = static_cast<T>(ck1);
       }
       if (active) {
-        Wl[kk * NT + tid] = w0;
-        Wl[(kk + 1) * NT + tid] = w1;
+        Wl[(kk - base) * WS + tid] = w0;
+        Wl[(kk + 1 - base) * WS + tid] = w1;
       }
       const double a = wkk, bb = wkr, cc = crr;
       double itl, ibr, ioff;
@@ -553,7 +591,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         dc0[kk + 1] = ioff;
         dc1[kk + 1] = ibr;
       }
-      if (has2) {
+      if (use2) {
         const bool rk2 = (r == k + 2);
         const double* src = rk2 ? colK1 : colK2;
         if (tid < kk) Lnext[tid] = static_cast<T>(ucol(dt, dc0, dc1, src, tid));
@@ -597,8 +635,28 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     for (int i = 0; i < 8; ++i) atomicAdd(&g_panel_profile[(gi == 0 ? 0 : 8) + i], pacc[i]);
 #endif
 
-  if (own)
-    for (int q = 0; q < kk; ++q) Wg[(int64_t)q * p.ldw + j] = Wl[q * NT + tid];
+  if (own) {
+    if constexpr (RR > 0) {
+      if (base > 0) {
+#pragma unroll
+        for (int q = 0; q < RR; ++q) Wg[(int64_t)q * p.ldw + j] = wr[q];
+      }
+    }
+    for (int q = base; q < kk; ++q) Wg[(int64_t)q * p.ldw + j] = Wl[(q - base) * WS + tid];
+  }
+  if (gi == 0 && nswap > 0) {
+    // the interchanges applied to perm as one gather (kept out of the
+    // pivot loop, where a read of perm would stall workgroup 0)
+    compose_swaps(swl, nswap, sw_slot, sw_content);
+    const int ne = 2 * nswap;
+    const int t0 = tid, t1 = tid + NT;
+    const int v0 = t0 < ne ? p.perm[sw_content[sw_slot[t0]]] : 0;
+    const int v1 = t1 < ne ? p.perm[sw_content[sw_slot[t1]]] : 0;
+    wait_vm();
+    __syncthreads();  // every read of perm precedes every write
+    if (t0 < ne) p.perm[swl[t0]] = v0;
+    if (t1 < ne) p.perm[swl[t1]] = v1;
+  }
   if (gi == 0 && tid == 0) {
     p.state[PanelState::kStart] = s;
     p.state[PanelState::kWidth] = kk;
@@ -636,7 +694,7 @@ hipError_t panel_max_groups(int* groups) {
 
 size_t panel_exchange_records(int groups, int elem_bytes) {
   const int bm = elem_bytes == 8 ? PanelCfg<double>::kMaxWidth : PanelCfg<float>::kMaxWidth;
-  return 2 * static_cast<size_t>(groups + 16 + 3 * bm + groups * bm);
+  return 2 * static_cast<size_t>(groups + 16 + 3 * bm);
 }
 
 template hipError_t launch_panel<double>(double*, const PanelArgs&, int, hipStream_t);

@@ -40,7 +40,8 @@ def check_against_oracle(oracle, A, g, bs, res_floor=1e-15):
 
 @pytest.mark.parametrize("n,nb", [(1, 1), (2, 1), (3, 2), (4, 2), (5, 3), (8, 1), (16, 4),
                                   (17, 8), (33, 5), (64, 16), (100, 32), (129, 64),
-                                  (257, 64), (300, 0), (513, 48), (700, 64)])
+                                  (257, 64), (300, 0), (513, 48), (700, 64),
+                                  (200, 100), (513, 128), (1100, 0), (1500, 120)])
 def test_random_matches_oracle(oracle, handle, n, nb):
     for seed in range(3):
         A = oracle.gen_reference_symmetric(n, 10000 * n + 100 * nb + seed)
@@ -104,7 +105,7 @@ def test_edge_cases(oracle, handle, case):
     else:
         A = rng.uniform(-1, 1, (n, n)); A = A + A.T
         A[np.diag_indices(n)] = 3 * n
-    for nb in (1, 5, 32):
+    for nb in (1, 5, 32, 128):
         g = gpu_factor(handle, oracle, A, nb)
         assert np.all(np.isfinite(g.compact))
         check_against_oracle(oracle, A, g, nb, res_floor=1e-15)
