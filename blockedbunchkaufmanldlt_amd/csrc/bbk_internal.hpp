@@ -163,26 +163,77 @@ __device__ __forceinline__ double div_or_zero(double w, double den) {
 // index (matches the strict '>' scan of src/block_ldlt_decomposition.cc:173).
 __device__ __forceinline__ void argmax_combine(double& v, int& i, double v2,
                                                int i2) {
-  if (v2 > v || (v2 == v && i2 < i)) {
-    v = v2;
-    i = i2;
-  }
+  const bool take = (v2 > v) | ((v2 == v) & (i2 < i));  // no short circuit
+  v = take ? v2 : v;
+  i = take ? i2 : i;
 }
 
+// Wavefront reductions by DPP lane exchanges (VALU operand modifiers, no
+// LDS round trip): two quad permutes, the half-row and row mirrors, then the
+// row broadcasts of lanes 15 and 31 leave the result in lane 63, which is
+// read back uniformly.  A lane whose DPP source is not written keeps its own
+// value (update_dpp's `old`), which every combine below leaves unchanged.
+constexpr int kDppQuadXor1 = 0xB1;       // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadXor2 = 0x4E;       // quad_perm [2, 3, 0, 1]
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppRowBcast15 = 0x142;    // with row mask 0xA
+constexpr int kDppRowBcast31 = 0x143;    // with row mask 0xC
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_i32(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_i32<CTRL, ROW_MASK>(static_cast<int>(b & 0xffffffffll));
+  const int hi = dpp_i32<CTRL, ROW_MASK>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// max with NaN never taken over a number already held (as (v < o) ? o : v)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void max_step(double& v) {
+  const double o = dpp_f64<CTRL, ROW_MASK>(v);
+  v = (v < o) ? o : v;
+}
 __device__ __forceinline__ double wave_max_nan_ignoring(double v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_xor(v, off, 64);
-    v = (v < o) ? o : v;
-  }
-  return v;
+  max_step<kDppQuadXor1, 0xf>(v);
+  max_step<kDppQuadXor2, 0xf>(v);
+  max_step<kDppRowHalfMirror, 0xf>(v);
+  max_step<kDppRowMirror, 0xf>(v);
+  max_step<kDppRowBcast15, 0xa>(v);
+  max_step<kDppRowBcast31, 0xc>(v);
+  return readlane_f64(v, 63);
 }
 
+// first maximum: larger value wins, ties go to the lower index; branch-free
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void argmax_step(double& v, int& i) {
+  const double ov = dpp_f64<CTRL, ROW_MASK>(v);
+  const int oi = dpp_i32<CTRL, ROW_MASK>(i);
+  const bool take = (ov > v) | ((ov == v) & (oi < i));  // no short circuit
+  v = take ? ov : v;
+  i = take ? oi : i;
+}
 __device__ __forceinline__ void wave_argmax(double& v, int& i) {
-  for (int off = 32; off > 0; off >>= 1) {
-    double ov = __shfl_xor(v, off, 64);
-    int oi = __shfl_xor(i, off, 64);
-    argmax_combine(v, i, ov, oi);
-  }
+  argmax_step<kDppQuadXor1, 0xf>(v, i);
+  argmax_step<kDppQuadXor2, 0xf>(v, i);
+  argmax_step<kDppRowHalfMirror, 0xf>(v, i);
+  argmax_step<kDppRowMirror, 0xf>(v, i);
+  argmax_step<kDppRowBcast15, 0xa>(v, i);
+  argmax_step<kDppRowBcast31, 0xc>(v, i);
+  v = readlane_f64(v, 63);
+  i = __builtin_amdgcn_readlane(i, 63);
 }
 
 // Workgroup-wide reductions; every thread receives the result.  sv/si

@@ -31,8 +31,10 @@
 namespace bbk {
 
 #if defined(BBK_PANEL_PROFILE)
-// Per-phase clock totals of groups 0 (slots 0-7) and G-1 (slots 8-15).
-__device__ unsigned long long g_panel_profile[16];
+// Per-phase clock totals over the pivots that need the candidate row, of
+// groups 0 (slots 0-11) and G-1 (slots 12-23): slots 0-8 the intervals
+// between the BBK_PT points 0..9, slot 9 the number of such pivots.
+__device__ unsigned long long g_panel_profile[24];
 #endif
 
 // Stores that stay in the XCD's L2 (scope bit sc0).  A device-coherent (sc1)
@@ -60,13 +62,62 @@ __device__ __forceinline__ void st_l2<float>(float* p, float v) {
 // no cache-wide writeback/invalidate is needed around an exchange: a
 // writer only waits for its stores to complete before posting records.
 template <typename T>
-__device__ __forceinline__ T ldc(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
 __device__ __forceinline__ void stc(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Device-coherent loads issued together, with the wait for them in the same
+// asm block.  Inside the pivot loop every load goes through these: a load
+// the compiler sees in flight makes it wait, where control flow merges, for
+// every store issued since (one counter on gfx9 covers loads and stores),
+// which would serialize the pivot's coherent stores.  Every pointer must be
+// valid; lanes that need no value pass a harmless address.
+template <typename T>
+struct CoherentLoad;
+template <>
+struct CoherentLoad<double> {
+  static __device__ __forceinline__ void one(const double* p0, double& v0) {
+    asm volatile("global_load_dwordx2 %0, %1, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=&v"(v0)
+                 : "v"(p0)
+                 : "memory");
+  }
+  static __device__ __forceinline__ void four(const double* p0, const double* p1,
+                                              const double* p2, const double* p3, double& v0,
+                                              double& v1, double& v2, double& v3) {
+    asm volatile(
+        "global_load_dwordx2 %0, %4, off sc1\n\t"
+        "global_load_dwordx2 %1, %5, off sc1\n\t"
+        "global_load_dwordx2 %2, %6, off sc1\n\t"
+        "global_load_dwordx2 %3, %7, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+        : "v"(p0), "v"(p1), "v"(p2), "v"(p3)
+        : "memory");
+  }
+};
+template <>
+struct CoherentLoad<float> {
+  static __device__ __forceinline__ void one(const float* p0, float& v0) {
+    asm volatile("global_load_dword %0, %1, off sc1\n\ts_waitcnt vmcnt(0)"
+                 : "=&v"(v0)
+                 : "v"(p0)
+                 : "memory");
+  }
+  static __device__ __forceinline__ void four(const float* p0, const float* p1,
+                                              const float* p2, const float* p3, float& v0,
+                                              float& v1, float& v2, float& v3) {
+    asm volatile(
+        "global_load_dword %0, %4, off sc1\n\t"
+        "global_load_dword %1, %5, off sc1\n\t"
+        "global_load_dword %2, %6, off sc1\n\t"
+        "global_load_dword %3, %7, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+        : "v"(p0), "v"(p1), "v"(p2), "v"(p3)
+        : "memory");
+  }
+};
 
 // sum_q L[q] * W(q, own column), q < kk.  Rows [0, RR) come from the
 // registers wr once base == RR; rows [base, kk) from LDS, row q at
@@ -77,12 +128,22 @@ __device__ __forceinline__ T panel_dot(const T* L, const T* Wl, const T (&wr)[NR
   T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   if constexpr (RR > 0) {
     if (base > 0) {
+      // L in blocks of 16, all loads of a block issued before its products
+      static_assert(RR % 16 == 0, "register rows come in blocks of 16");
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const d2* L2 = reinterpret_cast<const d2*>(L);
 #pragma unroll
-      for (int q = 0; q < RR; q += 4) {
-        a0 += L[q] * wr[q];
-        a1 += L[q + 1] * wr[q + 1];
-        a2 += L[q + 2] * wr[q + 2];
-        a3 += L[q + 3] * wr[q + 3];
+      for (int b = 0; b < RR; b += 16) {
+        d2 lv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) lv[u] = L2[b / 2 + u];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          a0 += lv[u].x * wr[b + 2 * u];
+          a1 += lv[u].y * wr[b + 2 * u + 1];
+          a2 += lv[u + 1].x * wr[b + 2 * u + 2];
+          a3 += lv[u + 1].y * wr[b + 2 * u + 3];
+        }
       }
     }
   }
@@ -135,8 +196,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   // W(q, own column): q < base (= 0 or RR) in registers wr[q], the rest in
   // LDS row q - base
   __shared__ T Wl[(LR + 1) * WS];
-  __shared__ T Lbuf[2][BM + 2];  // L(k, panel) of this pivot / of the next
-  __shared__ T Lr[BM + 2];
+  // L(k, panel) of this pivot / of the next, and of the candidate row
+  __shared__ __attribute__((aligned(16))) T Lbuf[2][BM + 2];
+  __shared__ __attribute__((aligned(16))) T Lr[BM + 2];
   __shared__ double colK[BM + 2], colK1[BM + 2], colK2[BM + 2];
   __shared__ int dt[BM + 2];
   __shared__ double dc0[BM + 2], dc1[BM + 2];
@@ -211,7 +273,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       stc(ap, v);
   };
 
-  T arow = own ? ldc(&A[(int64_t)s * lda + j]) : T(0);
+  // a harmless address for lanes that load nothing (a valid element)
+  const T* const dummy = A + (int64_t)s * lda + s;
+  T arow;
+  CoherentLoad<T>::one(own ? &A[(int64_t)s * lda + j] : dummy, arow);
+  if (!own) arow = T(0);
   int kk = 0, nswap = 0;
   T wr[NR];
   int base = 0;
@@ -243,7 +309,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   };
 #if defined(BBK_PANEL_PROFILE)
   // Phase timing with the 100 MHz device clock (build-time option only).
-  unsigned long long tp[8] = {0}, pacc[8] = {0};
+  unsigned long long tp[10] = {0}, pacc[12] = {0};
   int kind_last = 0;
 #define BBK_PT(i)                      \
   do {                                 \
@@ -280,6 +346,26 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     }
     T* const Lk = Lbuf[lp];
     T* const Lnext = Lbuf[lp ^ 1];
+    // ---- exchange 1, early part: the W columns k, k+1, k+2 (rows < kk) are
+    // final before row k is formed, so they go out now and travel while the
+    // row is computed (they depend on no load in flight, so posting them
+    // waits for nothing).  (Every workgroup has
+    // posted the previous exchange, so none still reads this record area;
+    // the slot records below, posted after the A writes completed, are
+    // what a reader waits for before it uses A.)
+    {
+      const int ex1 = ex + 1;
+      XRec* ar = area(ex1);
+      XRec* pcols = ar + G + kNumPub;
+      if (tid < kk) {
+        const int q = tid;
+        if (is_mine(k)) post(pcols + q, ex1, q, static_cast<double>(wcol(q, k)));
+        if (has1 && is_mine(k + 1))
+          post(pcols + BM + q, ex1, q, static_cast<double>(wcol(q, k + 1)));
+        if (use2 && is_mine(k + 2))
+          post(pcols + 2 * BM + q, ex1, q, static_cast<double>(wcol(q, k + 2)));
+      }
+    }
     // ---- phase A: W(k, j) for own j >= k, local argmax over j > k -------
     const bool active = own && j >= k;
     T w = T(0);
@@ -287,6 +373,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       w = arow - panel_dot<T, WS, RR>(Lk, Wl, wr, tid, kk, base);
       Wl[(kk - base) * WS + tid] = w;
     }
+    BBK_PT(1);
     double v = 0.0;
     int vi = INT_MAX;
     if (active && j > k) {
@@ -299,8 +386,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     // this thread's earlier (coherent) A writes have completed; the barrier
     // of the reduction then orders every thread's before the records below
     wait_vm();
+    BBK_PT(2);
     block_argmax1(v, vi, red_v, red_i, par);
-    BBK_PT(1);
+    BBK_PT(3);
     // ---- exchange 1: post --------------------------------------------------
     ++ex;
     {
@@ -310,7 +398,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         post(ar + gi, ex, vi,
              vi != INT_MAX ? static_cast<double>(Wl[(kk - base) * WS + (vi - c0)]) : 0.0);
       XRec* pubs = ar + G;
-      XRec* pcols = ar + G + kNumPub;
       if (j == k) {
         post(pubs + kPubWk, ex, 0, static_cast<double>(w));
         post(pubs + kPubAkk, ex, 0, static_cast<double>(arow));
@@ -320,16 +407,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         post(pubs + kPubAkk1, ex, 0, static_cast<double>(arow));
       }
       if (j == k + 2) post(pubs + kPubWk2, ex, 0, static_cast<double>(w));
-      if (tid < kk) {
-        const int q = tid;
-        if (is_mine(k)) post(pcols + q, ex, q, static_cast<double>(wcol(q, k)));
-        if (has1 && is_mine(k + 1))
-          post(pcols + BM + q, ex, q, static_cast<double>(wcol(q, k + 1)));
-        if (use2 && is_mine(k + 2))
-          post(pcols + 2 * BM + q, ex, q, static_cast<double>(wcol(q, k + 2)));
-      }
     }
-    BBK_PT(2);
+    BBK_PT(4);
     // ---- exchange 1: gather (each thread waits for its own records) --------
     double omega = 0.0;
     int r = INT_MAX;
@@ -365,24 +444,37 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     // W(k, r) with its sign, from the slot of r's owner
     const double wkr = (r != INT_MAX) ? xval[(r - s) / NT] : 0.0;
     const double wkk = pubv[kPubWk];
-    BBK_PT(3);
+    BBK_PT(5);
     const double akk = fabs(wkk);
-    // prefetch rows k+1, k+2 of A for the next pivot (patched after the swap)
-    const T P1 = (has1 && own && j >= k + 1) ? ldc(&A[(int64_t)(k + 1) * lda + j]) : T(0);
-    const T P2 = (has2 && own && j >= k + 2) ? ldc(&A[(int64_t)(k + 2) * lda + j]) : T(0);
+    // prefetch row k+1 of A for the next pivot (patched after the swap)
+    const bool want1 = has1 && own && j >= k + 1;
+    const bool want2 = has2 && own && j >= k + 2;  // row k+2: only after a 2x2 pivot
+    const T* const p1 = want1 ? &A[(int64_t)(k + 1) * lda + j] : dummy;
+    T P1, P2 = T(0);
     int kind;
     T c = T(0);
     double crr = 0.0, ck = 0.0, ck1 = 0.0, ck2 = 0.0, a11 = 0.0, a12 = 0.0;
     if (akk >= kAlpha * omega || omega == 0.0) {
       kind = 1;  // src/block_ldlt_decomposition.cc:92
+      CoherentLoad<T>::one(p1, P1);
+      if (!want1) P1 = T(0);
     } else {
       // ---- phase B: candidate row r ----------------------------------------
-      T a = T(0);
-      if (active) a = (j < r) ? ldc(&A[(int64_t)j * lda + r]) : ldc(&A[(int64_t)r * lda + j]);
-      // L(r, s+q) = U(q, r), q < kk: column r of the finished rows as stored
-      // (the same operations on W(q, r) that ucol() performs)
-      if (tid < kk) Lr[tid] = ldc(&A[(int64_t)(s + tid) * lda + r]);
+      // candidate row element, L(r, s+q) = U(q, r) for q = tid < kk (column r
+      // of the finished rows as stored: the same operations on W(q, r) that
+      // ucol() performs), and the rows k+1, k+2 for the next pivot
+      const T* pa = active ? ((j < r) ? &A[(int64_t)j * lda + r] : &A[(int64_t)r * lda + j])
+                           : dummy;
+      const T* pl = tid < kk ? &A[(int64_t)(s + tid) * lda + r] : dummy;
+      const T* p2 = want2 ? &A[(int64_t)(k + 2) * lda + j] : dummy;
+      T a, lr;
+      CoherentLoad<T>::four(pa, pl, p1, p2, a, lr, P1, P2);
+      if (!active) a = T(0);
+      if (!want1) P1 = T(0);
+      if (!want2) P2 = T(0);
+      if (tid < kk) Lr[tid] = lr;
       __syncthreads();
+      BBK_PT(6);
       if (active) {
         c = a - panel_dot<T, WS, RR>(Lr, Wl, wr, tid, kk, base);
       }
@@ -392,7 +484,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         sv = (aa >= 0.0) ? aa : 0.0;  // NaN ignored, as std::max does
       }
       sv = block_max1(sv, red_v, par);
-      BBK_PT(4);
+      BBK_PT(7);
       // ---- exchange 2 ------------------------------------------------------
       // Rows k+1/k+2 prefetches may still be in flight: the elements other
       // groups can overwrite in phase C are exactly the ones phase C
@@ -436,7 +528,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         a12 = pubv[kPubA12];
       }
       kind = select_pivot_nc(akk, omega, sigma, fabs(crr));
-      BBK_PT(5);
+      BBK_PT(8);
     }
 #if defined(BBK_PANEL_PROFILE)
     kind_last = kind;
@@ -613,26 +705,16 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     arow = arow_next;
     kk += size;
 #if defined(BBK_PANEL_PROFILE)
-    BBK_PT(6);
-    if (tid == 0) {
-      pacc[0] += tp[1] - tp[0];
-      pacc[1] += tp[2] - tp[1];
-      pacc[2] += tp[3] - tp[2];
-      if (kind_last != 1) {
-        pacc[3] += tp[4] - tp[3];
-        pacc[4] += tp[5] - tp[4];
-        pacc[5] += tp[6] - tp[5];
-        pacc[7] += 1;
-      } else {
-        pacc[5] += tp[6] - tp[3];
-        pacc[6] += 1;
-      }
+    BBK_PT(9);
+    if (tid == 0 && kind_last != 1) {
+      for (int u = 0; u < 9; ++u) pacc[u] += tp[u + 1] - tp[u];
+      pacc[9] += 1;
     }
 #endif
   }
 #if defined(BBK_PANEL_PROFILE)
   if (tid == 0 && (gi == 0 || gi == G - 1))
-    for (int i = 0; i < 8; ++i) atomicAdd(&g_panel_profile[(gi == 0 ? 0 : 8) + i], pacc[i]);
+    for (int i = 0; i < 12; ++i) atomicAdd(&g_panel_profile[(gi == 0 ? 0 : 12) + i], pacc[i]);
 #endif
 
   if (own) {
@@ -708,9 +790,9 @@ template hipError_t panel_max_groups<float>(int*);
 // Read (and optionally clear) the phase clock totals of the profiling build.
 extern "C" int bbk_panel_profile(unsigned long long* out, int reset) {
   hipError_t e = hipMemcpyFromSymbol(out, HIP_SYMBOL(bbk::g_panel_profile),
-                                     sizeof(unsigned long long) * 16);
+                                     sizeof(unsigned long long) * 24);
   if (reset) {
-    unsigned long long z[16] = {0};
+    unsigned long long z[24] = {0};
     (void)hipMemcpyToSymbol(HIP_SYMBOL(bbk::g_panel_profile), z, sizeof(z));
   }
   return e == hipSuccess ? 0 : 1;
