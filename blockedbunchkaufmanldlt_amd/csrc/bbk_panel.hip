@@ -1,15 +1,17 @@
 // bbk_panel.hip -- the Bunch-Kaufman panel kernel for gfx950.
 //
 // One cooperative launch factors one panel of up to b columns starting at
-// s (read from device state).  Workgroup g owns the active columns
-// [s + 256 g, s + 256 g + 256) of rows [s, n); thread t owns one column and
-// keeps that column of the panel's unscaled rows W = D L^T in LDS (binary64:
-// the first 64 rows move to registers once the panel passes them).
+// s (read from device state).  Workgroup g starts with the active columns
+// [s + 256 g, s + 256 g + 256) of rows [s, n); thread t holds one column
+// (position) and keeps that column of the panel's unscaled rows W = D L^T
+// in LDS (binary64: the first 64 rows move to registers once the panel
+// passes them).  An interchange swaps the positions of the two threads
+// holding the interchanged columns; no W entry ever moves.
 //
 // Per pivot k (local index kk) the workgroups meet at most twice:
 //   exchange 1  after W(k, :) = A(k, :) - L(k, panel) W(panel, :) and the
-//               local argmax; carries (omega_g, r_g), the W columns k, k+1,
-//               k+2 and a few scalars;
+//               local argmax; carries (omega_g, r_g, W(k, r_g)) and a few
+//               scalars;
 //   exchange 2  (only when the Bunch-Kaufman test needs the candidate row r)
 //               after the candidate row and its local max sigma_g; carries
 //               sigma_g and the candidate values at k, k+1, k+2, r.
@@ -18,9 +20,10 @@
 // record only when its tag equals the exchange's number, so an exchange is
 // one store per writer and one polling read per reader, with no atomics.
 // Two record areas alternate by exchange parity.  Everything the next
-// pivot needs (its L column, its A row) is rebuilt locally from published
-// values, so no third synchronization per pivot is needed; the candidate's
-// L row is column r of the finished U rows, read back from A.
+// pivot needs (its A row, its L row) is rebuilt locally from published
+// values and from A, so no third synchronization per pivot is needed: the
+// L rows of the candidate and of the next pivot are columns of the finished
+// U rows, read back from A.
 //
 // Arithmetic: pivot tests, 2x2 inverse and L = D^-1 W in binary64 without
 // contraction (src/block_ldlt_decomposition.cc:21-26,369-390,633-704).
@@ -82,17 +85,31 @@ struct CoherentLoad<double> {
                  : "v"(p0)
                  : "memory");
   }
-  static __device__ __forceinline__ void four(const double* p0, const double* p1,
-                                              const double* p2, const double* p3, double& v0,
-                                              double& v1, double& v2, double& v3) {
+  static __device__ __forceinline__ void two(const double* p0, const double* p1, double& v0,
+                                             double& v1) {
     asm volatile(
-        "global_load_dwordx2 %0, %4, off sc1\n\t"
-        "global_load_dwordx2 %1, %5, off sc1\n\t"
-        "global_load_dwordx2 %2, %6, off sc1\n\t"
-        "global_load_dwordx2 %3, %7, off sc1\n\t"
+        "global_load_dwordx2 %0, %2, off sc1\n\t"
+        "global_load_dwordx2 %1, %3, off sc1\n\t"
         "s_waitcnt vmcnt(0)"
-        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
-        : "v"(p0), "v"(p1), "v"(p2), "v"(p3)
+        : "=&v"(v0), "=&v"(v1)
+        : "v"(p0), "v"(p1)
+        : "memory");
+  }
+  static __device__ __forceinline__ void six(const double* p0, const double* p1,
+                                             const double* p2, const double* p3,
+                                             const double* p4, const double* p5, double& v0,
+                                             double& v1, double& v2, double& v3, double& v4,
+                                             double& v5) {
+    asm volatile(
+        "global_load_dwordx2 %0, %6, off sc1\n\t"
+        "global_load_dwordx2 %1, %7, off sc1\n\t"
+        "global_load_dwordx2 %2, %8, off sc1\n\t"
+        "global_load_dwordx2 %3, %9, off sc1\n\t"
+        "global_load_dwordx2 %4, %10, off sc1\n\t"
+        "global_load_dwordx2 %5, %11, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3), "=&v"(v4), "=&v"(v5)
+        : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4), "v"(p5)
         : "memory");
   }
 };
@@ -104,17 +121,30 @@ struct CoherentLoad<float> {
                  : "v"(p0)
                  : "memory");
   }
-  static __device__ __forceinline__ void four(const float* p0, const float* p1,
-                                              const float* p2, const float* p3, float& v0,
-                                              float& v1, float& v2, float& v3) {
+  static __device__ __forceinline__ void two(const float* p0, const float* p1, float& v0,
+                                             float& v1) {
     asm volatile(
-        "global_load_dword %0, %4, off sc1\n\t"
-        "global_load_dword %1, %5, off sc1\n\t"
-        "global_load_dword %2, %6, off sc1\n\t"
-        "global_load_dword %3, %7, off sc1\n\t"
+        "global_load_dword %0, %2, off sc1\n\t"
+        "global_load_dword %1, %3, off sc1\n\t"
         "s_waitcnt vmcnt(0)"
-        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
-        : "v"(p0), "v"(p1), "v"(p2), "v"(p3)
+        : "=&v"(v0), "=&v"(v1)
+        : "v"(p0), "v"(p1)
+        : "memory");
+  }
+  static __device__ __forceinline__ void six(const float* p0, const float* p1, const float* p2,
+                                             const float* p3, const float* p4, const float* p5,
+                                             float& v0, float& v1, float& v2, float& v3,
+                                             float& v4, float& v5) {
+    asm volatile(
+        "global_load_dword %0, %6, off sc1\n\t"
+        "global_load_dword %1, %7, off sc1\n\t"
+        "global_load_dword %2, %8, off sc1\n\t"
+        "global_load_dword %3, %9, off sc1\n\t"
+        "global_load_dword %4, %10, off sc1\n\t"
+        "global_load_dword %5, %11, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3), "=&v"(v4), "=&v"(v5)
+        : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4), "v"(p5)
         : "memory");
   }
 };
@@ -148,39 +178,51 @@ __device__ __forceinline__ T panel_dot(const T* L, const T* Wl, const T (&wr)[NR
     }
   }
   const T* Wc = Wl + tid;
-  const T* Lb = L + base;
+  const T* Lb = L + base;  // 16-byte aligned: L is, and base is 0 or 64
   const int cnt = kk - base;
   int q = 0;
-#pragma unroll 2
-  for (; q + 4 <= cnt; q += 4) {
-    a0 += Lb[q] * Wc[q * WS];
-    a1 += Lb[q + 1] * Wc[(q + 1) * WS];
-    a2 += Lb[q + 2] * Wc[(q + 2) * WS];
-    a3 += Lb[q + 3] * Wc[(q + 3) * WS];
+  // blocks of 8 rows: all 16 loads of a block are issued before its
+  // products (one LDS round trip per block)
+  for (; q + 8 <= cnt; q += 8) {
+    T lv[8], wv[8];
+    if constexpr (sizeof(T) == 8) {
+      typedef double d2 __attribute__((ext_vector_type(2)));
+      const d2* L2 = reinterpret_cast<const d2*>(Lb + q);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const d2 t = L2[u];
+        lv[2 * u] = t.x;
+        lv[2 * u + 1] = t.y;
+      }
+    } else {
+      typedef float f4 __attribute__((ext_vector_type(4)));
+      const f4* L4 = reinterpret_cast<const f4*>(Lb + q);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const f4 t = L4[u];
+        lv[4 * u] = t.x;
+        lv[4 * u + 1] = t.y;
+        lv[4 * u + 2] = t.z;
+        lv[4 * u + 3] = t.w;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) wv[u] = Wc[(q + u) * WS];
+    a0 += lv[0] * wv[0];
+    a1 += lv[1] * wv[1];
+    a2 += lv[2] * wv[2];
+    a3 += lv[3] * wv[3];
+    a0 += lv[4] * wv[4];
+    a1 += lv[5] * wv[5];
+    a2 += lv[6] * wv[6];
+    a3 += lv[7] * wv[7];
   }
   for (; q < cnt; ++q) a0 += Lb[q] * Wc[q * WS];
   return (a0 + a1) + (a2 + a3);
 }
 
-// Panel-row scaling table: row q of L^T = D^-1 W.
-//   type 1 : U(q) = W(q) / c0            (c0 = SafeDenominator(d), 0 -> 0)
-//   type 2 : U(q) = c0 W(q) + c1 W(q+1)  (c0 = itl, c1 = ioff)
-//   type -2: U(q) = c0 W(q-1) + c1 W(q)  (c0 = ioff, c1 = ibr)
-__device__ __forceinline__ double ucol(const int* dt, const double* c0, const double* c1,
-                                       const double* col, int q) {
-  const int ty = dt[q];
-  if (ty == 1) return div_or_zero(col[q], c0[q]);
-  double u0, u1;
-  if (ty == 2) {
-    apply_2x2_nc(c0[q], 0.0, c1[q], col[q], col[q + 1], &u0, &u1);
-    return u0;
-  }
-  apply_2x2_nc(0.0, c1[q], c0[q], col[q - 1], col[q], &u0, &u1);
-  return u1;
-}
-
 enum PubId {
-  kPubWk = 0, kPubAkk = 1, kPubWk1 = 2, kPubAkk1 = 3, kPubWk2 = 4,
+  kPubWk = 0, kPubAkk = 1, kPubWk1 = 2, kPubAkk1 = 3, kPubWk2 = 4, kPubWkr = 5,
   kPubCr = 8, kPubCk = 9, kPubCk1 = 10, kPubA11 = 11, kPubCk2 = 12, kPubA12 = 13,
   kNumPub = 16
 };
@@ -199,10 +241,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   // L(k, panel) of this pivot / of the next, and of the candidate row
   __shared__ __attribute__((aligned(16))) T Lbuf[2][BM + 2];
   __shared__ __attribute__((aligned(16))) T Lr[BM + 2];
-  __shared__ double colK[BM + 2], colK1[BM + 2], colK2[BM + 2];
-  __shared__ int dt[BM + 2];
-  __shared__ double dc0[BM + 2], dc1[BM + 2];
-  __shared__ double xval[kMaxPanelGroupsLds];
   __shared__ double pubv[kNumPub];
   __shared__ double red_v[2 * (NT / 64)];
   __shared__ int red_i[2 * (NT / 64)];
@@ -228,20 +266,22 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   const int gi = blockIdx.x / stride;  // working-group index
   const int G = gridDim.x / stride;
   const int tid = threadIdx.x;
-  const int c0 = s + gi * NT;  // first column owned by this group
-  const int j = c0 + tid;
+  // Thread tid starts at position (column) j = s + NT gi + tid and keeps
+  // its W column for the whole panel: a symmetric interchange of positions
+  // a and b swaps the positions of the two threads holding them instead of
+  // moving W entries.
+  int j = s + gi * NT + tid;
   const bool own = j < n;
   const int bt = min(p.b, n - s);
   T* Wg = reinterpret_cast<T*>(p.Wg);
-  // record areas: [slots G][pubs 16][pubcols 3 BM], two parities
-  const int PS = G + kNumPub + 3 * BM;
+  // record areas: [slots G][pubs 16], two parities
+  const int PS = G + kNumPub;
   XRec* const xbase = p.xrec;
   int ex = p.epoch;  // exchange number = record tag
   int par = 0;       // reduction scratch half
   int lp = 0;        // Lbuf index of the current pivot
 
   auto area = [&](int tag) -> XRec* { return xbase + (tag & 1) * PS; };
-  auto is_mine = [&](int col) { return col >= c0 && col < c0 + NT && col < n; };
 
   // L2-scope exchanges only when every working group reports the same
   // XCC_ID (one device-coherent exchange per launch); otherwise every store
@@ -281,31 +321,12 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   int kk = 0, nswap = 0;
   T wr[NR];
   int base = 0;
-  // Once base == RR, workgroup 0 keeps a copy of the register rows of the
-  // panel's own columns s + RR + c, c < LR + 1 (the only columns whose W
-  // entries are published: k, k+1, k+2) in its LDS columns [0, RR), which
-  // belong to the finished columns [s, s + RR): entry (q, c) in LDS row c,
-  // column q.
-  auto pb = [&](int q, int c) -> T& { return Wl[c * WS + q]; };
-  // W(q, col), q < kk, for a column of this group (col is a panel column
-  // when q < base)
-  auto wcol = [&](int q, int col) -> T {
-    if (RR > 0 && base > 0 && q < RR) return pb(q, col - s - RR);
-    return Wl[(q - base) * WS + (col - c0)];
-  };
-  // a column exchange of W: own column takes src[q], q < min(kk, base)
-  auto take_col = [&](const double* src) {
-    if constexpr (RR > 0) {
-      if (base > 0) {
-        const int c = j - s - RR;
-        const bool panel_col = gi == 0 && c >= 0 && c < LR + 1;
-#pragma unroll
-        for (int q = 0; q < RR; ++q) {
-          wr[q] = static_cast<T>(src[q]);
-          if (panel_col) pb(q, c) = wr[q];
-        }
-      }
-    }
+  // interchange of positions a < b: their holders swap positions
+  auto swap_pos = [&](int a, int b) {
+    if (j == a)
+      j = b;
+    else if (j == b)
+      j = a;
   };
 #if defined(BBK_PANEL_PROFILE)
   // Phase timing with the 100 MHz device clock (build-time option only).
@@ -325,8 +346,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     BBK_PT(0);
     const int k = s + kk;
     const bool has1 = k + 1 < n, has2 = k + 2 < n;
-    // column k+2's W entries are published only when a pivot at k+2 can
-    // follow in this panel
+    // a pivot at k+2 can follow in this panel
     const bool use2 = has2 && kk + 2 < bt;
     if constexpr (RR > 0) {
       if (base == 0 && kk >= RR) {
@@ -337,35 +357,10 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         for (int q = RR; q < kk; ++q) Wl[(q - RR) * WS + tid] = Wl[q * WS + tid];
         base = RR;
         __syncthreads();
-        if (gi == 0 && tid >= RR && tid < RR + LR + 1) {
-#pragma unroll
-          for (int q = 0; q < RR; ++q) pb(q, tid - RR) = wr[q];
-        }
-        __syncthreads();
       }
     }
     T* const Lk = Lbuf[lp];
     T* const Lnext = Lbuf[lp ^ 1];
-    // ---- exchange 1, early part: the W columns k, k+1, k+2 (rows < kk) are
-    // final before row k is formed, so they go out now and travel while the
-    // row is computed (they depend on no load in flight, so posting them
-    // waits for nothing).  (Every workgroup has
-    // posted the previous exchange, so none still reads this record area;
-    // the slot records below, posted after the A writes completed, are
-    // what a reader waits for before it uses A.)
-    {
-      const int ex1 = ex + 1;
-      XRec* ar = area(ex1);
-      XRec* pcols = ar + G + kNumPub;
-      if (tid < kk) {
-        const int q = tid;
-        if (is_mine(k)) post(pcols + q, ex1, q, static_cast<double>(wcol(q, k)));
-        if (has1 && is_mine(k + 1))
-          post(pcols + BM + q, ex1, q, static_cast<double>(wcol(q, k + 1)));
-        if (use2 && is_mine(k + 2))
-          post(pcols + 2 * BM + q, ex1, q, static_cast<double>(wcol(q, k + 2)));
-      }
-    }
     // ---- phase A: W(k, j) for own j >= k, local argmax over j > k -------
     const bool active = own && j >= k;
     T w = T(0);
@@ -393,10 +388,10 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     ++ex;
     {
       XRec* ar = area(ex);
-      // slot: local argmax column and its signed value W(k, r_g)
-      if (tid == 0)
-        post(ar + gi, ex, vi,
-             vi != INT_MAX ? static_cast<double>(Wl[(kk - base) * WS + (vi - c0)]) : 0.0);
+      // slot: local argmax position and its signed value W(k, r_g), posted
+      // by its holder
+      if (vi != INT_MAX ? j == vi : tid == 0)
+        post(ar + gi, ex, vi, vi != INT_MAX ? static_cast<double>(w) : 0.0);
       XRec* pubs = ar + G;
       if (j == k) {
         post(pubs + kPubWk, ex, 0, static_cast<double>(w));
@@ -415,34 +410,26 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     {
       const XRec* ar = area(ex);
       const int npub = 2 + (has1 ? 2 : 0) + (has2 ? 1 : 0);
-      const int ncol = kk * (1 + (has1 ? 1 : 0) + (use2 ? 1 : 0));
-      const int total = G + npub + ncol;
+      const int total = G + npub;
+      int slot_aux = INT_MAX;
+      double slot_val = 0.0;
       for (int id = tid; id < total; id += NT) {
         int aux;
         if (id < G) {
-          const double val = xwait(ar + id, ex, &aux);
-          xval[id] = val;
-          argmax_combine(omega, r, fabs(val), aux);
-        } else if (id < G + npub) {
+          slot_val = xwait(ar + id, ex, &aux);
+          slot_aux = aux;
+          argmax_combine(omega, r, fabs(slot_val), aux);
+        } else {
           const int u = id - G;
           const int pid = u < 2 ? u : (has1 && u < 4 ? u : kPubWk2);
           pubv[pid] = xwait(ar + G + pid, ex, &aux);
-        } else {
-          const int u = id - G - npub;
-          const int which = u / kk, q = u % kk;  // 0: k, 1: k+1, 2: k+2
-          const double val = xwait(ar + G + kNumPub + which * BM + q, ex, &aux);
-          if (which == 0)
-            colK[q] = val;
-          else if (which == 1)
-            colK1[q] = val;
-          else
-            colK2[q] = val;
         }
       }
-      block_argmax1(omega, r, red_v, red_i, par);  // also publishes xval/pubv/colK*
+      block_argmax1(omega, r, red_v, red_i, par);  // also publishes pubv
+      // W(k, r) with its sign, from the slot naming r (read after a later
+      // barrier: only the 2x2 pivot uses it)
+      if (r != INT_MAX && slot_aux == r) pubv[kPubWkr] = slot_val;
     }
-    // W(k, r) with its sign, from the slot of r's owner
-    const double wkr = (r != INT_MAX) ? xval[(r - s) / NT] : 0.0;
     const double wkk = pubv[kPubWk];
     BBK_PT(5);
     const double akk = fabs(wkk);
@@ -450,25 +437,32 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     const bool want1 = has1 && own && j >= k + 1;
     const bool want2 = has2 && own && j >= k + 2;  // row k+2: only after a 2x2 pivot
     const T* const p1 = want1 ? &A[(int64_t)(k + 1) * lda + j] : dummy;
-    T P1, P2 = T(0);
+    // L(k+1, s+q) = U(q, k+1) (and U(q, k+2)), q = tid < kk: the next
+    // pivot's L row from the finished rows as stored (bit for bit the
+    // values D^-1 W(q, .) the scaling steps wrote, also after interchanges,
+    // which move stored columns and W columns together)
+    const bool wantu = tid < kk;
+    const T* const pu1 = (wantu && has1) ? &A[(int64_t)(s + tid) * lda + k + 1] : dummy;
+    T P1, P2 = T(0), u1 = T(0), u2 = T(0);
     int kind;
     T c = T(0);
     double crr = 0.0, ck = 0.0, ck1 = 0.0, ck2 = 0.0, a11 = 0.0, a12 = 0.0;
     if (akk >= kAlpha * omega || omega == 0.0) {
       kind = 1;  // src/block_ldlt_decomposition.cc:92
-      CoherentLoad<T>::one(p1, P1);
+      CoherentLoad<T>::two(p1, pu1, P1, u1);
       if (!want1) P1 = T(0);
     } else {
       // ---- phase B: candidate row r ----------------------------------------
       // candidate row element, L(r, s+q) = U(q, r) for q = tid < kk (column r
-      // of the finished rows as stored: the same operations on W(q, r) that
-      // ucol() performs), and the rows k+1, k+2 for the next pivot
+      // of the finished rows as stored), and for the next pivot the rows
+      // k+1, k+2 and the L rows U(q, k+1), U(q, k+2)
       const T* pa = active ? ((j < r) ? &A[(int64_t)j * lda + r] : &A[(int64_t)r * lda + j])
                            : dummy;
-      const T* pl = tid < kk ? &A[(int64_t)(s + tid) * lda + r] : dummy;
+      const T* pl = wantu ? &A[(int64_t)(s + tid) * lda + r] : dummy;
       const T* p2 = want2 ? &A[(int64_t)(k + 2) * lda + j] : dummy;
+      const T* pu2 = (wantu && use2) ? &A[(int64_t)(s + tid) * lda + k + 2] : dummy;
       T a, lr;
-      CoherentLoad<T>::four(pa, pl, p1, p2, a, lr, P1, P2);
+      CoherentLoad<T>::six(pa, pl, p1, p2, pu1, pu2, a, lr, P1, P2, u1, u2);
       if (!active) a = T(0);
       if (!want1) P1 = T(0);
       if (!want2) P2 = T(0);
@@ -546,41 +540,31 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         sta(&A[(int64_t)k * lda + k], w);
         p.piv[k] = 1;
       }
-      if (tid == 0) {
-        dt[kk] = 1;
-        dc0[kk] = den;
-        dc1[kk] = 0.0;
-      }
       if (has1) {
-        if (tid < kk) Lnext[tid] = static_cast<T>(ucol(dt, dc0, dc1, colK1, tid));
+        if (tid < kk) Lnext[tid] = u1;
         if (tid == kk) Lnext[kk] = static_cast<T>(div_or_zero(pubv[kPubWk1], den));
       }
       size = 1;
     } else if (kind == -1) {
       // symmetric interchange k <-> r (src/block_ldlt_decomposition.cc:414-498)
-      // columns k and r of the finished panel rows, one row per thread; W
-      // column r takes column k's (column k is finished: its W entries are
-      // never read again)
-      if (tid < kk) {
+      // columns k and r of the finished panel rows, one row per thread of
+      // workgroup 0 (the protocol orders these after every earlier write of
+      // the same elements, made at an earlier pivot)
+      if (gi == 0 && tid < kk) {
         const int q = tid;
-        if (is_mine(k)) sta(&A[(int64_t)(s + q) * lda + k], Lr[q]);
-        if (is_mine(r)) {
-          if (q >= base) Wl[(q - base) * WS + (r - c0)] = static_cast<T>(colK[q]);
-          sta(&A[(int64_t)(s + q) * lda + r], Lk[q]);
-        }
+        sta(&A[(int64_t)(s + q) * lda + k], Lr[q]);
+        sta(&A[(int64_t)(s + q) * lda + r], Lk[q]);
       }
-      if (j == r) take_col(colK);
-      T nw = c;
-      if (j == k)
-        nw = static_cast<T>(crr);
-      else if (j == r)
-        nw = static_cast<T>(ck);
-      if (active) Wl[(kk - base) * WS + tid] = nw;
+      // trailing matrix: new row/column r take the old row k (= arow)
       if (own && j > r)
         sta(&A[(int64_t)r * lda + j], arow);
       else if (own && j > k && j < r)
         sta(&A[(int64_t)j * lda + r], arow);
       if (j == r) sta(&A[(int64_t)r * lda + r], static_cast<T>(pubv[kPubAkk]));
+      swap_pos(k, r);
+      // the pivot row is now the candidate row: each thread's own value
+      const T nw = c;
+      if (active) Wl[(kk - base) * WS + tid] = nw;
       const double den = safe_den(crr);
       if (own && j > k)
         sta(&A[(int64_t)k * lda + j], static_cast<T>(div_or_zero(static_cast<double>(nw), den)));
@@ -595,18 +579,14 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         p.swaps[2 * nswap + 1] = r;
       }
       ++nswap;
-      if (tid == 0) {
-        dt[kk] = 1;
-        dc0[kk] = den;
-        dc1[kk] = 0.0;
-      }
       if (has1) {
         if (r == k + 1) {
+          // the new row k+1 is the old row k (the holder of k+1 held k)
           if (tid < kk) Lnext[tid] = Lk[tid];
           if (tid == kk) Lnext[kk] = static_cast<T>(div_or_zero(ck, den));
-          arow_next = (j == k + 1) ? static_cast<T>(pubv[kPubAkk]) : arow;
+          arow_next = arow;
         } else {
-          if (tid < kk) Lnext[tid] = static_cast<T>(ucol(dt, dc0, dc1, colK1, tid));
+          if (tid < kk) Lnext[tid] = u1;
           if (tid == kk) Lnext[kk] = static_cast<T>(div_or_zero(ck1, den));
           if (j == r) arow_next = static_cast<T>(pubv[kPubAkk1]);
         }
@@ -617,22 +597,19 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       // (src/block_ldlt_decomposition.cc:500-587, 658-704)
       const int k1 = k + 1;
       if (r != k1) {
-        // columns k+1 and r of the finished panel rows, one row per thread
-        if (tid < kk) {
+        // columns k+1 and r of the finished panel rows (workgroup 0)
+        if (gi == 0 && tid < kk) {
           const int q = tid;
-          if (is_mine(k1)) sta(&A[(int64_t)(s + q) * lda + k1], Lr[q]);
-          if (is_mine(r)) {
-            if (q >= base) Wl[(q - base) * WS + (r - c0)] = static_cast<T>(colK1[q]);
-            sta(&A[(int64_t)(s + q) * lda + r], static_cast<T>(ucol(dt, dc0, dc1, colK1, q)));
-          }
+          sta(&A[(int64_t)(s + q) * lda + k1], Lr[q]);
+          sta(&A[(int64_t)(s + q) * lda + r], u1);
         }
-        if (j == r) take_col(colK1);
         // trailing matrix: new row/column r take the old row k+1 (= P1)
         if (own && j > r)
           sta(&A[(int64_t)r * lda + j], P1);
         else if (own && j > k1 && j < r)
           sta(&A[(int64_t)j * lda + r], P1);
         if (j == r) sta(&A[(int64_t)r * lda + r], static_cast<T>(a11));
+        swap_pos(k1, r);
         if (gi == 0 && tid == 0) {
           swl[2 * nswap] = k1;
           swl[2 * nswap + 1] = r;
@@ -641,21 +618,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         }
         ++nswap;
       }
-      // new W rows kk (current row) and kk+1 (candidate row), columns
-      // k+1 and r exchanged
-      T w0 = active ? Wl[(kk - base) * WS + tid] : T(0);
-      T w1 = c;
-      if (j == k1) {
-        w0 = static_cast<T>(wkr);
-        w1 = static_cast<T>(crr);
-      } else if (j == r) {
-        w0 = static_cast<T>(pubv[kPubWk1]);
-        w1 = static_cast<T>(ck1);
-      }
-      if (active) {
-        Wl[(kk - base) * WS + tid] = w0;
-        Wl[(kk + 1 - base) * WS + tid] = w1;
-      }
+      // new W rows kk (current row) and kk+1 (candidate row): each thread's
+      // own values (positions k+1 and r exchanged their holders)
+      const T w0 = w, w1 = c;
+      if (active) Wl[(kk + 1 - base) * WS + tid] = w1;
+      const double wkr = pubv[kPubWkr];
       const double a = wkk, bb = wkr, cc = crr;
       double itl, ibr, ioff;
       inverse_2x2_nc(a, bb, cc, &itl, &ibr, &ioff);
@@ -675,18 +642,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         sta(&A[(int64_t)k1 * lda + k1], static_cast<T>(cc));
         p.piv[k1] = -2;
       }
-      if (tid == 0) {
-        dt[kk] = 2;
-        dc0[kk] = itl;
-        dc1[kk] = ioff;
-        dt[kk + 1] = -2;
-        dc0[kk + 1] = ioff;
-        dc1[kk + 1] = ibr;
-      }
       if (use2) {
         const bool rk2 = (r == k + 2);
-        const double* src = rk2 ? colK1 : colK2;
-        if (tid < kk) Lnext[tid] = static_cast<T>(ucol(dt, dc0, dc1, src, tid));
+        if (tid < kk) Lnext[tid] = rk2 ? u1 : u2;
         const double wa = rk2 ? pubv[kPubWk1] : pubv[kPubWk2];
         const double wb = rk2 ? ck1 : ck2;
         double u0, u1;
@@ -694,7 +652,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         if (tid == kk) Lnext[kk] = static_cast<T>(u0);
         if (tid == kk + 1) Lnext[kk + 1] = static_cast<T>(u1);
         if (rk2)
-          arow_next = (j == k + 2) ? static_cast<T>(a11) : P1;
+          arow_next = P1;  // the new row k+2 is the old row k+1
         else
           arow_next = (j == r && r > k + 2) ? static_cast<T>(a12) : P2;
       }
@@ -775,8 +733,8 @@ hipError_t panel_max_groups(int* groups) {
 }
 
 size_t panel_exchange_records(int groups, int elem_bytes) {
-  const int bm = elem_bytes == 8 ? PanelCfg<double>::kMaxWidth : PanelCfg<float>::kMaxWidth;
-  return 2 * static_cast<size_t>(groups + 16 + 3 * bm);
+  (void)elem_bytes;
+  return 2 * static_cast<size_t>(groups + kNumPub);
 }
 
 template hipError_t launch_panel<double>(double*, const PanelArgs&, int, hipStream_t);
