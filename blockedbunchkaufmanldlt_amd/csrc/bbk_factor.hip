@@ -82,15 +82,15 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
   }
 }
 
-// Tile shape (measured on MI355X at m = 16384, K = 64 / 128): 64 x 64
-// tiles with four or more workgroups resident per CU let one workgroup's C
-// traffic overlap another's MFMAs (binary64 15.7 -> 25.1 TFLOP/s, binary32
-// 40 -> 59 TFLOP/s against 128 x 128 tiles); K chunks of 8; XCD-contiguous
-// tile order.
+// Tile shape (measured on MI355X at m = 16384, K = 128): 64 x 64 tiles
+// with eight workgroups resident per CU let one workgroup's C traffic
+// overlap another's MFMAs (binary64 34.6 TFLOP/s against 30.3 for 128 x 128
+// tiles and 33.6 with four resident; binary32 60.3 against 54.1); K chunks
+// of 8; XCD-contiguous supertile order.
 template <typename T>
 struct UpdateCfg {
   static constexpr int kTM = 64;
-  static constexpr int kMinBlocks = 4;
+  static constexpr int kMinBlocks = 8;
   static constexpr int kKC = 8;
 };
 

@@ -28,24 +28,24 @@ struct __attribute__((aligned(16))) XRec {
   double v;
 };
 
-// Panel width per type.  The panel's W rows live in LDS, (kLdsRows + 1) x
-// 256 elements (the 160 KB LDS bound); a binary64 panel is twice as wide:
-// once it passes row kRegRows, rows [0, kRegRows) move to registers (one
-// column per thread, 2 kRegRows VGPRs) and the LDS rows are reused for the
-// rest, so the trailing update runs at rank 128 in both types.
+// Panel width per type: 128.  The panel's W rows live in LDS, (kLdsRows +
+// 1) x 256 elements; once the panel passes row kRegRows, rows [0, kRegRows)
+// move to registers (one column per thread) and the LDS rows are reused for
+// the rest.  The register rows' products need no LDS traffic, so half of
+// the panel's rows sit there in both types.
 template <typename T>
 struct PanelCfg;
 template <>
 struct PanelCfg<double> {
   static constexpr int kLdsRows = 64;  // 65 x 256 x 8 B = 130 KB
-  static constexpr int kRegRows = 64;
+  static constexpr int kRegRows = 64;  // 128 VGPRs
   static constexpr int kMaxWidth = kLdsRows + kRegRows;
 };
 template <>
 struct PanelCfg<float> {
-  static constexpr int kLdsRows = 128;  // 129 x 256 x 4 B = 129 KB
-  static constexpr int kRegRows = 0;
-  static constexpr int kMaxWidth = kLdsRows;
+  static constexpr int kLdsRows = 64;  // 65 x 256 x 4 B = 65 KB
+  static constexpr int kRegRows = 64;  // 64 VGPRs
+  static constexpr int kMaxWidth = kLdsRows + kRegRows;
 };
 
 // Per-panel device state (ints).

@@ -158,21 +158,27 @@ __device__ __forceinline__ T panel_dot(const T* L, const T* Wl, const T (&wr)[NR
   T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   if constexpr (RR > 0) {
     if (base > 0) {
-      // L in blocks of 16, all loads of a block issued before its products
+      // L in blocks of 16 read as 16-byte vectors, all loads of a block
+      // issued before its products
       static_assert(RR % 16 == 0, "register rows come in blocks of 16");
-      typedef double d2 __attribute__((ext_vector_type(2)));
-      const d2* L2 = reinterpret_cast<const d2*>(L);
+      constexpr int VE = 16 / sizeof(T);  // elements per vector
+      typedef T vec_t __attribute__((ext_vector_type(VE)));
+      const vec_t* Lv = reinterpret_cast<const vec_t*>(L);
 #pragma unroll
       for (int b = 0; b < RR; b += 16) {
-        d2 lv[8];
+        T lv[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) lv[u] = L2[b / 2 + u];
+        for (int u = 0; u < 16 / VE; ++u) {
+          const vec_t t = Lv[b / VE + u];
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-          a0 += lv[u].x * wr[b + 2 * u];
-          a1 += lv[u].y * wr[b + 2 * u + 1];
-          a2 += lv[u + 1].x * wr[b + 2 * u + 2];
-          a3 += lv[u + 1].y * wr[b + 2 * u + 3];
+          for (int e = 0; e < VE; ++e) lv[u * VE + e] = t[e];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u += 4) {
+          a0 += lv[u] * wr[b + u];
+          a1 += lv[u + 1] * wr[b + u + 1];
+          a2 += lv[u + 2] * wr[b + u + 2];
+          a3 += lv[u + 3] * wr[b + u + 3];
         }
       }
     }
