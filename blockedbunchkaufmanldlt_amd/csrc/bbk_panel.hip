@@ -227,6 +227,78 @@ __device__ __forceinline__ T panel_dot(const T* L, const T* Wl, const T (&wr)[NR
   return (a0 + a1) + (a2 + a3);
 }
 
+// Two products at once over q < kk: d1 = sum_q L1[q] W(q, own column),
+// d2 = sum_q L2[q] W(q, own column) (same storage as panel_dot; W is read
+// once).  Used for the next pivot's row while an exchange is in flight.
+template <typename T, int WS, int RR, int NR>
+__device__ __forceinline__ void panel_dot2(const T* L1, const T* L2, const T* Wl,
+                                           const T (&wr)[NR], int tid, int kk, int base, T& d1,
+                                           T& d2) {
+  T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0);
+  constexpr int VE = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VE)));
+  if constexpr (RR > 0) {
+    if (base > 0) {
+      const vec_t* V1 = reinterpret_cast<const vec_t*>(L1);
+      const vec_t* V2 = reinterpret_cast<const vec_t*>(L2);
+#pragma unroll
+      for (int b = 0; b < RR; b += 8) {
+        T l1[8], l2[8];
+#pragma unroll
+        for (int u = 0; u < 8 / VE; ++u) {
+          const vec_t t1 = V1[b / VE + u], t2 = V2[b / VE + u];
+#pragma unroll
+          for (int e = 0; e < VE; ++e) {
+            l1[u * VE + e] = t1[e];
+            l2[u * VE + e] = t2[e];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          a0 += l1[u] * wr[b + u];
+          a1 += l1[u + 1] * wr[b + u + 1];
+          b0 += l2[u] * wr[b + u];
+          b1 += l2[u + 1] * wr[b + u + 1];
+        }
+      }
+    }
+  }
+  const T* Wc = Wl + tid;
+  const T* L1b = L1 + base;  // 16-byte aligned: L1, L2 are, base is 0 or RR
+  const T* L2b = L2 + base;
+  const int cnt = kk - base;
+  int q = 0;
+  for (; q + 8 <= cnt; q += 8) {
+    T l1[8], l2[8], wv[8];
+    const vec_t* V1 = reinterpret_cast<const vec_t*>(L1b + q);
+    const vec_t* V2 = reinterpret_cast<const vec_t*>(L2b + q);
+#pragma unroll
+    for (int u = 0; u < 8 / VE; ++u) {
+      const vec_t t1 = V1[u], t2 = V2[u];
+#pragma unroll
+      for (int e = 0; e < VE; ++e) {
+        l1[u * VE + e] = t1[e];
+        l2[u * VE + e] = t2[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) wv[u] = Wc[(q + u) * WS];
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      a0 += l1[u] * wv[u];
+      a1 += l1[u + 1] * wv[u + 1];
+      b0 += l2[u] * wv[u];
+      b1 += l2[u + 1] * wv[u + 1];
+    }
+  }
+  for (; q < cnt; ++q) {
+    a0 += L1b[q] * Wc[q * WS];
+    b0 += L2b[q] * Wc[q * WS];
+  }
+  d1 = a0 + a1;
+  d2 = b0 + b1;
+}
+
 enum PubId {
   kPubWk = 0, kPubAkk = 1, kPubWk1 = 2, kPubAkk1 = 3, kPubWk2 = 4, kPubWkr = 5,
   kPubCr = 8, kPubCk = 9, kPubCk1 = 10, kPubA11 = 11, kPubCk2 = 12, kPubA12 = 13,
@@ -247,6 +319,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   // L(k, panel) of this pivot / of the next, and of the candidate row
   __shared__ __attribute__((aligned(16))) T Lbuf[2][BM + 2];
   __shared__ __attribute__((aligned(16))) T Lr[BM + 2];
+  // U(q, k+1), U(q, k+2), q < kk: the candidates for the next pivot's L row
+  __shared__ __attribute__((aligned(16))) T Lu1[BM + 2];
+  __shared__ __attribute__((aligned(16))) T Lu2[BM + 2];
   __shared__ double pubv[kNumPub];
   __shared__ double red_v[2 * (NT / 64)];
   __shared__ int red_i[2 * (NT / 64)];
@@ -325,6 +400,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   CoherentLoad<T>::one(own ? &A[(int64_t)s * lda + j] : dummy, arow);
   if (!own) arow = T(0);
   int kk = 0, nswap = 0;
+  // the next pivot's row product, when it was formed during exchange 2
+  bool have_next = false;
+  T pnext = T(0);
   T wr[NR];
   int base = 0;
   // interchange of positions a < b: their holders swap positions
@@ -369,11 +447,13 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     T* const Lnext = Lbuf[lp ^ 1];
     // ---- phase A: W(k, j) for own j >= k, local argmax over j > k -------
     const bool active = own && j >= k;
-    T w = T(0);
+    T w = T(0), dotA = T(0);
     if (active) {
-      w = arow - panel_dot<T, WS, RR>(Lk, Wl, wr, tid, kk, base);
+      dotA = have_next ? pnext : panel_dot<T, WS, RR>(Lk, Wl, wr, tid, kk, base);
+      w = arow - dotA;
       Wl[(kk - base) * WS + tid] = w;
     }
+    have_next = false;
     BBK_PT(1);
     double v = 0.0;
     int vi = INT_MAX;
@@ -451,7 +531,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     const T* const pu1 = (wantu && has1) ? &A[(int64_t)(s + tid) * lda + k + 1] : dummy;
     T P1, P2 = T(0), u1 = T(0), u2 = T(0);
     int kind;
-    T c = T(0);
+    T c = T(0), pd1 = T(0), pd2 = T(0);
     double crr = 0.0, ck = 0.0, ck1 = 0.0, ck2 = 0.0, a11 = 0.0, a12 = 0.0;
     if (akk >= kAlpha * omega || omega == 0.0) {
       kind = 1;  // src/block_ldlt_decomposition.cc:92
@@ -472,7 +552,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (!active) a = T(0);
       if (!want1) P1 = T(0);
       if (!want2) P2 = T(0);
-      if (tid < kk) Lr[tid] = lr;
+      if (tid < kk) {
+        Lr[tid] = lr;
+        Lu1[tid] = u1;
+        Lu2[tid] = u2;
+      }
       __syncthreads();
       BBK_PT(6);
       if (active) {
@@ -503,6 +587,10 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         post(pubs + kPubCk2, ex, 0, static_cast<double>(c));
         post(pubs + kPubA12, ex, 0, static_cast<double>(P1));
       }
+      // while the records travel: the next pivot's row product over the
+      // finished rows q < kk for both possible next rows (k+1, k+2)
+      if (active && kk + 1 < bt && has1)
+        panel_dot2<T, WS, RR>(Lu1, Lu2, Wl, wr, tid, kk, base, pd1, pd2);
       const int npub = 2 + (has1 ? 2 : 0) + (has2 ? 2 : 0);
       const int total = G + npub;
       double sg = 0.0;
@@ -547,8 +635,13 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         p.piv[k] = 1;
       }
       if (has1) {
+        const T lkk = static_cast<T>(div_or_zero(pubv[kPubWk1], den));
         if (tid < kk) Lnext[tid] = u1;
-        if (tid == kk) Lnext[kk] = static_cast<T>(div_or_zero(pubv[kPubWk1], den));
+        if (tid == kk) Lnext[kk] = lkk;
+        if (kind == 3) {  // pd1 was formed during exchange 2
+          pnext = pd1 + lkk * w;
+          have_next = true;
+        }
       }
       size = 1;
     } else if (kind == -1) {
@@ -588,14 +681,19 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (has1) {
         if (r == k + 1) {
           // the new row k+1 is the old row k (the holder of k+1 held k)
+          const T lkk = static_cast<T>(div_or_zero(ck, den));
           if (tid < kk) Lnext[tid] = Lk[tid];
-          if (tid == kk) Lnext[kk] = static_cast<T>(div_or_zero(ck, den));
+          if (tid == kk) Lnext[kk] = lkk;
           arow_next = arow;
+          pnext = dotA + lkk * nw;
         } else {
+          const T lkk = static_cast<T>(div_or_zero(ck1, den));
           if (tid < kk) Lnext[tid] = u1;
-          if (tid == kk) Lnext[kk] = static_cast<T>(div_or_zero(ck1, den));
+          if (tid == kk) Lnext[kk] = lkk;
           if (j == r) arow_next = static_cast<T>(pubv[kPubAkk1]);
+          pnext = pd1 + lkk * nw;
         }
+        have_next = true;
       }
       size = 1;
     } else {
@@ -633,11 +731,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       double itl, ibr, ioff;
       inverse_2x2_nc(a, bb, cc, &itl, &ibr, &ioff);
       if (own && j >= k + 2) {
-        double u0, u1;
-        apply_2x2_nc(itl, ibr, ioff, static_cast<double>(w0), static_cast<double>(w1), &u0,
-                     &u1);
-        sta(&A[(int64_t)k * lda + j], static_cast<T>(u0));
-        sta(&A[(int64_t)k1 * lda + j], static_cast<T>(u1));
+        double y0, y1;
+        apply_2x2_nc(itl, ibr, ioff, static_cast<double>(w0), static_cast<double>(w1), &y0,
+                     &y1);
+        sta(&A[(int64_t)k * lda + j], static_cast<T>(y0));
+        sta(&A[(int64_t)k1 * lda + j], static_cast<T>(y1));
       }
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], static_cast<T>(a));
@@ -653,10 +751,13 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
         if (tid < kk) Lnext[tid] = rk2 ? u1 : u2;
         const double wa = rk2 ? pubv[kPubWk1] : pubv[kPubWk2];
         const double wb = rk2 ? ck1 : ck2;
-        double u0, u1;
-        apply_2x2_nc(itl, ibr, ioff, wa, wb, &u0, &u1);
-        if (tid == kk) Lnext[kk] = static_cast<T>(u0);
-        if (tid == kk + 1) Lnext[kk + 1] = static_cast<T>(u1);
+        double x0, x1;
+        apply_2x2_nc(itl, ibr, ioff, wa, wb, &x0, &x1);
+        const T l0 = static_cast<T>(x0), l1 = static_cast<T>(x1);
+        if (tid == kk) Lnext[kk] = l0;
+        if (tid == kk + 1) Lnext[kk + 1] = l1;
+        pnext = (rk2 ? pd1 : pd2) + l0 * w0 + l1 * w1;
+        have_next = true;
         if (rk2)
           arow_next = P1;  // the new row k+2 is the old row k+1
         else
