@@ -383,10 +383,25 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
   const int I = idx - J * (J + 1) / 2;
   const int i0 = I * TB, j0 = J * TB;
   const int t = threadIdx.x;
-  for (int e = t; e < TB * TB; e += 256) {
-    const int r = e / TB, c = e % TB;
-    const int i = i0 + r, jj = j0 + c;
-    tile[r][c] = (i < n && jj < n && jj >= i) ? A[(int64_t)i * lda + jj] : 0.0;
+  // all 16 loads of a thread in flight together: unconditional loads from
+  // clamped addresses, masked afterwards
+  {
+    static_assert(TB * TB % 256 == 0, "tile elements per thread");
+    constexpr int PER = TB * TB / 256;
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = t + 256 * u;
+      const int i = min(i0 + e / TB, n - 1), jj = min(j0 + e % TB, n - 1);
+      v[u] = A[(int64_t)i * lda + jj];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = t + 256 * u;
+      const int r = e / TB, c = e % TB;
+      const int i = i0 + r, jj = j0 + c;
+      tile[r][c] = (i < n && jj < n && jj >= i) ? v[u] : 0.0;
+    }
   }
   for (int e = t; e < TB * nr; e += 256) {
     const int r = e / nr, q = e % nr;
