@@ -4,8 +4,9 @@
 // Solve: x = P L^-T D^-1 L^-1 P^T b with (L D L^T)(i,j) = A(perm[i],perm[j]).
 // L is stored transposed in the upper triangle (L(j,i) = F(i,j), i < j,
 // except F(i,i+1) of a 2x2 pivot, which is D).  Both triangular sweeps
-// run as one cooperative launch each: 64-row blocks, workgroups take
-// blocks in dependency order and publish each finished block with a flag.
+// run as one cooperative launch each: 128-row blocks, workgroups take
+// blocks in dependency order and publish each finished half block as
+// tagged records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,7 +20,7 @@
 
 namespace bbk {
 
-constexpr int kSolveBlock = 64;
+constexpr int kSolveBlock = 128;
 constexpr int kSolveThreads = 256;
 constexpr int kSolveMaxRhs = 4;  // right-hand sides per launch
 
@@ -47,30 +48,6 @@ __device__ __forceinline__ float readlane_t<float>(float v, int lane) {
 // (one per row and right-hand side, index row * kSolveMaxRhs + q): a
 // reader polls the record itself, so publishing needs no flag, fence or
 // second round trip.  Tags are unique per launch.
-//
-// Lane l of a wavefront polls row (l >> 2), right-hand side (l & 3) of
-// the 16 rows the wavefront consumes; on return every lane holds its value
-// (0 for q >= NR or rows outside the matrix).
-template <int NR>
-__device__ __forceinline__ double poll_rows(const XRec* rec, int row0, int nrows,
-                                            int tag) {
-  const int lane = threadIdx.x & 63;
-  const int r = row0 + (lane >> 2), q = lane & 3;
-  bool need = q < NR && (lane >> 2) < nrows;
-  double v = 0.0;
-  const XRec* p = rec + (int64_t)r * kSolveMaxRhs + q;
-  while (__ballot(need) != 0) {
-    if (need) {
-      int aux;
-      double x;
-      if (xget(p, tag, &aux, &x)) {
-        v = x;
-        need = false;
-      }
-    }
-  }
-  return v;
-}
 
 // UB blocks at once: out[u] holds row row0[u] + (lane >> 2), rhs (lane & 3)
 // of block u (0 for q >= NR or rows >= nlim).  All UB loads of a lane are
@@ -135,226 +112,414 @@ __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row
 
 __device__ __forceinline__ double bcast(double v, int lane) { return readlane_d(v, lane); }
 
-// 16 consecutive rows i0 .. i0+15 of column j (forward: L(j, i) = F(i, j)).
+// The diagonal block's unit L, strictly lower part packed by columns of
+// the upper-triangle storage: Lp[c (c - 1) / 2 + r] = L(c, r) = F(b0 + r,
+// b0 + c), r < c < bn, with the D entry F(k, k+1) of a 2x2 pivot read as 0
+// (rows or columns >= bn read as 0).  The 128 x 128 tile is read as rows,
+// 32 loads per thread in flight together, in two halves.
 template <typename T>
-__device__ __forceinline__ void load_col16(T (&f)[16], const T* F, int64_t lda, int i0,
-                                           int j, bool ok) {
+__device__ __forceinline__ void stage_diag_block(T* Lp, const T* __restrict__ F, int64_t lda,
+                                                 const int* __restrict__ piv, int b0, int bn,
+                                                 int n) {
+  constexpr int NB = kSolveBlock;
+  constexpr int PER = NB * NB / kSolveThreads / 2;
 #pragma unroll
-  for (int ii = 0; ii < 16; ++ii) f[ii] = ok ? F[(int64_t)(i0 + ii) * lda + j] : T(0);
+  for (int h = 0; h < 2; ++h) {
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + kSolveThreads * (u + PER * h);
+      const int r = e / NB, c = e % NB;
+      v[u] = F[(int64_t)min(b0 + r, n - 1) * lda + min(b0 + c, n - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + kSolveThreads * (u + PER * h);
+      const int r = e / NB, c = e % NB;
+      if (r < c) {
+        T x = c < bn ? v[u] : T(0);
+        if (c == r + 1 && c < bn && piv[b0 + r] == 2) x = T(0);
+        Lp[c * (c - 1) / 2 + r] = x;
+      }
+    }
+  }
 }
-// 16 consecutive columns j0 .. j0+15 of row i (backward: L(j, i) = F(i, j)),
-// columns >= n read as 0.
-template <typename T>
-__device__ __forceinline__ void load_row16(T (&f)[16], const T* F, int64_t lda, int i,
-                                           int j0, int n, bool ok) {
+
+// D^-1 z for row i (binary64 arithmetic): a 1x1 pivot divides by d; row k or
+// k+1 of a 2x2 pivot (k, k+1) uses LAPACK dsytrs's scaled formula.
+template <typename T, int NR>
+__device__ __forceinline__ void dsolve_row(const T* __restrict__ F, int64_t lda,
+                                           const int* __restrict__ piv,
+                                           const T* __restrict__ Z, int64_t ldz, int i,
+                                           T (&wv)[NR]) {
+  const int pv = piv[i];
+  if (pv == 2 || pv == -2) {
+    const int k = pv == 2 ? i : i - 1;
+    const double a = F[(int64_t)k * lda + k];
+    const double b = F[(int64_t)k * lda + k + 1];
+    const double c = F[(int64_t)(k + 1) * lda + k + 1];
+    const double akm1 = a / b, ak = c / b;
+    const double denom = akm1 * ak - 1.0;
 #pragma unroll
-  for (int c = 0; c < 16; ++c) f[c] = (ok && j0 + c < n) ? F[(int64_t)i * lda + j0 + c] : T(0);
+    for (int q = 0; q < NR; ++q) {
+      const double bkm1 = (double)Z[(int64_t)k * ldz + q] / b;
+      const double bk = (double)Z[(int64_t)(k + 1) * ldz + q] / b;
+      wv[q] = (T)(pv == 2 ? (ak * bkm1 - bk) / denom : (akm1 * bk - bkm1) / denom);
+    }
+  } else {
+    const double d = F[(int64_t)i * lda + i];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) wv[q] = (T)((double)Z[(int64_t)i * ldz + q] / d);
+  }
+}
+
+// Serial substitution over 64 unknowns, one per lane (v): step k takes
+// unknown s = k (ASC) or 63 - k, broadcasts it with readlane and lanes with
+// s < lane (ASC) or s > lane subtract L * v_s, L = Lp[idx(s)].  The L values
+// come from LDS 16 at a time, the next 16 loaded ahead of the chain.
+template <typename T, int NR, bool ASC, typename Idx>
+__device__ __forceinline__ void serial64(T (&v)[NR], const T* Lp, int lane, Idx idx) {
+  T cur[16], nxt[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) cur[u] = Lp[idx(ASC ? u : 63 - u)];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (c < 3) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int k = 16 * (c + 1) + u;
+        nxt[u] = Lp[idx(ASC ? k : 63 - k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int k = 16 * c + u;
+      const int sidx = ASC ? k : 63 - k;
+      const T l = (ASC ? sidx < lane : sidx > lane) ? cur[u] : T(0);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) v[q] -= l * readlane_t<T>(v[q], sidx);
+    }
+    if (c < 3) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) cur[u] = nxt[u];
+    }
+  }
 }
 
 // Forward sweep: Z = L^-1 P^T B for NR (<= kSolveMaxRhs) columns.
-// Workgroup takes 64-row blocks J = blockIdx.x, blockIdx.x + gridDim.x, ...;
-// wavefront w accumulates rows 16w .. 16w+15 of every earlier block I < J
-// into the block's 64 columns (one lane each), F tiles prefetched one block
-// ahead; wavefront 0 then solves the diagonal block serially (readlane
-// broadcast) and publishes the block's records.
+// Workgroup takes 128-row blocks J = blockIdx.x, blockIdx.x + gridDim.x, ...
+// (the diagonal block staged in LDS first).  Lane l holds columns j0 + l
+// (A) and j0 + 64 + l (B); wavefront w accumulates rows 32w .. 32w+31 of
+// every earlier block I < J, polling their records 16 rows at a time.  Then
+// wavefront 0 solves the top 64 unknowns serially (readlane broadcast) and
+// publishes them at once, so later blocks start on them; wavefront 1
+// applies them to the bottom 64 and solves those.
 template <typename T, int NR>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ B, int64_t ldb,
     T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk) {
-  constexpr int NB = kSolveBlock;
+  constexpr int NB = kSolveBlock, H = NB / 2, NPK = NB * (NB - 1) / 2;
+  __shared__ T Lp[NPK];
   __shared__ T part[4][NB][NR];
+  __shared__ T ztop[H][NR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
     const int j0 = J * NB;
     const int jn = min(NB, n - j0);
-    const int j = j0 + lane;
-    const bool jok = lane < jn;
-    // wavefront 0: right-hand side and the diagonal block's L column
+    const int cA = j0 + lane, cB = j0 + H + lane;
+    const bool okA = lane < jn, okB = H + lane < jn;
+    stage_diag_block<T>(Lp, F, lda, piv, j0, jn, n);
     T bj[NR];
-    T Lc[NB];
-    if (w == 0) {
-      const int pj = jok ? perm[j] : 0;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) bj[q] = jok ? B[(int64_t)pj * ldb + q] : T(0);
+    for (int q = 0; q < NR; ++q) bj[q] = T(0);
+    if (w == 0 && okA) {
+      const int pj = perm[cA];
 #pragma unroll
-      for (int ii = 0; ii < NB; ++ii)
-        Lc[ii] = (jok && ii < lane) ? F[(int64_t)(j0 + ii) * lda + j] : T(0);
-      // inside a 2x2 pivot (k, k+1), F(k, k+1) holds D, not L
-      if (jok && lane > 0 && piv[j - 1] == 2) {
+      for (int q = 0; q < NR; ++q) bj[q] = B[(int64_t)pj * ldb + q];
+    } else if (w == 1 && okB) {
+      const int pj = perm[cB];
 #pragma unroll
-        for (int ii = 0; ii < NB; ++ii)
-          if (ii == lane - 1) Lc[ii] = T(0);
-      }
+      for (int q = 0; q < NR; ++q) bj[q] = B[(int64_t)pj * ldb + q];
     }
-    T acc[NR];
+    // 2x2 pivot across the block edge: L(j0, j0 - 1) = 0 (row j0 - 1 is row
+    // 127 of block J - 1: wavefront 3, second chunk, last row; column j0 is
+    // lane 0's column A)
+    const bool edge2 = J > 0 && w == 3 && lane == 0 && piv[j0 - 1] == 2;
+    T acc[2][NR];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = T(0);
-    // UB earlier blocks per step: their F tiles and their records are in
-    // flight together (one wait per step)
-    constexpr int UB = sizeof(T) == 8 ? 2 : 4;
+    for (int q = 0; q < NR; ++q) acc[0][q] = acc[1][q] = T(0);
+    // UB earlier blocks per step (2 UB chunks of 16 rows): F tiles and
+    // records in flight together, one wait per step
+    constexpr int UB = sizeof(T) == 8 ? 1 : 2;
+    const int cAc = min(cA, n - 1), cBc = min(cB, n - 1);
     for (int I = 0; I < J; I += UB) {
-      T fc[UB][16];
-      int row0[UB];
+      T fa[2 * UB][16], fb[2 * UB][16];
+      int row0[2 * UB];
 #pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        const int Iu = I + u;
-        load_col16(fc[u], F, lda, Iu * NB + w * 16, j, jok && Iu < J);
-        // row j0 - 1 with j = j0: 2x2 coupling across the block edge
-        if (Iu == J - 1 && w == 3 && lane == 0 && piv[j0 - 1] == 2) fc[u][15] = T(0);
-        row0[u] = min(Iu, J - 1) * NB + w * 16;  // past J - 1: fc is 0
+      for (int k = 0; k < 2 * UB; ++k) {
+        const int Iu = I + k / 2;
+        const int rb = Iu * NB + 32 * w + 16 * (k & 1);
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+          const int64_t ro = (int64_t)min(rb + ii, n - 1) * lda;
+          fa[k][ii] = F[ro + cAc];
+          fb[k][ii] = F[ro + cBc];
+        }
+        row0[k] = min(Iu, J - 1) * NB + 32 * w + 16 * (k & 1);
       }
-      double zv[UB];
-      poll_rows_multi<NR, UB>(rec, row0, n, tag, zv);
 #pragma unroll
-      for (int u = 0; u < UB; ++u)
+      for (int k = 0; k < 2 * UB; ++k) {
+        const bool live = I + k / 2 < J;
+#pragma unroll
+        for (int ii = 0; ii < 16; ++ii) {
+          if (!(live && okA)) fa[k][ii] = T(0);
+          if (!(live && okB)) fb[k][ii] = T(0);
+        }
+        if (edge2 && I + k / 2 == J - 1 && (k & 1)) fa[k][15] = T(0);
+      }
+      double zv[2 * UB];
+      poll_rows_multi<NR, 2 * UB>(rec, row0, n, tag, zv);
+#pragma unroll
+      for (int k = 0; k < 2 * UB; ++k)
 #pragma unroll
         for (int ii = 0; ii < 16; ++ii) {
 #pragma unroll
-          for (int q = 0; q < NR; ++q)
-            acc[q] += fc[u][ii] * static_cast<T>(bcast(zv[u], ii * 4 + q));
+          for (int q = 0; q < NR; ++q) {
+            const T z = static_cast<T>(bcast(zv[k], ii * 4 + q));
+            acc[0][q] += fa[k][ii] * z;
+            acc[1][q] += fb[k][ii] * z;
+          }
         }
     }
 #pragma unroll
-    for (int q = 0; q < NR; ++q) part[w][lane][q] = acc[q];
+    for (int q = 0; q < NR; ++q) {
+      part[w][lane][q] = acc[0][q];
+      part[w][H + lane][q] = acc[1][q];
+    }
     __syncthreads();
     if (w == 0) {
       T v[NR];
 #pragma unroll
       for (int q = 0; q < NR; ++q)
-        v[q] = jok ? bj[q] - (part[0][lane][q] + part[1][lane][q] + part[2][lane][q] +
-                              part[3][lane][q])
+        v[q] = okA ? bj[q] - ((part[0][lane][q] + part[1][lane][q]) +
+                              (part[2][lane][q] + part[3][lane][q]))
                    : T(0);
+      const int cb = lane * (lane - 1) / 2;  // L(lane, r) at Lp[cb + r], r < lane
+      serial64<T, NR, true>(v, Lp, lane, [&](int r) { return min(cb + r, NPK - 1); });
 #pragma unroll
-      for (int ii = 0; ii < NB; ++ii) {
-        if (ii < jn) {
-#pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            const T zi = readlane_t<T>(v[q], ii);
-            v[q] -= Lc[ii] * zi;  // Lc[ii] == 0 unless ii < lane
-          }
-        }
-      }
-      if (jok) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          Z[(int64_t)j * ldz + q] = v[q];
-          xput(rec + (int64_t)j * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+      for (int q = 0; q < NR; ++q) {
+        ztop[lane][q] = v[q];
+        if (okA) {
+          Z[(int64_t)cA * ldz + q] = v[q];
+          xput(rec + (int64_t)cA * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
         }
       }
     }
-    __syncthreads();  // part[] is reused by the next block
+    __syncthreads();
+    if (w == 1) {
+      T v[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        v[q] = okB ? bj[q] - ((part[0][H + lane][q] + part[1][H + lane][q]) +
+                              (part[2][H + lane][q] + part[3][H + lane][q]))
+                   : T(0);
+      const int c = H + lane;
+      const int cb = c * (c - 1) / 2;  // L(c, r) at Lp[cb + r], r < c
+      // the top unknowns applied in chunks of 8 (loads ahead of products)
+#pragma unroll
+      for (int i0 = 0; i0 < H; i0 += 8) {
+        T lc[8], zc[8][NR];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          lc[u] = Lp[cb + i0 + u];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) zc[u][q] = ztop[i0 + u][q];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) v[q] -= lc[u] * zc[u][q];
+      }
+      serial64<T, NR, true>(v, Lp, lane, [&](int r) { return min(cb + H + r, NPK - 1); });
+      if (okB) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          Z[(int64_t)cB * ldz + q] = v[q];
+          xput(rec + (int64_t)cB * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+        }
+      }
+    }
+    __syncthreads();  // Lp, part, ztop are reused by the next block
   }
 }
 
 // Backward sweep with the D solve fused in front:
-// V = L^-T D^-1 Z, X[perm[i]] = V[i].  Block I consumes blocks J > I;
-// lane = row, wavefront w covers columns 16w .. 16w+15 of each block J.
+// V = L^-T D^-1 Z, X[perm[i]] = V[i].  Block I consumes blocks J > I; lane
+// l holds rows i0 + l (A) and i0 + 64 + l (B), wavefront w covers columns
+// 32w .. 32w+31 of each block J (16 at a time, contiguous per row).  The
+// bottom 64 unknowns are solved first (wavefront 1) and published, then
+// applied to the top 64, which wavefront 0 solves.
 template <typename T, int NR>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ Z, int64_t ldz,
     T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk) {
-  constexpr int NB = kSolveBlock;
+  constexpr int NB = kSolveBlock, H = NB / 2, NPK = NB * (NB - 1) / 2;
+  __shared__ T Lp[NPK];
   __shared__ T part[4][NB][NR];
+  __shared__ T vbot[H][NR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
     const int i0 = I * NB;
     const int in = min(NB, n - i0);
-    const int i = i0 + lane;
-    const bool iok = lane < in;
+    const int rA = i0 + lane, rB = i0 + H + lane;
+    const bool okA = lane < in, okB = H + lane < in;
+    stage_diag_block<T>(Lp, F, lda, piv, i0, in, n);
     T wv[NR];
-    T Lr[NB];
-    if (w == 0) {
 #pragma unroll
-      for (int q = 0; q < NR; ++q) wv[q] = T(0);
-      if (iok) {
-        const int pv = piv[i];
-        if (pv == 2 || pv == -2) {
-          // 2x2 block (k, k+1): LAPACK dsytrs's scaled formula, binary64
-          const int k = pv == 2 ? i : i - 1;
-          const double a = F[(int64_t)k * lda + k];
-          const double b = F[(int64_t)k * lda + k + 1];
-          const double c = F[(int64_t)(k + 1) * lda + k + 1];
-          const double akm1 = a / b, ak = c / b;
-          const double denom = akm1 * ak - 1.0;
+    for (int q = 0; q < NR; ++q) wv[q] = T(0);
+    if (w == 0 && okA) dsolve_row<T, NR>(F, lda, piv, Z, ldz, rA, wv);
+    if (w == 1 && okB) dsolve_row<T, NR>(F, lda, piv, Z, ldz, rB, wv);
+    // 2x2 pivot across the block edge: L(i0 + 128, i0 + 127) = 0 (row
+    // i0 + 127 is lane 63's row B; column i0 + 128 is the first column of
+    // block I + 1: wavefront 0, first chunk, first column)
+    const bool edge2 = w == 0 && lane == H - 1 && okB && I + 1 < nblk && piv[rB] == 2;
+    T acc[2][NR];
 #pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            const double bkm1 = (double)Z[(int64_t)k * ldz + q] / b;
-            const double bk = (double)Z[(int64_t)(k + 1) * ldz + q] / b;
-            wv[q] = (T)(pv == 2 ? (ak * bkm1 - bk) / denom : (akm1 * bk - bkm1) / denom);
+    for (int q = 0; q < NR; ++q) acc[0][q] = acc[1][q] = T(0);
+    constexpr int UB = sizeof(T) == 8 ? 1 : 2;
+    const int64_t roA = (int64_t)min(rA, n - 1) * lda, roB = (int64_t)min(rB, n - 1) * lda;
+    // Each lane reads 16 consecutive columns of its two rows: as 16-byte
+    // vectors when the rows are 16-byte aligned (one address per 16 bytes
+    // instead of per element: the rows differ per lane, so every load
+    // instruction touches 64 cache lines)
+    constexpr int VE = 16 / sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VE)));
+    const bool vec_ok = lda % VE == 0 && (reinterpret_cast<uintptr_t>(F) & 15) == 0;
+    for (int J = nblk - 1; J > I; J -= UB) {
+      T fa[2 * UB][16], fb[2 * UB][16];
+      int row0[2 * UB];
+#pragma unroll
+      for (int k = 0; k < 2 * UB; ++k) {
+        const int Ju = J - k / 2;
+        const int cb = max(Ju, I + 1) * NB + 32 * w + 16 * (k & 1);
+        if (vec_ok && cb + 16 <= n) {
+          const vec_t* pa = reinterpret_cast<const vec_t*>(F + roA + cb);
+          const vec_t* pb = reinterpret_cast<const vec_t*>(F + roB + cb);
+#pragma unroll
+          for (int u = 0; u < 16 / VE; ++u) {
+            const vec_t a = pa[u], b = pb[u];
+#pragma unroll
+            for (int e = 0; e < VE; ++e) {
+              fa[k][u * VE + e] = a[e];
+              fb[k][u * VE + e] = b[e];
+            }
           }
         } else {
-          const double d = F[(int64_t)i * lda + i];
 #pragma unroll
-          for (int q = 0; q < NR; ++q) wv[q] = (T)((double)Z[(int64_t)i * ldz + q] / d);
+          for (int cc = 0; cc < 16; ++cc) {
+            const int col = min(cb + cc, n - 1);
+            fa[k][cc] = F[roA + col];
+            fb[k][cc] = F[roB + col];
+          }
         }
+        row0[k] = max(Ju, I + 1) * NB + 32 * w + 16 * (k & 1);
       }
 #pragma unroll
-      for (int jj = 0; jj < NB; ++jj)
-        Lr[jj] = (iok && jj > lane && jj < in) ? F[(int64_t)i * lda + i0 + jj] : T(0);
-      if (iok && lane + 1 < in && piv[i] == 2) {
+      for (int k = 0; k < 2 * UB; ++k) {
+        const int Ju = J - k / 2;
+        const int cb = Ju * NB + 32 * w + 16 * (k & 1);
+        const bool live = Ju > I;
 #pragma unroll
-        for (int jj = 0; jj < NB; ++jj)
-          if (jj == lane + 1) Lr[jj] = T(0);
+        for (int cc = 0; cc < 16; ++cc) {
+          const bool colok = live && cb + cc < n;
+          if (!(colok && okA)) fa[k][cc] = T(0);
+          if (!(colok && okB)) fb[k][cc] = T(0);
+        }
+        if (edge2 && Ju == I + 1 && (k & 1) == 0) fb[k][0] = T(0);
       }
-    }
-    T acc[NR];
+      double vv[2 * UB];
+      poll_rows_multi<NR, 2 * UB>(rec, row0, n, tag, vv);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = T(0);
-    constexpr int UB = sizeof(T) == 8 ? 2 : 4;
-    for (int J = nblk - 1; J > I; J -= UB) {
-      T fc[UB][16];
-      int row0[UB];
+      for (int k = 0; k < 2 * UB; ++k)
 #pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        const int Ju = J - u;
-        const int c0 = Ju * NB + w * 16;
-        load_row16(fc[u], F, lda, i, c0, n, iok && Ju > I);
-        // column i + 1 with i = i0 + 63: 2x2 coupling across the block edge
-        if (Ju == I + 1 && w == 0 && lane == 63 && piv[i] == 2) fc[u][0] = T(0);
-        row0[u] = max(Ju, I + 1) * NB + w * 16;  // at or below I: fc is 0
-      }
-      double vv[UB];
-      poll_rows_multi<NR, UB>(rec, row0, n, tag, vv);
+        for (int cc = 0; cc < 16; ++cc) {
 #pragma unroll
-      for (int u = 0; u < UB; ++u)
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-#pragma unroll
-          for (int q = 0; q < NR; ++q)
-            acc[q] += fc[u][c] * static_cast<T>(bcast(vv[u], c * 4 + q));
+          for (int q = 0; q < NR; ++q) {
+            const T v = static_cast<T>(bcast(vv[k], cc * 4 + q));
+            acc[0][q] += fa[k][cc] * v;
+            acc[1][q] += fb[k][cc] * v;
+          }
         }
     }
 #pragma unroll
-    for (int q = 0; q < NR; ++q) part[w][lane][q] = acc[q];
+    for (int q = 0; q < NR; ++q) {
+      part[w][lane][q] = acc[0][q];
+      part[w][H + lane][q] = acc[1][q];
+    }
+    __syncthreads();
+    if (w == 1) {
+      T v[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        v[q] = okB ? wv[q] - ((part[0][H + lane][q] + part[1][H + lane][q]) +
+                              (part[2][H + lane][q] + part[3][H + lane][q]))
+                   : T(0);
+      const int r = H + lane;  // L(c, r) at Lp[c (c - 1) / 2 + r], c = H + cc > r
+      serial64<T, NR, false>(v, Lp, lane, [&](int cc) {
+        return min((H + cc) * (H + cc - 1) / 2 + r, NPK - 1);
+      });
+#pragma unroll
+      for (int q = 0; q < NR; ++q) vbot[lane][q] = okB ? v[q] : T(0);
+      if (okB) {
+        const int pi = perm[rB];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          X[(int64_t)pi * ldx + q] = v[q];
+          xput(rec + (int64_t)rB * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+        }
+      }
+    }
     __syncthreads();
     if (w == 0) {
       T v[NR];
 #pragma unroll
       for (int q = 0; q < NR; ++q)
-        v[q] = wv[q] - (part[0][lane][q] + part[1][lane][q] + part[2][lane][q] +
-                        part[3][lane][q]);
+        v[q] = okA ? wv[q] - ((part[0][lane][q] + part[1][lane][q]) +
+                              (part[2][lane][q] + part[3][lane][q]))
+                   : T(0);
+      // the bottom unknowns applied in chunks of 8 (loads ahead of products)
 #pragma unroll
-      for (int jj = NB - 1; jj >= 0; --jj) {
-        if (jj < in) {
+      for (int c0 = 0; c0 < H; c0 += 8) {
+        T lc[8], vc[8][NR];
 #pragma unroll
-          for (int q = 0; q < NR; ++q) {
-            const T vj = readlane_t<T>(v[q], jj);
-            v[q] -= Lr[jj] * vj;  // Lr[jj] == 0 unless jj > lane
-          }
+        for (int u = 0; u < 8; ++u) {
+          const int c = H + c0 + u;
+          lc[u] = Lp[c * (c - 1) / 2 + lane];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) vc[u][q] = vbot[c0 + u][q];
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+          for (int q = 0; q < NR; ++q) v[q] -= lc[u] * vc[u][q];
       }
-      if (iok) {
-        const int pi = perm[i];
+      // L(c, lane) at Lp[c (c - 1) / 2 + lane], c > lane (c = 0 never
+      // contributes)
+      serial64<T, NR, false>(v, Lp, lane, [&](int c) {
+        return min(c * (c - 1) / 2 + lane, NPK - 1);
+      });
+      if (okA) {
+        const int pi = perm[rA];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
           X[(int64_t)pi * ldx + q] = v[q];
-          xput(rec + (int64_t)i * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+          xput(rec + (int64_t)rA * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
         }
       }
     }
-    __syncthreads();
+    __syncthreads();  // Lp, part, vbot are reused by the next block
   }
 }
 
