@@ -336,7 +336,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"FP64 blocked Bunch-Kaufman LDL^T, n={n}, "
                                f"A=Q diag(lambda) Q^T (cond 1e6)",
-                   "n": n, "lda": lda, "panel_width": 64, "parallelism": "single GPU"},
+                   "n": n, "lda": lda, "panel_width": -(-n // int(panels_per_step)),
+                   "parallelism": "single GPU"},
         "roofline": dominant,
         "roofline_update": roof_update,
         "roofline_panel": roof_panel,

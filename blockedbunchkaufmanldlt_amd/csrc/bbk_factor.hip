@@ -61,9 +61,17 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
     T* __restrict__ A, int64_t lda, const T* __restrict__ Wg, int64_t ldw,
     const int* __restrict__ state, const int* __restrict__ swaps, int n, int gemm_blocks) {
   constexpr int NE = 2 * (kPanelMaxWidth + 1);
-  __shared__ T Us[2][KC * (TM + 4)];
-  __shared__ T Ws[2][KC * (TM + 4)];
-  __shared__ int cols[NE], slot[NE], content[NE];
+  constexpr int kStage = 2 * KC * (TM + 4);  // elements of one operand's two buffers
+  static_assert(3 * NE * sizeof(int) <= 2 * kStage * sizeof(T),
+                "the swap role's index arrays share the operand buffers");
+  // one LDS block serves both roles (a workgroup takes one), so its size
+  // is the GEMM staging alone and eight workgroups fit per CU
+  __shared__ __align__(16) T stage[2 * kStage];
+  T(*Us)[KC * (TM + 4)] = reinterpret_cast<T(*)[KC * (TM + 4)]>(stage);
+  T(*Ws)[KC * (TM + 4)] = reinterpret_cast<T(*)[KC * (TM + 4)]>(stage + kStage);
+  int* cols = reinterpret_cast<int*>(stage);
+  int* slot = cols + NE;
+  int* content = slot + NE;
   const int s = state[PanelState::kStart];
   const int bw = state[PanelState::kWidth];
   int idx = blockIdx.x;

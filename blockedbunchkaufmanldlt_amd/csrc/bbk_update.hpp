@@ -24,6 +24,8 @@ struct Mfma<double> {
   static __device__ __forceinline__ int row(int lane, int v) {
     return (lane >> 4) + 4 * v;
   }
+  static __device__ __forceinline__ int lane_row(int lane) { return lane >> 4; }
+  static constexpr int v_row(int v) { return 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -35,6 +37,8 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) {
     return 4 * (lane >> 4) + v;
   }
+  static __device__ __forceinline__ int lane_row(int lane) { return 4 * (lane >> 4); }
+  static constexpr int v_row(int v) { return v; }
 };
 
 __device__ __forceinline__ void tri_tile(int idx, int* I, int* J) {
@@ -80,6 +84,44 @@ __device__ __forceinline__ void supertile_of(int idx, int T, int* I, int* J) {
   }
 }
 
+// C tile element (ti, tj, v) of a lane: row wr WT + 16 ti + Mfma::row(lane, v),
+// column wc WT + 16 tj + lane % 16 of the tile.  Its address is a
+// wave-uniform part (tile origin, wavefront quadrant, ti, tj, v) plus one
+// 32-bit lane offset, so the sixteen addresses live in scalar registers and
+// the accumulators keep the vector registers.  FULL: a tile wholly above the
+// diagonal and inside the matrix, loaded and stored without checks.
+template <typename T, int TM, bool FULL>
+__device__ __forceinline__ void c_tile_io(T* __restrict__ A, int64_t lda, int n, int i0, int j0,
+                                          int wr, int wc, int lane,
+                                          typename Mfma<T>::acc_t (*acc)[TM / 32], bool store) {
+  constexpr int WT = TM / 2, TI = WT / 16;
+  const int ib = i0 + wr * WT, jb = j0 + wc * WT;
+  char* base = reinterpret_cast<char*>(A + (int64_t)ib * lda + jb);
+  const unsigned lofs =
+      (unsigned)((Mfma<T>::lane_row(lane) * lda + (lane & 15)) * (int64_t)sizeof(T));
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TI; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        T* p = reinterpret_cast<T*>(
+            base + ((int64_t)(ti * 16 + Mfma<T>::v_row(v)) * lda + tj * 16) * (int64_t)sizeof(T) +
+            lofs);
+        bool ok = true;
+        if (!FULL) {
+          const int i = ib + ti * 16 + Mfma<T>::row(lane, v);
+          const int jj = jb + tj * 16 + (lane & 15);
+          ok = i < n && jj < n && jj >= i;
+        }
+        if (store) {
+          if (ok) *p = acc[ti][tj][v];
+        } else {
+          acc[ti][tj][v] = ok ? *p : T(0);
+        }
+      }
+}
+
 template <typename T, int TM, int KC, bool SUPER = true>
 __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
                                             const T* __restrict__ Wg, int64_t ldw, int s,
@@ -103,40 +145,35 @@ __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
     if (J >= tiles) return;
   }
   const int i0 = e + I * TM, j0 = e + J * TM;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wr = wave >> 1, wc = wave & 1;
+  const bool full = I < J && j0 + TM <= n;
 
   acc_t acc[TI][TI];
-#pragma unroll
-  for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < TI; ++tj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * WT + ti * 16 + Mfma<T>::row(lane, v);
-        const int jj = j0 + wc * WT + tj * 16 + (lane & 15);
-        acc[ti][tj][v] = (i < n && jj < n && jj >= i) ? A[(int64_t)i * lda + jj] : T(0);
-      }
+  if (full)
+    c_tile_io<T, TM, true>(A, lda, n, i0, j0, wr, wc, lane, acc, false);
+  else
+    c_tile_io<T, TM, false>(A, lda, n, i0, j0, wr, wc, lane, acc, false);
 
+  // staging: wavefront w fetches K rows w, w + 4, ... of a chunk, one
+  // 64-wide row per load (row address wave-uniform, lane = column)
+  static_assert(TM == 64, "staging assumes one tile row per wavefront load");
+  const bool iok = full || i0 + lane < n, jok = full || j0 + lane < n;
   T ru[PER], rw[PER];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int t = tid + 256 * u;
-      const int q = t / TM, cidx = t % TM;
-      const int kq = k0 + q;
-      ru[u] = (kq < bw && i0 + cidx < n) ? -A[(int64_t)(s + kq) * lda + i0 + cidx] : T(0);
-      rw[u] = (kq < bw && j0 + cidx < n) ? Wg[(int64_t)kq * ldw + j0 + cidx] : T(0);
+      const int kq = k0 + wave + 4 * u;
+      ru[u] = (kq < bw && iok) ? -A[(int64_t)(s + kq) * lda + i0 + lane] : T(0);
+      rw[u] = (kq < bw && jok) ? Wg[(int64_t)kq * ldw + j0 + lane] : T(0);
     }
   };
   auto stash = [&](int buf) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int t = tid + 256 * u;
-      const int q = t / TM, cidx = t % TM;
-      Us[buf][q * LDS_LD + cidx] = ru[u];
-      Ws[buf][q * LDS_LD + cidx] = rw[u];
+      Us[buf][(wave + 4 * u) * LDS_LD + lane] = ru[u];
+      Ws[buf][(wave + 4 * u) * LDS_LD + lane] = rw[u];
     }
   };
 
@@ -165,16 +202,10 @@ __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
     if (kc + 1 < nk) stash(buf ^ 1);
     __syncthreads();
   }
-#pragma unroll
-  for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < TI; ++tj)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + wr * WT + ti * 16 + Mfma<T>::row(lane, v);
-        const int jj = j0 + wc * WT + tj * 16 + (lane & 15);
-        if (i < n && jj < n && jj >= i) A[(int64_t)i * lda + jj] = acc[ti][tj][v];
-      }
+  if (full)
+    c_tile_io<T, TM, true>(A, lda, n, i0, j0, wr, wc, lane, acc, true);
+  else
+    c_tile_io<T, TM, false>(A, lda, n, i0, j0, wr, wc, lane, acc, true);
 }
 
 }  // namespace bbk
