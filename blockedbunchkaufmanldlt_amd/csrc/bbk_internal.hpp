@@ -2,6 +2,8 @@
 // gfx950 blocked Bunch-Kaufman kernels.
 #pragma once
 
+#include <climits>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -236,6 +238,40 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i) {
   i = __builtin_amdgcn_readlane(i, 63);
 }
 
+// Forms for values that are never NaN (the callers map NaN to 0 with index
+// INT_MAX): the maximum by DPP steps of v_max_f64, then the first index of
+// it as the lowest index among the lanes holding it (one lane in the common
+// case, read directly).  Same result as wave_argmax on such values.
+__device__ __forceinline__ double wave_max_nonan(double v) {
+  v = fmax(v, dpp_f64<kDppQuadXor1, 0xf>(v));
+  v = fmax(v, dpp_f64<kDppQuadXor2, 0xf>(v));
+  v = fmax(v, dpp_f64<kDppRowHalfMirror, 0xf>(v));
+  v = fmax(v, dpp_f64<kDppRowMirror, 0xf>(v));
+  v = fmax(v, dpp_f64<kDppRowBcast15, 0xa>(v));
+  v = fmax(v, dpp_f64<kDppRowBcast31, 0xc>(v));
+  return readlane_f64(v, 63);
+}
+__device__ __forceinline__ void wave_argmax_nonan(double& v, int& i) {
+  const double m = wave_max_nonan(v);
+  const bool hold = v == m;
+  const unsigned long long hit = __builtin_amdgcn_ballot_w64(hold);
+  int im;
+  if (__builtin_popcountll(hit) == 1) {
+    im = __builtin_amdgcn_readlane(i, __builtin_ctzll(hit));
+  } else {
+    int c = hold ? i : INT_MAX;
+    c = min(c, dpp_i32<kDppQuadXor1, 0xf>(c));
+    c = min(c, dpp_i32<kDppQuadXor2, 0xf>(c));
+    c = min(c, dpp_i32<kDppRowHalfMirror, 0xf>(c));
+    c = min(c, dpp_i32<kDppRowMirror, 0xf>(c));
+    c = min(c, dpp_i32<kDppRowBcast15, 0xa>(c));
+    c = min(c, dpp_i32<kDppRowBcast31, 0xc>(c));
+    im = __builtin_amdgcn_readlane(c, 63);
+  }
+  v = m;
+  i = im;
+}
+
 // Workgroup-wide reductions; every thread receives the result.  sv/si
 // are LDS scratch of at least blockDim.x / 64 entries.
 __device__ __forceinline__ void block_argmax(double& v, int& i, double* sv,
@@ -257,9 +293,10 @@ __device__ __forceinline__ void block_argmax(double& v, int& i, double* sv,
 // hold 2 * blockDim.x / 64 entries; `parity` flips on every call), so a
 // later reduction never overwrites values an earlier one may still be
 // reading -- the barrier of the reduction in between orders them.
+// (block_argmax1, block_max1: values never NaN)
 __device__ __forceinline__ void block_argmax1(double& v, int& i, double* sv, int* si,
                                               int& parity) {
-  wave_argmax(v, i);
+  wave_argmax_nonan(v, i);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double* svp = sv + parity * nw;
   int* sip = si + parity * nw;
@@ -275,7 +312,7 @@ __device__ __forceinline__ void block_argmax1(double& v, int& i, double* sv, int
 }
 
 __device__ __forceinline__ double block_max1(double v, double* sv, int& parity) {
-  v = wave_max_nan_ignoring(v);
+  v = wave_max_nonan(v);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double* svp = sv + parity * nw;
   if ((threadIdx.x & 63) == 0) svp[w] = v;

@@ -347,12 +347,16 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   const int gi = blockIdx.x / stride;  // working-group index
   const int G = gridDim.x / stride;
   const int tid = threadIdx.x;
-  // Thread tid starts at position (column) j = s + NT gi + tid and keeps
-  // its W column for the whole panel: a symmetric interchange of positions
-  // a and b swaps the positions of the two threads holding them instead of
-  // moving W entries.
-  int j = s + gi * NT + tid;
+  // Lane l of wavefront w starts at position (column) j = s + 64 (w G + gi)
+  // + l: the 64-column chunks are dealt to the groups round-robin, so every
+  // group holds columns across the whole active range and the candidate
+  // row's strided part (the columns left of r) is spread evenly.  A thread
+  // keeps its W column for the whole panel: a symmetric interchange of
+  // positions a and b swaps the positions of the two threads holding them
+  // instead of moving W entries.
+  int j = s + 64 * ((tid >> 6) * G + gi) + (tid & 63);
   const bool own = j < n;
+  const int qrow = gi + tid * G;  // finished panel row whose interchanges this thread writes
   const int bt = min(p.b, n - s);
   T* Wg = reinterpret_cast<T*>(p.Wg);
   // record areas: [slots G][pubs 16], two parities
@@ -646,13 +650,12 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       size = 1;
     } else if (kind == -1) {
       // symmetric interchange k <-> r (src/block_ldlt_decomposition.cc:414-498)
-      // columns k and r of the finished panel rows, one row per thread of
-      // workgroup 0 (the protocol orders these after every earlier write of
-      // the same elements, made at an earlier pivot)
-      if (gi == 0 && tid < kk) {
-        const int q = tid;
-        sta(&A[(int64_t)(s + q) * lda + k], Lr[q]);
-        sta(&A[(int64_t)(s + q) * lda + r], Lk[q]);
+      // columns k and r of the finished panel rows: row q by thread q / G of
+      // group q % G, the same thread at every pivot (so its writes of an
+      // element follow each other in program order)
+      if (qrow < kk) {
+        sta(&A[(int64_t)(s + qrow) * lda + k], Lr[qrow]);
+        sta(&A[(int64_t)(s + qrow) * lda + r], Lk[qrow]);
       }
       // trailing matrix: new row/column r take the old row k (= arow)
       if (own && j > r)
@@ -701,11 +704,10 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       // (src/block_ldlt_decomposition.cc:500-587, 658-704)
       const int k1 = k + 1;
       if (r != k1) {
-        // columns k+1 and r of the finished panel rows (workgroup 0)
-        if (gi == 0 && tid < kk) {
-          const int q = tid;
-          sta(&A[(int64_t)(s + q) * lda + k1], Lr[q]);
-          sta(&A[(int64_t)(s + q) * lda + r], u1);
+        // columns k+1 and r of the finished panel rows (rows as above)
+        if (qrow < kk) {
+          sta(&A[(int64_t)(s + qrow) * lda + k1], Lr[qrow]);
+          sta(&A[(int64_t)(s + qrow) * lda + r], Lu1[qrow]);
         }
         // trailing matrix: new row/column r take the old row k+1 (= P1)
         if (own && j > r)
