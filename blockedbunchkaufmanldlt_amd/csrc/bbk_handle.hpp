@@ -56,9 +56,11 @@ int default_panel_width(int n, int elem_bytes);
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
               int* piv);
+// Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
+// refinement's solves instead of substitution; NULL substitutes.
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
-          const int* piv, T* B, int64_t ldb);
+          const int* piv, T* B, int64_t ldb, const T* Li = nullptr);
 int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes);
 template <typename T>
 int batched_sytrf(Handle* h, int count, const int* h_n, T* const* A, const int64_t* lda,

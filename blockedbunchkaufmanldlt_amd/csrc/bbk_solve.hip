@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bbk_ldlt.h"
@@ -173,6 +174,89 @@ __device__ __forceinline__ void dsolve_row(const T* __restrict__ F, int64_t lda,
   }
 }
 
+// Inverse of each 128 x 128 unit lower diagonal block of L: Li[J] row-major,
+// Li(r, c) = (L_JJ^-1)(r, c), zero above the diagonal and outside the
+// matrix.  L_JJ is read like stage_diag_block (the D entry F(k, k+1) of a
+// 2x2 pivot as 0); thread c forms column c by forward substitution, its
+// column of the inverse kept in LDS (other threads never read it).
+template <typename T>
+__global__ __launch_bounds__(kSolveBlock) void k_diag_inv(int n, const T* __restrict__ F,
+                                                          int64_t lda, const int* __restrict__ piv,
+                                                          T* __restrict__ Li) {
+  constexpr int NB = kSolveBlock;
+  __shared__ T Ls[NB][NB];      // L(r, k), k < r
+  __shared__ T Xs[NB][NB + 1];  // (L^-1)(i, c) at Xs[i][c]
+  const int J = blockIdx.x;
+  const int b0 = J * NB, bn = min(NB, n - b0);
+  const int c = threadIdx.x;
+  // row k of F holds L(., k): Ls[r][k] = F(b0 + k, b0 + r) for k < r
+#pragma unroll 16
+  for (int k = 0; k < NB; ++k) {
+    T x = T(0);
+    if (k < c && c < bn) {
+      x = F[(int64_t)(b0 + k) * lda + b0 + c];
+      if (c == k + 1 && piv[b0 + k] == 2) x = T(0);
+    }
+    Ls[c][k] = x;
+  }
+  __syncthreads();
+  for (int i = 0; i < NB; ++i) {
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+    int k = 0;
+    for (; k + 4 <= i; k += 4) {
+      a0 += Ls[i][k] * Xs[k][c];
+      a1 += Ls[i][k + 1] * Xs[k + 1][c];
+      a2 += Ls[i][k + 2] * Xs[k + 2][c];
+      a3 += Ls[i][k + 3] * Xs[k + 3][c];
+    }
+    for (; k < i; ++k) a0 += Ls[i][k] * Xs[k][c];
+    const T acc = (a0 + a1) + (a2 + a3);
+    Xs[i][c] = i < c ? T(0) : (i == c ? T(1) : -acc);
+  }
+  T* out = Li + (int64_t)J * NB * NB;
+  for (int i = 0; i < NB; ++i) out[(int64_t)i * NB + c] = (i < bn && c < bn) ? Xs[i][c] : T(0);
+}
+
+// Li (128 x 128 row-major) into LDS with row stride 129.
+template <typename T>
+__device__ __forceinline__ void stage_inverse_block(T* Lp, const T* __restrict__ Li) {
+  constexpr int NB = kSolveBlock;
+  constexpr int PER = NB * NB / kSolveThreads / 2;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) v[u] = Li[threadIdx.x + kSolveThreads * (u + PER * h)];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + kSolveThreads * (u + PER * h);
+      Lp[(e / NB) * (NB + 1) + e % NB] = v[u];
+    }
+  }
+}
+
+// Half of a product with the staged inverse: thread (r = tid % 128, h = tid
+// / 128) sums, over c in [64 h, 64 h + 64), Li(r, c) t[c] (TRANS false) or
+// Li(c, r) t[c] (TRANS true).  Lanes take consecutive r, so both forms read
+// LDS without bank conflicts (row stride 129); t[c] is one broadcast read.
+template <typename T, int NR, bool TRANS>
+__device__ __forceinline__ void inverse_product(const T* Lp, const T (*t)[NR], T (&sv)[NR]) {
+  constexpr int NB = kSolveBlock;
+  const int r = threadIdx.x & (NB - 1), c0 = (threadIdx.x >> 7) * 64;
+  T a[2][NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) a[0][q] = a[1][q] = T(0);
+#pragma unroll 16
+  for (int c = 0; c < 64; ++c) {
+    const int cc = c0 + c;
+    const T l = TRANS ? Lp[cc * (NB + 1) + r] : Lp[r * (NB + 1) + cc];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) a[c & 1][q] += l * t[cc][q];
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) sv[q] = a[0][q] + a[1][q];
+}
+
 // Serial substitution over 64 unknowns, one per lane (v): step k takes
 // unknown s = k (ASC) or 63 - k, broadcasts it with readlane and lanes with
 // s < lane (ASC) or s > lane subtract L * v_s, L = Lp[idx(s)].  The L values
@@ -214,22 +298,32 @@ __device__ __forceinline__ void serial64(T (&v)[NR], const T* Lp, int lane, Idx 
 // wavefront 0 solves the top 64 unknowns serially (readlane broadcast) and
 // publishes them at once, so later blocks start on them; wavefront 1
 // applies them to the bottom 64 and solves those.
-template <typename T, int NR>
+//
+// INV: the diagonal block's inverse Li (k_diag_inv) replaces the serial
+// substitution: t = b_J - sum_I L_JI z_I, then z_J = Li t as a product
+// (thread (r, h) sums row r over columns 64h .. 64h+63), so no serial chain
+// remains on the path from one block to the next.
+template <typename T, int NR, bool INV>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ B, int64_t ldb,
-    T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk) {
-  constexpr int NB = kSolveBlock, H = NB / 2, NPK = NB * (NB - 1) / 2;
-  __shared__ T Lp[NPK];
+    T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk,
+    const T* __restrict__ Li) {
+  constexpr int NB = kSolveBlock, H = NB / 2;
+  constexpr int NPK = INV ? NB * (NB + 1) : NB * (NB - 1) / 2;
+  __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
   __shared__ T part[4][NB][NR];
-  __shared__ T ztop[H][NR];
+  __shared__ T ztop[INV ? NB : H][NR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
     const int j0 = J * NB;
     const int jn = min(NB, n - j0);
     const int cA = j0 + lane, cB = j0 + H + lane;
     const bool okA = lane < jn, okB = H + lane < jn;
-    stage_diag_block<T>(Lp, F, lda, piv, j0, jn, n);
+    if constexpr (INV)
+      stage_inverse_block<T>(Lp, Li + (int64_t)J * NB * NB);
+    else
+      stage_diag_block<T>(Lp, F, lda, piv, j0, jn, n);
     T bj[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) bj[q] = T(0);
@@ -298,6 +392,37 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
       part[w][H + lane][q] = acc[1][q];
     }
     __syncthreads();
+    if constexpr (INV) {
+      // t into ztop (rows 0..127), then z = Li t
+      if (w < 2) {
+        const int r = w * H + lane;
+        const bool ok = w == 0 ? okA : okB;
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+          ztop[r][q] = ok ? bj[q] - ((part[0][r][q] + part[1][r][q]) +
+                                     (part[2][r][q] + part[3][r][q]))
+                          : T(0);
+      }
+      __syncthreads();
+      T sv[NR];
+      inverse_product<T, NR, false>(Lp, ztop, sv);
+      const int r = threadIdx.x & (NB - 1);
+      if (threadIdx.x >= NB) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) part[0][r][q] = sv[q];
+      }
+      __syncthreads();
+      if (threadIdx.x < NB && r < jn) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const T v = sv[q] + part[0][r][q];
+          Z[(int64_t)(j0 + r) * ldz + q] = v;
+          xput(rec + (int64_t)(j0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(v));
+        }
+      }
+      __syncthreads();  // Lp, part, ztop are reused by the next block
+      continue;
+    }
     if (w == 0) {
       T v[NR];
 #pragma unroll
@@ -360,22 +485,31 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
 // 32w .. 32w+31 of each block J (16 at a time, contiguous per row).  The
 // bottom 64 unknowns are solved first (wavefront 1) and published, then
 // applied to the top 64, which wavefront 0 solves.
-template <typename T, int NR>
+//
+// INV: v_I = Li^T s with Li the diagonal block's inverse (k_diag_inv), s =
+// D^-1 z_I - sum_J L_JI^T v_J; thread (r, h) sums column r of Li over rows
+// 64h .. 64h+63.
+template <typename T, int NR, bool INV>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ Z, int64_t ldz,
-    T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk) {
-  constexpr int NB = kSolveBlock, H = NB / 2, NPK = NB * (NB - 1) / 2;
-  __shared__ T Lp[NPK];
+    T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk,
+    const T* __restrict__ Li) {
+  constexpr int NB = kSolveBlock, H = NB / 2;
+  constexpr int NPK = INV ? NB * (NB + 1) : NB * (NB - 1) / 2;
+  __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
   __shared__ T part[4][NB][NR];
-  __shared__ T vbot[H][NR];
+  __shared__ T vbot[INV ? NB : H][NR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
     const int i0 = I * NB;
     const int in = min(NB, n - i0);
     const int rA = i0 + lane, rB = i0 + H + lane;
     const bool okA = lane < in, okB = H + lane < in;
-    stage_diag_block<T>(Lp, F, lda, piv, i0, in, n);
+    if constexpr (INV)
+      stage_inverse_block<T>(Lp, Li + (int64_t)I * NB * NB);
+    else
+      stage_diag_block<T>(Lp, F, lda, piv, i0, in, n);
     T wv[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) wv[q] = T(0);
@@ -459,6 +593,37 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
       part[w][H + lane][q] = acc[1][q];
     }
     __syncthreads();
+    if constexpr (INV) {
+      if (w < 2) {
+        const int r = w * H + lane;
+        const bool ok = w == 0 ? okA : okB;
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+          vbot[r][q] = ok ? wv[q] - ((part[0][r][q] + part[1][r][q]) +
+                                     (part[2][r][q] + part[3][r][q]))
+                          : T(0);
+      }
+      __syncthreads();
+      T sv[NR];
+      inverse_product<T, NR, true>(Lp, vbot, sv);
+      const int r = threadIdx.x & (NB - 1);
+      if (threadIdx.x >= NB) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) part[0][r][q] = sv[q];
+      }
+      __syncthreads();
+      if (threadIdx.x < NB && r < in) {
+        const int pi = perm[i0 + r];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const T v = sv[q] + part[0][r][q];
+          X[(int64_t)pi * ldx + q] = v;
+          xput(rec + (int64_t)(i0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(v));
+        }
+      }
+      __syncthreads();  // Lp, part, vbot are reused by the next block
+      continue;
+    }
     if (w == 1) {
       T v[NR];
 #pragma unroll
@@ -757,12 +922,12 @@ static int coop_groups(K kernel, int want) {
 // and Z (n x kSolveMaxRhs of T).  The record area is zeroed when it is
 // (re)allocated; afterwards every launch uses a fresh tag, so records of
 // earlier launches are never accepted and nothing is cleared per call.
-template <typename T, int NR>
+template <typename T, int NR, bool INV>
 static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, const int* perm,
                         const int* piv, const T* B, int64_t ldb, T* X, int64_t ldx, T* Z,
-                        XRec* rf, XRec* rb) {
-  const int gf = coop_groups(k_trsv_fwd<T, NR>, nblk);
-  const int gb = coop_groups(k_trsv_bwd<T, NR>, nblk);
+                        XRec* rf, XRec* rb, const T* Li) {
+  const int gf = coop_groups(k_trsv_fwd<T, NR, INV>, nblk);
+  const int gb = coop_groups(k_trsv_bwd<T, NR, INV>, nblk);
   int64_t ldz = kSolveMaxRhs;
   {
     int tag = h->next_solve_tag();
@@ -773,8 +938,9 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     const T* B_ = B;
     T* Z_ = Z;
     XRec* r_ = rf;
-    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &B_, &ldb_, &Z_, &ldz, &r_, &tag, &nblk_};
-    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR>),
+    const T* Li_ = Li;
+    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &B_, &ldb_, &Z_, &ldz, &r_, &tag, &nblk_, &Li_};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR, INV>),
                                               dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
   }
@@ -787,17 +953,20 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     const T* Z_ = Z;
     T* X_ = X;
     XRec* r_ = rb;
-    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &Z_, &ldz, &X_, &ldx_, &r_, &tag, &nblk_};
-    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR>),
+    const T* Li_ = Li;
+    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &Z_, &ldz, &X_, &ldx_, &r_, &tag, &nblk_, &Li_};
+    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR, INV>),
                                               dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
   }
   return BBK_OK;
 }
 
+// Li: the diagonal blocks' inverses from k_diag_inv (binary32 only), or
+// NULL for substitution.
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
-          const int* piv, T* B, int64_t ldb) {
+          const int* piv, T* B, int64_t ldb, const T* Li) {
   if (n <= 0) return -2;
   if (nrhs < 0) return -3;
   if (!F) return -4;
@@ -827,11 +996,23 @@ int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
     const int nr = std::min(kSolveMaxRhs, nrhs - q0);
     const T* Bq = B + q0;
     T* Xq = B + q0;
+    if constexpr (std::is_same<T, float>::value) {
+      if (Li) {
+        switch (nr) {
+          case 1: rc = solve_launch<T, 1, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
+          case 2: rc = solve_launch<T, 2, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
+          case 3: rc = solve_launch<T, 3, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
+          default: rc = solve_launch<T, 4, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
+        }
+        if (rc != BBK_OK) return rc;
+        continue;
+      }
+    }
     switch (nr) {
-      case 1: rc = solve_launch<T, 1>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
-      case 2: rc = solve_launch<T, 2>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
-      case 3: rc = solve_launch<T, 3>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
-      default: rc = solve_launch<T, 4>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb); break;
+      case 1: rc = solve_launch<T, 1, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
+      case 2: rc = solve_launch<T, 2, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
+      case 3: rc = solve_launch<T, 3, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
+      default: rc = solve_launch<T, 4, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
     }
     if (rc != BBK_OK) return rc;
   }
@@ -878,9 +1059,9 @@ int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
 }
 
 template int solve<double>(Handle*, int, int, const double*, int64_t, const int*, const int*,
-                           double*, int64_t);
+                           double*, int64_t, const double*);
 template int solve<float>(Handle*, int, int, const float*, int64_t, const int*, const int*,
-                          float*, int64_t);
+                          float*, int64_t, const float*);
 
 }  // namespace bbk
 
@@ -983,7 +1164,9 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   const size_t ibytes = (2 * (size_t)n * sizeof(int) + 255) / 256 * 256;
   const size_t rbytes = ((size_t)n * nrhs * sizeof(double) + 255) / 256 * 256;
   const size_t dbytes = ((size_t)n * nrhs * sizeof(float) + 255) / 256 * 256;
-  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + 1024);
+  const int nblk = (n + bbk::kSolveBlock - 1) / bbk::kSolveBlock;
+  const size_t lbytes = (size_t)nblk * bbk::kSolveBlock * bbk::kSolveBlock * sizeof(float);
+  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + 1024 + lbytes);
   if (rc != BBK_OK) return rc;
   char* base = reinterpret_cast<char*>(h->mixed_ws);
   float* S = reinterpret_cast<float*>(base);
@@ -992,16 +1175,21 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   double* R = reinterpret_cast<double*>(base + fbytes + ibytes);
   float* Dv = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes);
   double* eta = reinterpret_cast<double*>(base + fbytes + ibytes + rbytes + dbytes);
+  float* Li = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes + dbytes + 1024);
   // FP32 copy of the upper triangle, FP32 factorization
   dim3 cg(bbk::blocks_for(n, 256), n);
   hipLaunchKernelGGL(bbk::k_cast_upper, cg, dim3(256), 0, h->stream, n, d_A, lda, S, ldf);
   rc = bbk::factorize<float>(h, n, 0, S, ldf, perm, piv);
   if (rc != BBK_OK) return rc;
+  // inverses of the diagonal blocks of L: the refinement's solves multiply
+  // by them instead of substituting
+  hipLaunchKernelGGL(bbk::k_diag_inv<float>, dim3(nblk), dim3(bbk::kSolveBlock), 0, h->stream, n,
+                     S, ldf, piv, Li);
   // x0 = solve32(b)
   const unsigned eb = bbk::blocks_for((int64_t)n * nrhs, 256);
   hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream, n,
                      nrhs, d_B, ldb, Dv, (int64_t)nrhs);
-  rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs);
+  rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs, Li);
   if (rc != BBK_OK) return rc;
   hipLaunchKernelGGL((bbk::k_copy_cols<float, double>), dim3(eb), dim3(256), 0, h->stream, n,
                      nrhs, Dv, (int64_t)nrhs, d_X, ldx);
@@ -1027,7 +1215,7 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
     // d = solve32(r); x += d
     hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream, n,
                        nrhs, R, (int64_t)nrhs, Dv, (int64_t)nrhs);
-    rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs);
+    rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs, Li);
     if (rc != BBK_OK) return rc;
     hipLaunchKernelGGL(bbk::k_axpy_f32_to_f64, dim3(eb), dim3(256), 0, h->stream, n, nrhs, Dv,
                        (int64_t)nrhs, d_X, ldx);

@@ -119,3 +119,23 @@ def test_multi_rhs_mixed(oracle, handle):
     info = handle.sysv_mixed(Ad, torch.from_numpy(B).cuda(), X, tol=1e-12)
     assert info["converged"]
     assert berr_np(A, B, X.cpu().numpy()).max() <= 1.5e-12
+
+
+@pytest.mark.parametrize("n", [1000, 1153])
+def test_mixed_many_2x2_ragged(oracle, handle, n):
+    """Refinement whose FP32 solves multiply by inverted 128 x 128 diagonal
+    blocks: orders that are not multiples of 128 and a zero diagonal (2x2
+    pivots everywhere, also across block edges).  A wrong block inverse or
+    edge entry stalls the refinement, so the iteration count is bounded."""
+    import torch
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1.0, 1.0, (n, n))
+    A = np.triu(A) + np.triu(A, 1).T
+    np.fill_diagonal(A, 0.0)
+    b = rng.standard_normal((n, 2))
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()
+    X = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    info = handle.sysv_mixed(Ad, torch.from_numpy(b).cuda(), X, tol=1e-12, max_iter=30)
+    assert info["converged"] and info["iters"] <= 8, info
+    assert berr_np(A, b, X.cpu().numpy()).max() <= 1.5e-12
