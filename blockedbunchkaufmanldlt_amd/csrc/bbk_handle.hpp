@@ -57,7 +57,9 @@ template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
               int* piv);
 // Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
-// refinement's solves instead of substitution; NULL substitutes.
+// refinement's solves instead of substitution, with F's strictly lower
+// triangle holding the mirror of the upper one; NULL substitutes (F upper
+// only).
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
           const int* piv, T* B, int64_t ldb, const T* Li = nullptr);

@@ -217,6 +217,32 @@ __global__ __launch_bounds__(kSolveBlock) void k_diag_inv(int n, const T* __rest
   for (int i = 0; i < NB; ++i) out[(int64_t)i * NB + c] = (i < bn && c < bn) ? Xs[i][c] : T(0);
 }
 
+// Strictly lower triangle of F := mirror of the strictly upper one (F(j, i)
+// = F(i, j), i < j), through 64 x 64 LDS tiles so that reads and writes
+// both run along rows.  Tile t of the upper triangle: (I, J), I <= J.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mirror_upper(int n, T* __restrict__ F, int64_t lda) {
+  constexpr int TB = 64;
+  __shared__ T tile[TB][TB + 1];
+  const int t = blockIdx.x;
+  int J = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (J * (J + 1) / 2 > t) --J;
+  while ((J + 1) * (J + 2) / 2 <= t) ++J;
+  const int I = t - J * (J + 1) / 2;
+  const int i0 = I * TB, j0 = J * TB;
+  const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+  for (int r = r0; r < TB; r += 4) {
+    const int i = i0 + r, j = j0 + c;
+    tile[r][c] = (i < n && j < n) ? F[(int64_t)i * lda + j] : T(0);
+  }
+  __syncthreads();
+  // row j0 + r of the lower triangle, columns i0 + c: F(i0 + c, j0 + r)
+  for (int r = r0; r < TB; r += 4) {
+    const int j = j0 + r, i = i0 + c;
+    if (j < n && i < j) F[(int64_t)j * lda + i] = tile[c][r];
+  }
+}
+
 // Li (128 x 128 row-major) into LDS with row stride 129.
 template <typename T>
 __device__ __forceinline__ void stage_inverse_block(T* Lp, const T* __restrict__ Li) {
@@ -486,9 +512,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
 // bottom 64 unknowns are solved first (wavefront 1) and published, then
 // applied to the top 64, which wavefront 0 solves.
 //
-// INV: v_I = Li^T s with Li the diagonal block's inverse (k_diag_inv), s =
-// D^-1 z_I - sum_J L_JI^T v_J; thread (r, h) sums column r of Li over rows
-// 64h .. 64h+63.
+// INV (refinement solves): v_I = Li^T s with Li the diagonal block's inverse
+// (k_diag_inv), s = D^-1 z_I - sum_J L_JI^T v_J; thread (r, h) sums column
+// r of Li over rows 64h .. 64h+63.  The strictly lower triangle of F then
+// holds the mirror of the upper one (k_mirror_upper), and L_JI^T is read
+// from it: element F(i, j), i in I, as F[j lda + i], consecutive lanes on
+// consecutive i (coalesced, where the upper rows give every lane its own
+// cache line).
 template <typename T, int NR, bool INV>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
@@ -538,7 +568,15 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
       for (int k = 0; k < 2 * UB; ++k) {
         const int Ju = J - k / 2;
         const int cb = max(Ju, I + 1) * NB + 32 * w + 16 * (k & 1);
-        if (vec_ok && cb + 16 <= n) {
+        if constexpr (INV) {
+          const int rAc = min(rA, n - 1), rBc = min(rB, n - 1);
+#pragma unroll
+          for (int cc = 0; cc < 16; ++cc) {
+            const int64_t co = (int64_t)min(cb + cc, n - 1) * lda;
+            fa[k][cc] = F[co + rAc];
+            fb[k][cc] = F[co + rBc];
+          }
+        } else if (vec_ok && cb + 16 <= n) {
           const vec_t* pa = reinterpret_cast<const vec_t*>(F + roA + cb);
           const vec_t* pb = reinterpret_cast<const vec_t*>(F + roB + cb);
 #pragma unroll
@@ -1182,9 +1220,15 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   rc = bbk::factorize<float>(h, n, 0, S, ldf, perm, piv);
   if (rc != BBK_OK) return rc;
   // inverses of the diagonal blocks of L: the refinement's solves multiply
-  // by them instead of substituting
+  // by them instead of substituting; the backward sweeps read L^T from the
+  // mirrored (otherwise unused) lower triangle of the binary32 factor
   hipLaunchKernelGGL(bbk::k_diag_inv<float>, dim3(nblk), dim3(bbk::kSolveBlock), 0, h->stream, n,
                      S, ldf, piv, Li);
+  {
+    const unsigned tb = static_cast<unsigned>((n + 63) / 64);
+    hipLaunchKernelGGL(bbk::k_mirror_upper<float>, dim3(tb * (tb + 1) / 2), dim3(256), 0,
+                       h->stream, n, S, ldf);
+  }
   // x0 = solve32(b)
   const unsigned eb = bbk::blocks_for((int64_t)n * nrhs, 256);
   hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream, n,
