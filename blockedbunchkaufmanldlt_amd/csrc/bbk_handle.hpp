@@ -59,16 +59,20 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
 // Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
 // refinement's solves instead of substitution, with F's strictly lower
 // triangle holding the mirror of the upper one; NULL substitutes (F upper
-// only).
+// only).  stop: device flag; when set, the sweeps return at once (NULL:
+// always run).
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
-          const int* piv, T* B, int64_t ldb, const T* Li = nullptr);
+          const int* piv, T* B, int64_t ldb, const T* Li = nullptr,
+          const int* stop = nullptr);
 int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes);
 template <typename T>
 int batched_sytrf(Handle* h, int count, const int* h_n, T* const* A, const int64_t* lda,
                   int* const* perm, int* const* piv);
+// stop: as for solve (the residual kernels return at once when it is set).
 int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
          const double* B, int64_t ldb, const double* X, int64_t ldx, double* R,
-         int64_t ldr, double* d_eta, double anorm, bool want_norm);
+         int64_t ldr, double* d_eta, double anorm, bool want_norm,
+         const int* stop = nullptr);
 
 }  // namespace bbk

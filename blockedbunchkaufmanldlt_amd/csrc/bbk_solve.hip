@@ -334,12 +334,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ B, int64_t ldb,
     T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk,
-    const T* __restrict__ Li) {
+    const T* __restrict__ Li, const int* __restrict__ stop) {
   constexpr int NB = kSolveBlock, H = NB / 2;
   constexpr int NPK = INV ? NB * (NB + 1) : NB * (NB - 1) / 2;
   __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
   __shared__ T part[4][NB][NR];
   __shared__ T ztop[INV ? NB : H][NR];
+  if (stop && *stop) return;  // every workgroup reads the same flag
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
     const int j0 = J * NB;
@@ -524,12 +525,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ Z, int64_t ldz,
     T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk,
-    const T* __restrict__ Li) {
+    const T* __restrict__ Li, const int* __restrict__ stop) {
   constexpr int NB = kSolveBlock, H = NB / 2;
   constexpr int NPK = INV ? NB * (NB + 1) : NB * (NB - 1) / 2;
   __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
   __shared__ T part[4][NB][NR];
   __shared__ T vbot[INV ? NB : H][NR];
+  if (stop && *stop) return;  // every workgroup reads the same flag
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
     const int i0 = I * NB;
@@ -737,8 +739,9 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
     int n, int nr, const double* __restrict__ A, int64_t lda,
     const double* __restrict__ X, int64_t ldx, double* __restrict__ prow,
     double* __restrict__ pcol, double* __restrict__ pabs_row,
-    double* __restrict__ pabs_col, int ntile, int with_norm) {
+    double* __restrict__ pabs_col, int ntile, int with_norm, const int* __restrict__ stop) {
   constexpr int TB = kSymvTile;
+  if (stop && *stop) return;
   // the tile is read from HBM once, coalesced, and both passes run from LDS
   __shared__ double tile[TB][TB + 1];
   __shared__ double xs_i[TB * kSolveMaxRhs];
@@ -854,8 +857,9 @@ __global__ __launch_bounds__(256) void k_symv_reduce(
     const double* __restrict__ pabs_col, const double* __restrict__ Bv,
     int64_t ldb, const double* __restrict__ X, int64_t ldx,
     double* __restrict__ R, int64_t ldr, double* __restrict__ norms,
-    int with_norm) {
+    int with_norm, const int* __restrict__ stop) {
   __shared__ double part[4][64][kSolveMaxRhs + 1];
+  if (stop && *stop) return;
   const int li = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = blockIdx.x * 64 + li;
   const bool ok = i < n;
@@ -904,6 +908,28 @@ __global__ void k_berr(int nr, const double* __restrict__ norms,
   eta[q] = den > 0.0 ? norms[q] / den : (norms[q] > 0.0 ? __builtin_huge_val() : 0.0);
 }
 
+// Refinement stopping test on the device, after correction number it:
+// stop (ctl[0] = 1, ctl[1] = it, *best = max_q eta_q) when the largest
+// backward error is <= tol, is NaN, or it reached max_iter.  Once set, the
+// flag stays: later checks and the kernels that take it do nothing.
+__global__ void k_refine_check(const double* __restrict__ eta, int nr, double tol, int it,
+                               int max_iter, int* __restrict__ ctl, double* __restrict__ best) {
+  if (threadIdx.x != 0 || ctl[0]) return;
+  double m = 0.0;
+  bool nan = false;
+  for (int q = 0; q < nr; ++q) {
+    const double e = eta[q];
+    if (!(e == e)) nan = true;
+    m = e > m ? e : m;
+  }
+  if (nan) m = __builtin_nan("");
+  if (nan || m <= tol || it >= max_iter) {
+    ctl[0] = 1;
+    ctl[1] = it;
+    *best = m;
+  }
+}
+
 __global__ void k_fill_f64(double* p, int count, double v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;
@@ -930,10 +956,11 @@ __global__ void k_copy_cols(int n, int nr, const S* __restrict__ src,
   dst[(int64_t)i * ldd + q] = (D)src[(int64_t)i * lds + q];
 }
 
-// X += (double) Dv
+// X += (double) Dv, unless *stop
 __global__ void k_axpy_f32_to_f64(int n, int nr, const float* __restrict__ dv,
                                   int64_t ldd, double* __restrict__ X,
-                                  int64_t ldx) {
+                                  int64_t ldx, const int* __restrict__ stop) {
+  if (*stop) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n * nr) return;
   const int i = e / nr, q = e % nr;
@@ -963,7 +990,7 @@ static int coop_groups(K kernel, int want) {
 template <typename T, int NR, bool INV>
 static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, const int* perm,
                         const int* piv, const T* B, int64_t ldb, T* X, int64_t ldx, T* Z,
-                        XRec* rf, XRec* rb, const T* Li) {
+                        XRec* rf, XRec* rb, const T* Li, const int* stop) {
   const int gf = coop_groups(k_trsv_fwd<T, NR, INV>, nblk);
   const int gb = coop_groups(k_trsv_bwd<T, NR, INV>, nblk);
   int64_t ldz = kSolveMaxRhs;
@@ -977,7 +1004,9 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     T* Z_ = Z;
     XRec* r_ = rf;
     const T* Li_ = Li;
-    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &B_, &ldb_, &Z_, &ldz, &r_, &tag, &nblk_, &Li_};
+    const int* stop_ = stop;
+    void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &B_,   &ldb_,
+                    &Z_,  &ldz,  &r_,   &tag,   &nblk_, &Li_,  &stop_};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR, INV>),
                                               dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
@@ -992,7 +1021,9 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     T* X_ = X;
     XRec* r_ = rb;
     const T* Li_ = Li;
-    void* args[] = {&n_, &F_, &lda_, &piv_, &perm_, &Z_, &ldz, &X_, &ldx_, &r_, &tag, &nblk_, &Li_};
+    const int* stop_ = stop;
+    void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &Z_,   &ldz,
+                    &X_,  &ldx_, &r_,   &tag,   &nblk_, &Li_,  &stop_};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR, INV>),
                                               dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
@@ -1004,7 +1035,7 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
 // NULL for substitution.
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
-          const int* piv, T* B, int64_t ldb, const T* Li) {
+          const int* piv, T* B, int64_t ldb, const T* Li, const int* stop) {
   if (n <= 0) return -2;
   if (nrhs < 0) return -3;
   if (!F) return -4;
@@ -1037,20 +1068,20 @@ int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
     if constexpr (std::is_same<T, float>::value) {
       if (Li) {
         switch (nr) {
-          case 1: rc = solve_launch<T, 1, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
-          case 2: rc = solve_launch<T, 2, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
-          case 3: rc = solve_launch<T, 3, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
-          default: rc = solve_launch<T, 4, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li); break;
+          case 1: rc = solve_launch<T, 1, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li, stop); break;
+          case 2: rc = solve_launch<T, 2, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li, stop); break;
+          case 3: rc = solve_launch<T, 3, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li, stop); break;
+          default: rc = solve_launch<T, 4, true>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, Li, stop); break;
         }
         if (rc != BBK_OK) return rc;
         continue;
       }
     }
     switch (nr) {
-      case 1: rc = solve_launch<T, 1, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
-      case 2: rc = solve_launch<T, 2, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
-      case 3: rc = solve_launch<T, 3, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
-      default: rc = solve_launch<T, 4, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr); break;
+      case 1: rc = solve_launch<T, 1, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr, stop); break;
+      case 2: rc = solve_launch<T, 2, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr, stop); break;
+      case 3: rc = solve_launch<T, 3, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr, stop); break;
+      default: rc = solve_launch<T, 4, false>(h, n, nblk, F, lda, perm, piv, Bq, ldb, Xq, ldb, Z, rf, rb, nullptr, stop); break;
     }
     if (rc != BBK_OK) return rc;
   }
@@ -1061,7 +1092,7 @@ int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
 // B - A X.  anorm: pass < 0 to compute ||A||_inf (written to *anorm_out).
 int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
          const double* B, int64_t ldb, const double* X, int64_t ldx, double* R,
-         int64_t ldr, double* d_eta, double anorm, bool want_norm) {
+         int64_t ldr, double* d_eta, double anorm, bool want_norm, const int* stop) {
   const int T = (n + kSymvTile - 1) / kSymvTile;
   const size_t part = (size_t)T * n * kSolveMaxRhs * sizeof(double);
   const size_t partabs = (size_t)T * n * sizeof(double);
@@ -1080,10 +1111,10 @@ int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(32), 0, h->stream, norms, 16, 0.0);
     const int tiles = T * (T + 1) / 2;
     hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, h->stream, n, nr, A, lda,
-                       X + q0, ldx, prow, pcol, parow, pacol, T, with_norm);
+                       X + q0, ldx, prow, pcol, parow, pacol, T, with_norm, stop);
     hipLaunchKernelGGL(k_symv_reduce, dim3(blocks_for(n, 64)), dim3(256), 0, h->stream, n, nr,
                        prow, pcol, parow, pacol, B + q0, ldb, X + q0, ldx,
-                       R ? R + q0 : nullptr, ldr, norms, with_norm);
+                       R ? R + q0 : nullptr, ldr, norms, with_norm, stop);
     hipLaunchKernelGGL(k_berr, dim3(1), dim3(32), 0, h->stream, nr, norms,
                        want_norm ? -1.0 : anorm, d_eta + q0);
     if (want_norm && q0 == 0) {
@@ -1097,9 +1128,9 @@ int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
 }
 
 template int solve<double>(Handle*, int, int, const double*, int64_t, const int*, const int*,
-                           double*, int64_t, const double*);
+                           double*, int64_t, const double*, const int*);
 template int solve<float>(Handle*, int, int, const float*, int64_t, const int*, const int*,
-                          float*, int64_t, const float*);
+                          float*, int64_t, const float*, const int*);
 
 }  // namespace bbk
 
@@ -1204,7 +1235,9 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   const size_t dbytes = ((size_t)n * nrhs * sizeof(float) + 255) / 256 * 256;
   const int nblk = (n + bbk::kSolveBlock - 1) / bbk::kSolveBlock;
   const size_t lbytes = (size_t)nblk * bbk::kSolveBlock * bbk::kSolveBlock * sizeof(float);
-  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + 1024 + lbytes);
+  // eta (nrhs), the best backward error and the stop flags
+  const size_t ebytes = ((size_t)nrhs * sizeof(double) + 64 + 255) / 256 * 256;
+  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + ebytes + lbytes);
   if (rc != BBK_OK) return rc;
   char* base = reinterpret_cast<char*>(h->mixed_ws);
   float* S = reinterpret_cast<float*>(base);
@@ -1213,7 +1246,9 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   double* R = reinterpret_cast<double*>(base + fbytes + ibytes);
   float* Dv = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes);
   double* eta = reinterpret_cast<double*>(base + fbytes + ibytes + rbytes + dbytes);
-  float* Li = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes + dbytes + 1024);
+  double* best_dev = eta + nrhs;
+  int* ctl = reinterpret_cast<int*>(best_dev + 1);
+  float* Li = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes + dbytes + ebytes);
   // FP32 copy of the upper triangle, FP32 factorization
   dim3 cg(bbk::blocks_for(n, 256), n);
   hipLaunchKernelGGL(bbk::k_cast_upper, cg, dim3(256), 0, h->stream, n, d_A, lda, S, ldf);
@@ -1237,33 +1272,55 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   if (rc != BBK_OK) return rc;
   hipLaunchKernelGGL((bbk::k_copy_cols<float, double>), dim3(eb), dim3(256), 0, h->stream, n,
                      nrhs, Dv, (int64_t)nrhs, d_X, ldx);
+  // r = b - A x and eta in binary64 for x0, checked on the host (this call
+  // also forms ||A||_inf)
   std::vector<double> he(nrhs);
   double anorm = -1.0;
   int it = 0;
-  double best = INFINITY;
-  for (;;) {
-    // r = b - A x and eta in binary64
-    rc = bbk::berr(h, n, nrhs, d_A, lda, d_B, ldb, d_X, ldx, R, nrhs, eta, anorm, anorm < 0.0);
-    if (rc != BBK_OK) return rc;
-    if (anorm < 0.0) {
-      hipError_t e = hipMemcpyAsync(&anorm, h->last_anorm_dev, sizeof(double),
-                                    hipMemcpyDeviceToHost, h->stream);
-      if (e != hipSuccess) return h->hip_fail(e, "read anorm");
+  rc = bbk::berr(h, n, nrhs, d_A, lda, d_B, ldb, d_X, ldx, R, nrhs, eta, -1.0, true);
+  if (rc != BBK_OK) return rc;
+  hipError_t e = hipMemcpyAsync(&anorm, h->last_anorm_dev, sizeof(double),
+                                hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(he.data(), eta, nrhs * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "read backward error");
+  double best = *std::max_element(he.begin(), he.end());
+  for (double v : he)
+    if (!(v == v)) best = v;
+  if (!(best <= tol || it >= max_iter || !(best == best))) {
+    // Corrections d = solve32(r), x += d, then r and eta again, queued a few
+    // at a time with the stopping test on the device (k_refine_check): after
+    // the test has stopped, the queued kernels return at once, so x is the
+    // iterate the test accepted and the host waits once per chunk.
+    constexpr int kChunk = 4;
+    hipLaunchKernelGGL(bbk::k_fill_i32, dim3(1), dim3(32), 0, h->stream, ctl, 2, 0);
+    int queued = 0;
+    for (;;) {
+      for (int c = 0; c < kChunk && queued < max_iter; ++c) {
+        hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream,
+                           n, nrhs, R, (int64_t)nrhs, Dv, (int64_t)nrhs);
+        rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs, Li, ctl);
+        if (rc != BBK_OK) return rc;
+        hipLaunchKernelGGL(bbk::k_axpy_f32_to_f64, dim3(eb), dim3(256), 0, h->stream, n, nrhs,
+                           Dv, (int64_t)nrhs, d_X, ldx, ctl);
+        ++queued;
+        rc = bbk::berr(h, n, nrhs, d_A, lda, d_B, ldb, d_X, ldx, R, nrhs, eta, anorm, false, ctl);
+        if (rc != BBK_OK) return rc;
+        hipLaunchKernelGGL(bbk::k_refine_check, dim3(1), dim3(32), 0, h->stream, eta, nrhs, tol,
+                           queued, max_iter, ctl, best_dev);
+      }
+      int hc[2] = {0, 0};
+      e = hipMemcpyAsync(hc, ctl, sizeof(hc), hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(&best, best_dev, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) return h->hip_fail(e, "read refinement state");
+      if (hc[0]) {
+        it = hc[1];
+        break;
+      }
     }
-    hipError_t e = hipMemcpyAsync(he.data(), eta, nrhs * sizeof(double), hipMemcpyDeviceToHost,
-                                  h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return h->hip_fail(e, "read backward error");
-    best = *std::max_element(he.begin(), he.end());
-    if (best <= tol || it >= max_iter || !(best == best)) break;
-    // d = solve32(r); x += d
-    hipLaunchKernelGGL((bbk::k_copy_cols<double, float>), dim3(eb), dim3(256), 0, h->stream, n,
-                       nrhs, R, (int64_t)nrhs, Dv, (int64_t)nrhs);
-    rc = bbk::solve<float>(h, n, nrhs, S, ldf, perm, piv, Dv, nrhs, Li);
-    if (rc != BBK_OK) return rc;
-    hipLaunchKernelGGL(bbk::k_axpy_f32_to_f64, dim3(eb), dim3(256), 0, h->stream, n, nrhs, Dv,
-                       (int64_t)nrhs, d_X, ldx);
-    ++it;
   }
   if (iters) *iters = it;
   if (berr_out) *berr_out = best;
