@@ -69,6 +69,13 @@ __device__ __forceinline__ void stc(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Device-coherent load the compiler sees and waits for (phase B, where no
+// store of the pivot is in flight behind it).
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Device-coherent loads issued together, with the wait for them in the same
 // asm block.  Inside the pivot loop every load goes through these: a load
 // the compiler sees in flight makes it wait, where control flow merges, for
@@ -551,11 +558,23 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       const T* pl = wantu ? &A[(int64_t)(s + tid) * lda + r] : dummy;
       const T* p2 = want2 ? &A[(int64_t)(k + 2) * lda + j] : dummy;
       const T* pu2 = (wantu && use2) ? &A[(int64_t)(s + tid) * lda + k + 2] : dummy;
-      T a, lr;
-      CoherentLoad<T>::six(pa, pl, p1, p2, pu1, pu2, a, lr, P1, P2, u1, u2);
-      if (!active) a = T(0);
-      if (!want1) P1 = T(0);
-      if (!want2) P2 = T(0);
+      // binary32: the finished rows' columns (recently written,
+      // cache-resident) are issued first and the candidate element and rows
+      // k+1, k+2 (HBM) last: loads return in issue order, so the product over
+      // the finished rows runs while the HBM loads are still in flight.
+      // binary64 (all 256 VGPRs in use) waits for all six at once, which
+      // measured faster there.
+      T lr, a;
+      if constexpr (sizeof(T) == 4) {
+        lr = ldc(pl);
+        u1 = ldc(pu1);
+        u2 = ldc(pu2);
+        a = ldc(pa);
+        P1 = ldc(p1);
+        P2 = ldc(p2);
+      } else {
+        CoherentLoad<T>::six(pa, pl, p1, p2, pu1, pu2, a, lr, P1, P2, u1, u2);
+      }
       if (tid < kk) {
         Lr[tid] = lr;
         Lu1[tid] = u1;
@@ -563,9 +582,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
       __syncthreads();
       BBK_PT(6);
-      if (active) {
-        c = a - panel_dot<T, WS, RR>(Lr, Wl, wr, tid, kk, base);
-      }
+      const T dotB = active ? panel_dot<T, WS, RR>(Lr, Wl, wr, tid, kk, base) : T(0);
+      if (!active) a = T(0);
+      if (!want1) P1 = T(0);
+      if (!want2) P2 = T(0);
+      if (active) c = a - dotB;
       double sv = 0.0;
       if (active && j != r) {
         const double aa = fabs(static_cast<double>(c));
