@@ -61,6 +61,9 @@ def lib() -> ctypes.CDLL:
     for name in ("bbk_dsytrs", "bbk_ssytrs"):
         getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp, vp, i64]
     L.bbk_dsyberr.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, vp]
+    L.bbk_dresidual.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, ctypes.POINTER(f64)]
+    L.bbk_dsampled_residual.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, i32, i32,
+                                        ctypes.POINTER(f64)]
     L.bbk_dsysv_mixed.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, f64,
                                   i32, ip, ctypes.POINTER(f64)]
     L.bbk_dsysv.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64,
@@ -214,6 +217,27 @@ class Handle:
             self.check(lib().bbk_dsyberr(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb,
                                          _ptr(X), ldx, _ptr(eta)), "syberr")
         return eta
+
+    def residual(self, A, F, perm, piv) -> float:
+        """||P L D L^T P^T - A||_F / max(||A||_F, 1) on the device (A: the
+        factored matrix's upper triangle; F, perm, piv from sytrf)."""
+        out = ctypes.c_double()
+        with self.ordered():
+            self.check(lib().bbk_dresidual(self.h, A.shape[0], _ptr(A), A.stride(0), _ptr(F),
+                                           F.stride(0), _ptr(perm), _ptr(piv),
+                                           ctypes.byref(out)), "residual")
+        return out.value
+
+    def sampled_residual(self, A, F, perm, piv, seed: int, count: int = 4096) -> float:
+        """The stress test's sampled relative residual (same sample positions
+        for the same seed) on the device."""
+        out = ctypes.c_double()
+        with self.ordered():
+            self.check(lib().bbk_dsampled_residual(self.h, A.shape[0], _ptr(A), A.stride(0),
+                                                   _ptr(F), F.stride(0), _ptr(perm), _ptr(piv),
+                                                   seed, count, ctypes.byref(out)),
+                       "sampled residual")
+        return out.value
 
     def sysv_mixed(self, A, B, X, tol: float = 1e-12, max_iter: int = 30):
         n, lda = A.shape[0], A.stride(0)

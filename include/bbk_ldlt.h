@@ -240,6 +240,30 @@ int bbk_kernel_update_trailing_host(bbk_handle_t handle, int rows, int cols,
                                     int work_stride, double* target,
                                     int target_stride);
 
+/* ---- reconstruction residuals of compact factors ----------------------
+ * A: the matrix that was factored, its upper triangle read as the
+ * symmetric matrix it defines (row-major, lda); F, perm, piv: the output of
+ * bbk_dsytrf.  Both return the residual in *residual (host) after the
+ * handle's stream has finished them.
+ *
+ * bbk_dresidual: ||P L D L^T P^T - A||_F / max(||A||_F, 1), the relative
+ * residual of ReconstructOriginal() against the input
+ * (src/block_ldlt_decomposition.cc:61-78; tests/block_ldlt_decomposition_
+ * test.cc:131-135, RelativeError), formed on the device with FP64 MFMA
+ * from F without building L, D or the product (n^3/3 flops).
+ *
+ * bbk_dsampled_residual: SampledRelativeResidual(original, decomposition,
+ * seed) of tests/block_ldlt_large_stress_test.cc:196-244: the same `count`
+ * sample positions (the four fixed ones, then std::mt19937(seed + 170003)
+ * pairs; the reference uses count = 4096), each entry reconstructed as
+ * ReconstructedPermutedEntry does. */
+int bbk_dresidual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
+                  const double* d_F, int64_t ldf, const int* d_perm, const int* d_piv,
+                  double* residual);
+int bbk_dsampled_residual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
+                          const double* d_F, int64_t ldf, const int* d_perm,
+                          const int* d_piv, int seed, int count, double* residual);
+
 /* ---- synthetic inputs (bench/test plumbing) ---------------------------
  * A(i,j) = A(j,i) = scale * u(seed,i,j), u uniform in [-1,1) from
  * SplitMix64 keyed by (seed, i, j); A(i,i) = 0 when zero_diag_stride > 0
