@@ -62,6 +62,7 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp, vp, i64]
     L.bbk_dsyberr.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, vp]
     L.bbk_dresidual.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, ctypes.POINTER(f64)]
+    L.bbk_dreconstruct.argtypes = [vp, i32, vp, i64, vp, vp, vp, i64]
     L.bbk_dsampled_residual.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, i32, i32,
                                         ctypes.POINTER(f64)]
     L.bbk_dsysv_mixed.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, f64,
@@ -217,6 +218,16 @@ class Handle:
             self.check(lib().bbk_dsyberr(self.h, n, nrhs, _ptr(A), lda, _ptr(B), ldb,
                                          _ptr(X), ldx, _ptr(eta)), "syberr")
         return eta
+
+    def reconstruct(self, F, perm, piv):
+        """P L D L^T P^T (n x n float64 on F's device) from compact factors."""
+        n = F.shape[0]
+        out = self._torch.empty((n, n), dtype=self._torch.float64, device=F.device)
+        with self.ordered():
+            self.check(lib().bbk_dreconstruct(self.h, n, _ptr(F), F.stride(0), _ptr(perm),
+                                              _ptr(piv), _ptr(out), out.stride(0)),
+                       "reconstruct")
+        return out
 
     def residual(self, A, F, perm, piv) -> float:
         """||P L D L^T P^T - A||_F / max(||A||_F, 1) on the device (A: the

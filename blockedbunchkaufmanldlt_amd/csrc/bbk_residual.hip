@@ -55,14 +55,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 //                           + D(p-1, p) L(j, p-1) (piv[p] = -2),
 // over p < min(i0 + 64, n) (L(i, p) = 0 for p > i).  Both operands are
 // formed while staged, from rows p, p-1, p+1 of F (contiguous in i, j).
-// Epilogue: r = M(i, j) - A(perm i, perm j) for i <= j < n, weighted 2 off
-// the diagonal (both triangles); part[2 t] = sum w r^2, part[2 t + 1] =
-// sum w A^2, fixed order inside the tile.
-__global__ __launch_bounds__(256) void k_residual_tiles(int n, const double* __restrict__ A,
-                                                         int64_t lda, const double* __restrict__ F,
-                                                         int64_t ldf, const int* __restrict__ perm,
-                                                         const int* __restrict__ piv,
-                                                         double* __restrict__ part) {
+// Epilogue (STORE false): r = M(i, j) - A(perm i, perm j) for i <= j < n,
+// weighted 2 off the diagonal (both triangles); part[2 t] = sum w r^2,
+// part[2 t + 1] = sum w A^2, fixed order inside the tile.  STORE true:
+// out(perm i, perm j) = out(perm j, perm i) = M(i, j), the reconstruction
+// P L D L^T P^T (ReconstructOriginal, src/block_ldlt_decomposition.cc:61-78).
+template <bool STORE>
+__global__ __launch_bounds__(256) void k_ldlt_tiles(int n, const double* __restrict__ A,
+                                                    int64_t lda, const double* __restrict__ F,
+                                                    int64_t ldf, const int* __restrict__ perm,
+                                                    const int* __restrict__ piv,
+                                                    double* __restrict__ part,
+                                                    double* __restrict__ out, int64_t ldo) {
   constexpr int TM = kResTile, KC = kResKC, LDS_LD = TM + 4;
   constexpr int WT = TM / 2, TI = WT / 16;
   typedef Mfma<double>::acc_t acc_t;
@@ -135,6 +139,23 @@ __global__ __launch_bounds__(256) void k_residual_tiles(int n, const double* __r
     }
     if (kc + 1 < nk) stash(buf ^ 1);
     __syncthreads();
+  }
+  if constexpr (STORE) {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < TI; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = i0 + wr * WT + ti * 16 + Mfma<double>::row(lane, v);
+          const int j = j0 + wc * WT + tj * 16 + (lane & 15);
+          if (i < n && j < n && i <= j) {
+            const int pi = perm[i], pj = perm[j];
+            out[(int64_t)pi * ldo + pj] = acc[ti][tj][v];
+            out[(int64_t)pj * ldo + pi] = acc[ti][tj][v];
+          }
+        }
+    return;
   }
   double sr = 0.0, sa = 0.0;
 #pragma unroll
@@ -315,12 +336,77 @@ int bbk_dresidual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
   if (rc != BBK_OK) return rc;
   double* part = reinterpret_cast<double*>(h->berr_ws);
   double* res = part + 2 * ntile;
-  hipLaunchKernelGGL(bbk::k_residual_tiles, dim3(static_cast<unsigned>(ntile)), dim3(256), 0,
-                     h->stream, n, d_A, lda, d_F, ldf, d_perm, d_piv, part);
+  hipLaunchKernelGGL(bbk::k_ldlt_tiles<false>, dim3(static_cast<unsigned>(ntile)), dim3(256), 0,
+                     h->stream, n, d_A, lda, d_F, ldf, d_perm, d_piv, part, nullptr, 0);
   hipLaunchKernelGGL(bbk::k_residual_reduce, dim3(1), dim3(256), 0, h->stream, part, ntile, res);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return h->hip_fail(e, "residual kernels");
   return bbk::read_result(h, res, residual);
+}
+
+int bbk_dreconstruct(bbk_handle_t handle, int n, const double* d_F, int64_t ldf,
+                     const int* d_perm, const int* d_piv, double* d_out, int64_t ldo) {
+  if (!handle) return -1;
+  if (n <= 0) return -2;
+  if (!d_F) return -3;
+  if (ldf < n) return -4;
+  if (!d_perm) return -5;
+  if (!d_piv) return -6;
+  if (!d_out) return -7;
+  if (ldo < n) return -8;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  const int64_t tiles = (n + bbk::kResTile - 1) / bbk::kResTile;
+  const int64_t ntile = tiles * (tiles + 1) / 2;
+  hipLaunchKernelGGL(bbk::k_ldlt_tiles<true>, dim3(static_cast<unsigned>(ntile)), dim3(256), 0,
+                     h->stream, n, nullptr, 0, d_F, ldf, d_perm, d_piv, nullptr, d_out, ldo);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h->hip_fail(e, "reconstruction kernel");
+  return BBK_OK;
+}
+
+int bbk_dreconstruct_host(bbk_handle_t handle, int n, const double* F, int64_t ldf,
+                          const int* perm, const int* piv, double* out, int64_t ldo) {
+  if (!handle) return -1;
+  if (n <= 0) return -2;
+  if (!F) return -3;
+  if (ldf < n) return -4;
+  if (!perm) return -5;
+  if (!piv) return -6;
+  if (!out) return -7;
+  if (ldo < n) return -8;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  const size_t mat = sizeof(double) * static_cast<size_t>(n) * n;
+  double *dF = nullptr, *dO = nullptr;
+  int *dp = nullptr, *dv = nullptr;
+  hipError_t e = hipMalloc(&dF, mat);
+  if (e == hipSuccess) e = hipMalloc(&dO, mat);
+  if (e == hipSuccess) e = hipMalloc(&dp, sizeof(int) * n);
+  if (e == hipSuccess) e = hipMalloc(&dv, sizeof(int) * n);
+  int rc = BBK_OK;
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(dF, sizeof(double) * n, F, sizeof(double) * ldf, sizeof(double) * n, n,
+                         hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dp, perm, sizeof(int) * n, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dv, piv, sizeof(int) * n, hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) {
+    rc = h->hip_fail(e, "reconstruction upload");
+  } else {
+    rc = bbk_dreconstruct(handle, n, dF, n, dp, dv, dO, n);
+    if (rc == BBK_OK) {
+      e = hipMemcpy2DAsync(out, sizeof(double) * ldo, dO, sizeof(double) * n,
+                           sizeof(double) * n, n, hipMemcpyDeviceToHost, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) rc = h->hip_fail(e, "reconstruction download");
+    }
+  }
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(dF);
+  (void)hipFree(dO);
+  (void)hipFree(dp);
+  (void)hipFree(dv);
+  return rc;
 }
 
 int bbk_dsampled_residual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,

@@ -180,10 +180,16 @@ RegularMatrix BlockLdltDecomposition::ReconstructOriginal() const {
     throw std::logic_error("ReconstructOriginal() requires full factor storage");
   if (!factorized_)
     throw std::logic_error("Factorize() must be called before ReconstructOriginal()");
-  const RegularMatrix permuted = Multiply(Multiply(l_factor_, d_factor_), Transpose(l_factor_));
+  // P L D L^T P^T formed on the GPU from the compact factors (the reference
+  // multiplies the dense L, D, L^T on the host)
+  const RegularMatrix compact = factors_.ToUpperRegular();
   RegularMatrix out(original_.Rows(), original_.Cols());
-  for (int i = 0; i < out.Rows(); ++i)
-    for (int j = 0; j < out.Cols(); ++j) out(permutation_[i], permutation_[j]) = permuted(i, j);
+  std::lock_guard<std::mutex> lock(CallMutex());
+  bbk_handle_t h = SharedHandle();
+  Check(bbk_dreconstruct_host(h, out.Rows(), compact.Data(), compact.LeadingDimension(),
+                              permutation_.data(), pivot_types_.data(), out.Data(),
+                              out.LeadingDimension()),
+        h, "bbk_dreconstruct_host");
   return out;
 }
 

@@ -257,6 +257,15 @@ int bbk_kernel_update_trailing_host(bbk_handle_t handle, int rows, int cols,
  * sample positions (the four fixed ones, then std::mt19937(seed + 170003)
  * pairs; the reference uses count = 4096), each entry reconstructed as
  * ReconstructedPermutedEntry does. */
+/* bbk_dreconstruct: out = P L D L^T P^T (n x n, row-major, ldo), the matrix
+ * ReconstructOriginal() returns (src/block_ldlt_decomposition.cc:61-78),
+ * formed on the device from F with FP64 MFMA (queued on the handle's
+ * stream).  bbk_dreconstruct_host: the same from and into host buffers
+ * (returns when `out` is filled). */
+int bbk_dreconstruct(bbk_handle_t handle, int n, const double* d_F, int64_t ldf,
+                     const int* d_perm, const int* d_piv, double* d_out, int64_t ldo);
+int bbk_dreconstruct_host(bbk_handle_t handle, int n, const double* F, int64_t ldf,
+                          const int* perm, const int* piv, double* out, int64_t ldo);
 int bbk_dresidual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
                   const double* d_F, int64_t ldf, const int* d_perm, const int* d_piv,
                   double* residual);

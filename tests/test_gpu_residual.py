@@ -122,3 +122,23 @@ def test_residual_argument_errors(handle):
     assert L.bbk_dresidual(handle.h, 0, None, 1, None, 1, None, None, ctypes.byref(out)) == -2
     assert L.bbk_dsampled_residual(handle.h, 4, 1, 4, 1, 4, 1, 1, 0, 2,
                                    ctypes.byref(out)) == -11
+
+
+@pytest.mark.parametrize("n", [1, 5, 130, 400])
+def test_reconstruct_matches_dense_product(oracle, handle, n):
+    """bbk_dreconstruct against P L D L^T P^T built densely from the
+    oracle's ExtractRegularFactorsFromCompact restatement (same factors)."""
+    A = oracle.gen_synth_symmetric(n, 60 + n)
+    T, perm, piv = factor(handle, A, 32)
+    F = np.triu(T[:, :n].cpu().numpy())
+    g = oracle.Factorization(n, 32, F, perm.cpu().numpy().astype(np.int32),
+                             piv.cpu().numpy().astype(np.int32))
+    L, D = oracle.extract_ld(g)
+    M = L @ D @ L.T
+    R = np.empty_like(M)
+    R[np.ix_(g.perm, g.perm)] = M
+    out = handle.reconstruct(T, perm, piv).cpu().numpy()
+    assert np.allclose(out, out.T, rtol=0, atol=0)
+    scale = max(1.0, np.abs(M).max())
+    assert np.abs(out - R).max() <= 1e-13 * scale * max(1, n)
+    assert np.abs(out - A).max() <= 1e-12 * scale * max(1, n)
