@@ -241,13 +241,29 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(K):
-        h.sytrf(copies[W + k])
+        last = h.sytrf(copies[W + k])
     h.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     tm = h.last_timing()
     h.set_timing(False)
     log(f"timed {K} factorizations: {elapsed * 1e3 / K:.2f} ms each; kernels {tm}")
+    # verification of the last timed factorization (outside the timed region):
+    # ||P L D L^T P^T - A||_F / max(||A||_F, 1) and the reference stress
+    # test's sampled residual, both on the device
+    verification = None
+    if K > 0:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res_full = h.residual(A0, copies[W + K - 1], last[0], last[1])
+        res_ms = (time.perf_counter() - t1) * 1e3
+        res_sampled = h.sampled_residual(A0, copies[W + K - 1], last[0], last[1], args.seed)
+        log(f"residual full {res_full:.3e} ({res_ms:.1f} ms), sampled {res_sampled:.3e}")
+        verification = {"relative_residual": res_full,
+                        "sampled_relative_residual": res_sampled,
+                        "residual_ms": round(res_ms, 2),
+                        "n_2x2_pivots": int((last[1] == 2).sum())}
+        del last
     del copies
     torch.cuda.empty_cache()
 
@@ -343,6 +359,7 @@ def main():
         "roofline_panel": roof_panel,
         "fraction_of_fp64_peak": round(gflops * 1e9 / PEAK_FP64, 4),
         "time_to_1e-12": solve,
+        "verification": verification,
         "batched": batched,
         "cpu_baseline": cpu,
     }
