@@ -234,6 +234,90 @@ __device__ __forceinline__ T panel_dot(const T* L, const T* Wl, const T (&wr)[NR
   return (a0 + a1) + (a2 + a3);
 }
 
+// Three products at once over q < kk (W read once): d0, d1, d2 with L0,
+// L1, L2.  Binary32 forms the candidate row's product and both possible next
+// rows' products together in phase B, under the candidate element's HBM
+// load, instead of the next rows' during exchange 2.
+template <typename T, int WS, int RR, int NR>
+__device__ __forceinline__ void panel_dot3(const T* L0, const T* L1, const T* L2, const T* Wl,
+                                           const T (&wr)[NR], int tid, int kk, int base, T& d0,
+                                           T& d1, T& d2) {
+  T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0), c0 = T(0), c1 = T(0);
+  constexpr int VE = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VE)));
+  if constexpr (RR > 0) {
+    if (base > 0) {
+      const vec_t* V0 = reinterpret_cast<const vec_t*>(L0);
+      const vec_t* V1 = reinterpret_cast<const vec_t*>(L1);
+      const vec_t* V2 = reinterpret_cast<const vec_t*>(L2);
+#pragma unroll
+      for (int b = 0; b < RR; b += 8) {
+        T l0[8], l1[8], l2[8];
+#pragma unroll
+        for (int u = 0; u < 8 / VE; ++u) {
+          const vec_t t0 = V0[b / VE + u], t1 = V1[b / VE + u], t2 = V2[b / VE + u];
+#pragma unroll
+          for (int e = 0; e < VE; ++e) {
+            l0[u * VE + e] = t0[e];
+            l1[u * VE + e] = t1[e];
+            l2[u * VE + e] = t2[e];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          a0 += l0[u] * wr[b + u];
+          a1 += l0[u + 1] * wr[b + u + 1];
+          b0 += l1[u] * wr[b + u];
+          b1 += l1[u + 1] * wr[b + u + 1];
+          c0 += l2[u] * wr[b + u];
+          c1 += l2[u + 1] * wr[b + u + 1];
+        }
+      }
+    }
+  }
+  const T* Wc = Wl + tid;
+  const T* L0b = L0 + base;
+  const T* L1b = L1 + base;
+  const T* L2b = L2 + base;
+  const int cnt = kk - base;
+  int q = 0;
+  for (; q + 8 <= cnt; q += 8) {
+    T l0[8], l1[8], l2[8], wv[8];
+    const vec_t* V0 = reinterpret_cast<const vec_t*>(L0b + q);
+    const vec_t* V1 = reinterpret_cast<const vec_t*>(L1b + q);
+    const vec_t* V2 = reinterpret_cast<const vec_t*>(L2b + q);
+#pragma unroll
+    for (int u = 0; u < 8 / VE; ++u) {
+      const vec_t t0 = V0[u], t1 = V1[u], t2 = V2[u];
+#pragma unroll
+      for (int e = 0; e < VE; ++e) {
+        l0[u * VE + e] = t0[e];
+        l1[u * VE + e] = t1[e];
+        l2[u * VE + e] = t2[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) wv[u] = Wc[(q + u) * WS];
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      a0 += l0[u] * wv[u];
+      a1 += l0[u + 1] * wv[u + 1];
+      b0 += l1[u] * wv[u];
+      b1 += l1[u + 1] * wv[u + 1];
+      c0 += l2[u] * wv[u];
+      c1 += l2[u + 1] * wv[u + 1];
+    }
+  }
+  for (; q < cnt; ++q) {
+    a0 += L0b[q] * Wc[q * WS];
+    b0 += L1b[q] * Wc[q * WS];
+    c0 += L2b[q] * Wc[q * WS];
+  }
+  d0 = a0 + a1;
+  d1 = b0 + b1;
+  d2 = c0 + c1;
+}
+
 // Two products at once over q < kk: d1 = sum_q L1[q] W(q, own column),
 // d2 = sum_q L2[q] W(q, own column) (same storage as panel_dot; W is read
 // once).  Used for the next pivot's row while an exchange is in flight.
@@ -582,7 +666,13 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
       __syncthreads();
       BBK_PT(6);
-      const T dotB = active ? panel_dot<T, WS, RR>(Lr, Wl, wr, tid, kk, base) : T(0);
+      T dotB = T(0);
+      if (active) {
+        if constexpr (sizeof(T) == 4)
+          panel_dot3<T, WS, RR>(Lr, Lu1, Lu2, Wl, wr, tid, kk, base, dotB, pd1, pd2);
+        else
+          dotB = panel_dot<T, WS, RR>(Lr, Wl, wr, tid, kk, base);
+      }
       if (!active) a = T(0);
       if (!want1) P1 = T(0);
       if (!want2) P2 = T(0);
@@ -614,7 +704,9 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       }
       // while the records travel: the next pivot's row product over the
       // finished rows q < kk for both possible next rows (k+1, k+2)
-      if (active && kk + 1 < bt && has1)
+      // (binary32 formed them in phase B; binary64, at 256 VGPRs, measured
+      // faster forming them here)
+      if (sizeof(T) == 8 && active && kk + 1 < bt && has1)
         panel_dot2<T, WS, RR>(Lu1, Lu2, Wl, wr, tid, kk, base, pd1, pd2);
       const int npub = 2 + (has1 ? 2 : 0) + (has2 ? 2 : 0);
       const int total = G + npub;
