@@ -90,15 +90,17 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
   }
 }
 
-// Tile shape (measured on MI355X at m = 16384, K = 128): 64 x 64 tiles
-// with eight workgroups resident per CU let one workgroup's C traffic
-// overlap another's MFMAs (binary64 34.6 TFLOP/s against 30.3 for 128 x 128
-// tiles and 33.6 with four resident; binary32 60.3 against 54.1); K chunks
-// of 8; XCD-contiguous supertile order.
+// Tile shape (measured on MI355X at n = 16384, K = 128, inside the
+// factorization): binary64 64 x 64 tiles with eight workgroups resident per
+// CU, so one workgroup's C traffic overlaps another's MFMAs (42.6 ms for all
+// updates; C added after the K loop at five per CU 47.5 ms, operands two
+// chunks ahead at seven per CU 44.8 ms); binary32 128 x 128 tiles, five per
+// CU by registers (27.3 ms against 27.8 ms for 64 x 64 and 28.4 ms with K
+// chunks of 16).  K chunks of 8; XCD-contiguous supertile order.
 template <typename T>
 struct UpdateCfg {
-  static constexpr int kTM = 64;
-  static constexpr int kMinBlocks = 8;
+  static constexpr int kTM = sizeof(T) == 4 ? 128 : 64;
+  static constexpr int kMinBlocks = sizeof(T) == 4 ? 4 : 8;
   static constexpr int kKC = 8;
 };
 

@@ -158,22 +158,24 @@ __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
 
   // staging: wavefront w fetches K rows w, w + 4, ... of a chunk, one
   // 64-wide row per load (row address wave-uniform, lane = column)
-  static_assert(TM == 64, "staging assumes one tile row per wavefront load");
-  const bool iok = full || i0 + lane < n, jok = full || j0 + lane < n;
+  static_assert(TM == 64 || TM == 128, "staging: a wavefront loads 64 columns of one row");
+  constexpr int RSTEP = 256 / TM;  // rows per pass of the workgroup
+  const int qw = (wave * 64) / TM, cw = (wave * 64) % TM + lane;
+  const bool iok = full || i0 + cw < n, jok = full || j0 + cw < n;
   T ru[PER], rw[PER];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int kq = k0 + wave + 4 * u;
-      ru[u] = (kq < bw && iok) ? -A[(int64_t)(s + kq) * lda + i0 + lane] : T(0);
-      rw[u] = (kq < bw && jok) ? Wg[(int64_t)kq * ldw + j0 + lane] : T(0);
+      const int kq = k0 + qw + RSTEP * u;
+      ru[u] = (kq < bw && iok) ? -A[(int64_t)(s + kq) * lda + i0 + cw] : T(0);
+      rw[u] = (kq < bw && jok) ? Wg[(int64_t)kq * ldw + j0 + cw] : T(0);
     }
   };
   auto stash = [&](int buf) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      Us[buf][(wave + 4 * u) * LDS_LD + lane] = ru[u];
-      Ws[buf][(wave + 4 * u) * LDS_LD + lane] = rw[u];
+      Us[buf][(qw + RSTEP * u) * LDS_LD + cw] = ru[u];
+      Ws[buf][(qw + RSTEP * u) * LDS_LD + cw] = rw[u];
     }
   };
 
