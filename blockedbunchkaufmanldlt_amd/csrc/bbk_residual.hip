@@ -409,6 +409,19 @@ int bbk_dreconstruct_host(bbk_handle_t handle, int n, const double* F, int64_t l
   return rc;
 }
 
+int bbk_residual_samples(int n, int seed, int count, int* rows, int* cols) {
+  if (n <= 0) return -1;
+  if (count < 4) return -3;
+  if (!rows) return -4;
+  if (!cols) return -5;
+  const std::vector<int> smp = bbk::residual_samples(n, seed, count);
+  for (int t = 0; t < count; ++t) {
+    rows[t] = smp[2 * t];
+    cols[t] = smp[2 * t + 1];
+  }
+  return BBK_OK;
+}
+
 int bbk_dsampled_residual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
                           const double* d_F, int64_t ldf, const int* d_perm, const int* d_piv,
                           int seed, int count, double* residual) {

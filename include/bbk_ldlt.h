@@ -272,6 +272,10 @@ int bbk_dresidual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
 int bbk_dsampled_residual(bbk_handle_t handle, int n, const double* d_A, int64_t lda,
                           const double* d_F, int64_t ldf, const int* d_perm,
                           const int* d_piv, int seed, int count, double* residual);
+/* The sample positions bbk_dsampled_residual uses (ResidualSamples,
+ * tests/block_ldlt_large_stress_test.cc:212-227), into host arrays of
+ * `count` entries; host only, no device work. */
+int bbk_residual_samples(int n, int seed, int count, int* rows, int* cols);
 
 /* ---- synthetic inputs (bench/test plumbing) ---------------------------
  * A(i,j) = A(j,i) = scale * u(seed,i,j), u uniform in [-1,1) from

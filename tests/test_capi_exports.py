@@ -34,8 +34,27 @@ def test_header_declares_the_api():
     for must in ("bbk_create", "bbk_destroy", "bbk_dsytrf", "bbk_ssytrf", "bbk_dsytrf_host",
                  "bbk_dsytrs", "bbk_ssytrs", "bbk_dsyberr", "bbk_dsysv", "bbk_dsysv_mixed",
                  "bbk_dsytrf_batched", "bbk_ssytrf_batched", "bbk_kernel_scale_1x1",
-                 "bbk_kernel_scale_2x2", "bbk_kernel_update_trailing"):
+                 "bbk_kernel_scale_2x2", "bbk_kernel_update_trailing", "bbk_dresidual",
+                 "bbk_dsampled_residual", "bbk_dreconstruct", "bbk_dreconstruct_host",
+                 "bbk_residual_samples"):
         assert must in names, must
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built (python -m blockedbunchkaufmanldlt_amd.build)")
+def test_residual_sample_positions_match_reference_fixture():
+    """bbk_dsampled_residual's sample positions (drawn on the host) are the
+    reference stress test's: checked against the fixture written from the
+    reference's own generator (tests/block_ldlt_large_stress_test.cc:212-227)."""
+    import numpy as np
+    g = np.load(os.path.join(ROOT, "tests", "golden", "reference_factors.npz"))
+    expected = g["samples_5000_931300"]
+    count = expected.shape[1]
+    lib = ctypes.CDLL(LIB)
+    rows = (ctypes.c_int * count)()
+    cols = (ctypes.c_int * count)()
+    assert lib.bbk_residual_samples(5000, 931300, count, rows, cols) == 0
+    assert np.array_equal(np.stack([np.array(rows), np.array(cols)]), expected)
+    assert lib.bbk_residual_samples(5000, 1, 3, rows, cols) == -3
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built (python -m blockedbunchkaufmanldlt_amd.build)")
