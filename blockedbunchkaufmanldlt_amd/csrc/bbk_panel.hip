@@ -69,16 +69,19 @@ __device__ __forceinline__ void stc(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Device-coherent load the compiler sees and waits for (phase B, where no
-// store of the pivot is in flight behind it).
+// Device-coherent load the compiler sees and waits for.  Used in phase B:
+// there every value loaded is consumed before the next store of the pivot,
+// so the compiler's waits never hold back a store, and it can wait for the
+// older loads (vmcnt(n) > 0) while the younger ones are still in flight.
 template <typename T>
 __device__ __forceinline__ T ldc(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Device-coherent loads issued together, with the wait for them in the same
-// asm block.  Inside the pivot loop every load goes through these: a load
-// the compiler sees in flight makes it wait, where control flow merges, for
+// asm block, for loads whose values live across the pivot's stores (the
+// next pivot's row and L row on the 1x1 path, the first row): a load the
+// compiler sees in flight makes it wait, where control flow merges, for
 // every store issued since (one counter on gfx9 covers loads and stores),
 // which would serialize the pivot's coherent stores.  Every pointer must be
 // valid; lanes that need no value pass a harmless address.
@@ -102,23 +105,6 @@ struct CoherentLoad<double> {
         : "v"(p0), "v"(p1)
         : "memory");
   }
-  static __device__ __forceinline__ void six(const double* p0, const double* p1,
-                                             const double* p2, const double* p3,
-                                             const double* p4, const double* p5, double& v0,
-                                             double& v1, double& v2, double& v3, double& v4,
-                                             double& v5) {
-    asm volatile(
-        "global_load_dwordx2 %0, %6, off sc1\n\t"
-        "global_load_dwordx2 %1, %7, off sc1\n\t"
-        "global_load_dwordx2 %2, %8, off sc1\n\t"
-        "global_load_dwordx2 %3, %9, off sc1\n\t"
-        "global_load_dwordx2 %4, %10, off sc1\n\t"
-        "global_load_dwordx2 %5, %11, off sc1\n\t"
-        "s_waitcnt vmcnt(0)"
-        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3), "=&v"(v4), "=&v"(v5)
-        : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4), "v"(p5)
-        : "memory");
-  }
 };
 template <>
 struct CoherentLoad<float> {
@@ -136,22 +122,6 @@ struct CoherentLoad<float> {
         "s_waitcnt vmcnt(0)"
         : "=&v"(v0), "=&v"(v1)
         : "v"(p0), "v"(p1)
-        : "memory");
-  }
-  static __device__ __forceinline__ void six(const float* p0, const float* p1, const float* p2,
-                                             const float* p3, const float* p4, const float* p5,
-                                             float& v0, float& v1, float& v2, float& v3,
-                                             float& v4, float& v5) {
-    asm volatile(
-        "global_load_dword %0, %6, off sc1\n\t"
-        "global_load_dword %1, %7, off sc1\n\t"
-        "global_load_dword %2, %8, off sc1\n\t"
-        "global_load_dword %3, %9, off sc1\n\t"
-        "global_load_dword %4, %10, off sc1\n\t"
-        "global_load_dword %5, %11, off sc1\n\t"
-        "s_waitcnt vmcnt(0)"
-        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3), "=&v"(v4), "=&v"(v5)
-        : "v"(p0), "v"(p1), "v"(p2), "v"(p3), "v"(p4), "v"(p5)
         : "memory");
   }
 };
@@ -642,23 +612,16 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       const T* pl = wantu ? &A[(int64_t)(s + tid) * lda + r] : dummy;
       const T* p2 = want2 ? &A[(int64_t)(k + 2) * lda + j] : dummy;
       const T* pu2 = (wantu && use2) ? &A[(int64_t)(s + tid) * lda + k + 2] : dummy;
-      // binary32: the finished rows' columns (recently written,
-      // cache-resident) are issued first and the candidate element and rows
-      // k+1, k+2 (HBM) last: loads return in issue order, so the product over
-      // the finished rows runs while the HBM loads are still in flight.
-      // binary64 (all 256 VGPRs in use) waits for all six at once, which
-      // measured faster there.
-      T lr, a;
-      if constexpr (sizeof(T) == 4) {
-        lr = ldc(pl);
-        u1 = ldc(pu1);
-        u2 = ldc(pu2);
-        a = ldc(pa);
-        P1 = ldc(p1);
-        P2 = ldc(p2);
-      } else {
-        CoherentLoad<T>::six(pa, pl, p1, p2, pu1, pu2, a, lr, P1, P2, u1, u2);
-      }
+      // The finished rows' columns (recently written, cache-resident) are
+      // issued first and the candidate element and rows k+1, k+2 (HBM) last:
+      // loads return in issue order, so the product over the finished rows
+      // runs while the HBM loads are still in flight.
+      const T lr = ldc(pl);
+      u1 = ldc(pu1);
+      u2 = ldc(pu2);
+      T a = ldc(pa);
+      P1 = ldc(p1);
+      P2 = ldc(p2);
       if (tid < kk) {
         Lr[tid] = lr;
         Lu1[tid] = u1;
