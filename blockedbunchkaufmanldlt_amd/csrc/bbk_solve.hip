@@ -243,6 +243,44 @@ __global__ __launch_bounds__(256) void k_mirror_upper(int n, T* __restrict__ F, 
   }
 }
 
+// One 128 x 128 factor tile into LDS (row-major, stride 128), read along
+// rows of F: Fl[a][b] = F(p0 + a, q0 + b) for a < pn and b < qn, else 0
+// (outside the matrix).  Forward sweep (MIRROR false): rows of block J-1,
+// columns of block J, i.e. L(q0 + b, p0 + a); the D entry of a 2x2 pivot
+// across the block edge, F(q0 - 1, q0) at (127, 0), reads as 0.  Backward
+// sweep (MIRROR true): F's mirrored lower triangle, rows of block I+1 and
+// columns of block I, i.e. L(p0 + a, q0 + b); the edge entry is (0, 127)
+// when the 2x2 pivot is (q0 + 127, q0 + 128).
+template <typename T, bool MIRROR>
+__device__ __forceinline__ void stage_last_tile(T* Fl, const T* __restrict__ F, int64_t lda,
+                                                const int* __restrict__ piv, int p0, int pn,
+                                                int q0, int qn) {
+  constexpr int NB = kSolveBlock;
+  constexpr int PER = NB * NB / kSolveThreads / 2;
+  const int b = threadIdx.x & (NB - 1), a0 = threadIdx.x >> 7;
+  const bool ok = b < qn;
+  bool edge;
+  if (MIRROR)
+    edge = qn == NB && b == NB - 1 && piv[q0 + NB - 1] == 2;  // Fl[0][127]
+  else
+    edge = b == 0 && piv[q0 - 1] == 2;                         // Fl[127][0]
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int a = a0 + 2 * (u + PER * hh);
+      v[u] = (ok && a < pn) ? F[(int64_t)(p0 + a) * lda + q0 + b] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int a = a0 + 2 * (u + PER * hh);
+      const bool zero = edge && (MIRROR ? a == 0 : a == NB - 1);
+      Fl[a * NB + b] = zero ? T(0) : v[u];
+    }
+  }
+}
+
 // Li (128 x 128 row-major) into LDS with row stride 129.
 template <typename T>
 __device__ __forceinline__ void stage_inverse_block(T* Lp, const T* __restrict__ Li) {
@@ -340,17 +378,26 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
   __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
   __shared__ T part[4][NB][NR];
   __shared__ T ztop[INV ? NB : H][NR];
+  // INV: the tile L_{J,J-1} (block J-1's rows, this block's columns), the
+  // last contribution before the diagonal product, staged when the
+  // workgroup starts so that no HBM load sits between block J-1's records
+  // arriving and block J's product
+  __shared__ T Fl[INV ? NB * NB : 1];
   if (stop && *stop) return;  // every workgroup reads the same flag
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
     const int j0 = J * NB;
     const int jn = min(NB, n - j0);
     const int cA = j0 + lane, cB = j0 + H + lane;
     const bool okA = lane < jn, okB = H + lane < jn;
-    if constexpr (INV)
+    if constexpr (INV) {
       stage_inverse_block<T>(Lp, Li + (int64_t)J * NB * NB);
-    else
+      if (J > 0) stage_last_tile<T, false>(Fl, F, lda, piv, (J - 1) * NB, NB, j0, jn);
+      __syncthreads();  // Fl is read, by other threads, before the next barrier
+    } else {
       stage_diag_block<T>(Lp, F, lda, piv, j0, jn, n);
+    }
     T bj[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) bj[q] = T(0);
@@ -374,7 +421,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     // records in flight together, one wait per step
     constexpr int UB = sizeof(T) == 8 ? 1 : 2;
     const int cAc = min(cA, n - 1), cBc = min(cB, n - 1);
-    for (int I = 0; I < J; I += UB) {
+    const int Jm = J - LAST;  // blocks I < Jm in the main loop
+    for (int I = 0; I < Jm; I += UB) {
       T fa[2 * UB][16], fb[2 * UB][16];
       int row0[2 * UB];
 #pragma unroll
@@ -387,11 +435,11 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
           fa[k][ii] = F[ro + cAc];
           fb[k][ii] = F[ro + cBc];
         }
-        row0[k] = min(Iu, J - 1) * NB + 32 * w + 16 * (k & 1);
+        row0[k] = min(Iu, Jm - 1) * NB + 32 * w + 16 * (k & 1);
       }
 #pragma unroll
       for (int k = 0; k < 2 * UB; ++k) {
-        const bool live = I + k / 2 < J;
+        const bool live = I + k / 2 < Jm;
 #pragma unroll
         for (int ii = 0; ii < 16; ++ii) {
           if (!(live && okA)) fa[k][ii] = T(0);
@@ -412,6 +460,27 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
             acc[1][q] += fb[k][ii] * z;
           }
         }
+    }
+    if constexpr (INV) {
+      if (J > 0) {  // block J-1 from the staged tile
+        const int rb = (J - 1) * NB + 32 * w;
+        const int row0[2] = {rb, rb + 16};
+        double zv[2];
+        poll_rows_multi<NR, 2>(rec, row0, n, tag, zv);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int ii = 0; ii < 16; ++ii) {
+            const int r = 32 * w + 16 * k + ii;
+            const T a = Fl[r * NB + lane], b = Fl[r * NB + H + lane];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const T z = static_cast<T>(bcast(zv[k], ii * 4 + q));
+              acc[0][q] += a * z;
+              acc[1][q] += b * z;
+            }
+          }
+      }
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -531,17 +600,26 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
   __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
   __shared__ T part[4][NB][NR];
   __shared__ T vbot[INV ? NB : H][NR];
+  // INV: the tile L_{I+1,I}^T (rows of block I, columns of block I+1; from
+  // the mirrored lower triangle), staged when the workgroup starts
+  __shared__ T Fl[INV ? NB * NB : 1];
   if (stop && *stop) return;  // every workgroup reads the same flag
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
     const int i0 = I * NB;
     const int in = min(NB, n - i0);
     const int rA = i0 + lane, rB = i0 + H + lane;
     const bool okA = lane < in, okB = H + lane < in;
-    if constexpr (INV)
+    if constexpr (INV) {
       stage_inverse_block<T>(Lp, Li + (int64_t)I * NB * NB);
-    else
+      if (I + 1 < nblk)
+        stage_last_tile<T, true>(Fl, F, lda, piv, (I + 1) * NB, min(NB, n - (I + 1) * NB), i0,
+                                 in);
+      __syncthreads();  // Fl is read, by other threads, before the next barrier
+    } else {
       stage_diag_block<T>(Lp, F, lda, piv, i0, in, n);
+    }
     T wv[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) wv[q] = T(0);
@@ -563,13 +641,14 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     constexpr int VE = 16 / sizeof(T);
     typedef T vec_t __attribute__((ext_vector_type(VE)));
     const bool vec_ok = lda % VE == 0 && (reinterpret_cast<uintptr_t>(F) & 15) == 0;
-    for (int J = nblk - 1; J > I; J -= UB) {
+    const int Im = I + LAST;  // blocks J > Im in the main loop
+    for (int J = nblk - 1; J > Im; J -= UB) {
       T fa[2 * UB][16], fb[2 * UB][16];
       int row0[2 * UB];
 #pragma unroll
       for (int k = 0; k < 2 * UB; ++k) {
         const int Ju = J - k / 2;
-        const int cb = max(Ju, I + 1) * NB + 32 * w + 16 * (k & 1);
+        const int cb = max(Ju, Im + 1) * NB + 32 * w + 16 * (k & 1);
         if constexpr (INV) {
           const int rAc = min(rA, n - 1), rBc = min(rB, n - 1);
 #pragma unroll
@@ -598,13 +677,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
             fb[k][cc] = F[roB + col];
           }
         }
-        row0[k] = max(Ju, I + 1) * NB + 32 * w + 16 * (k & 1);
+        row0[k] = max(Ju, Im + 1) * NB + 32 * w + 16 * (k & 1);
       }
 #pragma unroll
       for (int k = 0; k < 2 * UB; ++k) {
         const int Ju = J - k / 2;
         const int cb = Ju * NB + 32 * w + 16 * (k & 1);
-        const bool live = Ju > I;
+        const bool live = Ju > Im;
 #pragma unroll
         for (int cc = 0; cc < 16; ++cc) {
           const bool colok = live && cb + cc < n;
@@ -626,6 +705,27 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
             acc[1][q] += fb[k][cc] * v;
           }
         }
+    }
+    if constexpr (INV) {
+      if (I + 1 < nblk) {  // block I+1 from the staged tile
+        const int cb = (I + 1) * NB + 32 * w;
+        const int row0[2] = {cb, cb + 16};
+        double vv[2];
+        poll_rows_multi<NR, 2>(rec, row0, n, tag, vv);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int cc = 0; cc < 16; ++cc) {
+            const int c = 32 * w + 16 * k + cc;
+            const T a = Fl[c * NB + lane], b = Fl[c * NB + H + lane];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const T v = static_cast<T>(bcast(vv[k], cc * 4 + q));
+              acc[0][q] += a * v;
+              acc[1][q] += b * v;
+            }
+          }
+      }
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
