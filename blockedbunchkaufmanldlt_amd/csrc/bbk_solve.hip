@@ -281,8 +281,11 @@ __device__ __forceinline__ void stage_last_tile(T* Fl, const T* __restrict__ F, 
   }
 }
 
-// Li (128 x 128 row-major) into LDS with row stride 129.
-template <typename T>
+// Li (128 x 128 row-major, binary32) into LDS, rows of kInvLd elements
+// (16-byte aligned): Lp[r kInvLd + c] = Li(r, c), or Li(c, r) when TRANS, so
+// that both sweeps read rows of the staged matrix.
+constexpr int kInvLd = kSolveBlock + 4;
+template <typename T, bool TRANS>
 __device__ __forceinline__ void stage_inverse_block(T* Lp, const T* __restrict__ Li) {
   constexpr int NB = kSolveBlock;
   constexpr int PER = NB * NB / kSolveThreads / 2;
@@ -294,28 +297,37 @@ __device__ __forceinline__ void stage_inverse_block(T* Lp, const T* __restrict__
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int e = threadIdx.x + kSolveThreads * (u + PER * h);
-      Lp[(e / NB) * (NB + 1) + e % NB] = v[u];
+      const int r = e / NB, c = e % NB;
+      Lp[TRANS ? c * kInvLd + r : r * kInvLd + c] = v[u];
     }
   }
 }
 
 // Half of a product with the staged inverse: thread (r = tid % 128, h = tid
-// / 128) sums, over c in [64 h, 64 h + 64), Li(r, c) t[c] (TRANS false) or
-// Li(c, r) t[c] (TRANS true).  Lanes take consecutive r, so both forms read
-// LDS without bank conflicts (row stride 129); t[c] is one broadcast read.
-template <typename T, int NR, bool TRANS>
+// / 128) sums Lp[r][c] t[c] over c in [64 h, 64 h + 64), reading its row
+// 16 bytes at a time (lanes on consecutive rows, 528 bytes apart: each
+// group of 16 lanes covers the 64 banks once); t[c] is a broadcast read.
+template <typename T, int NR>
 __device__ __forceinline__ void inverse_product(const T* Lp, const T (*t)[NR], T (&sv)[NR]) {
+  static_assert(sizeof(T) == 4, "binary32 inverse blocks");
   constexpr int NB = kSolveBlock;
+  typedef float f4 __attribute__((ext_vector_type(4)));
   const int r = threadIdx.x & (NB - 1), c0 = (threadIdx.x >> 7) * 64;
+  const f4* row = reinterpret_cast<const f4*>(Lp + r * kInvLd + c0);
   T a[2][NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) a[0][q] = a[1][q] = T(0);
-#pragma unroll 16
-  for (int c = 0; c < 64; ++c) {
-    const int cc = c0 + c;
-    const T l = TRANS ? Lp[cc * (NB + 1) + r] : Lp[r * (NB + 1) + cc];
+#pragma unroll 4
+  for (int c4 = 0; c4 < 16; ++c4) {
+    const f4 l = row[c4];
+    const int cc = c0 + 4 * c4;
 #pragma unroll
-    for (int q = 0; q < NR; ++q) a[c & 1][q] += l * t[cc][q];
+    for (int q = 0; q < NR; ++q) {
+      a[0][q] += l.x * t[cc][q];
+      a[1][q] += l.y * t[cc + 1][q];
+      a[0][q] += l.z * t[cc + 2][q];
+      a[1][q] += l.w * t[cc + 3][q];
+    }
   }
 #pragma unroll
   for (int q = 0; q < NR; ++q) sv[q] = a[0][q] + a[1][q];
@@ -374,8 +386,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk,
     const T* __restrict__ Li, const int* __restrict__ stop) {
   constexpr int NB = kSolveBlock, H = NB / 2;
-  constexpr int NPK = INV ? NB * (NB + 1) : NB * (NB - 1) / 2;
-  __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
+  constexpr int NPK = INV ? NB * kInvLd : NB * (NB - 1) / 2;
+  __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
   __shared__ T part[4][NB][NR];
   __shared__ T ztop[INV ? NB : H][NR];
   // INV: the tile L_{J,J-1} (block J-1's rows, this block's columns), the
@@ -392,7 +404,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     const int cA = j0 + lane, cB = j0 + H + lane;
     const bool okA = lane < jn, okB = H + lane < jn;
     if constexpr (INV) {
-      stage_inverse_block<T>(Lp, Li + (int64_t)J * NB * NB);
+      stage_inverse_block<T, false>(Lp, Li + (int64_t)J * NB * NB);
       if (J > 0) stage_last_tile<T, false>(Fl, F, lda, piv, (J - 1) * NB, NB, j0, jn);
       __syncthreads();  // Fl is read, by other threads, before the next barrier
     } else {
@@ -501,7 +513,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
       }
       __syncthreads();
       T sv[NR];
-      inverse_product<T, NR, false>(Lp, ztop, sv);
+      inverse_product<T, NR>(Lp, ztop, sv);
       const int r = threadIdx.x & (NB - 1);
       if (threadIdx.x >= NB) {
 #pragma unroll
@@ -596,8 +608,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk,
     const T* __restrict__ Li, const int* __restrict__ stop) {
   constexpr int NB = kSolveBlock, H = NB / 2;
-  constexpr int NPK = INV ? NB * (NB + 1) : NB * (NB - 1) / 2;
-  __shared__ T Lp[NPK];  // packed L, or Li with row stride NB + 1
+  constexpr int NPK = INV ? NB * kInvLd : NB * (NB - 1) / 2;
+  __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
   __shared__ T part[4][NB][NR];
   __shared__ T vbot[INV ? NB : H][NR];
   // INV: the tile L_{I+1,I}^T (rows of block I, columns of block I+1; from
@@ -612,7 +624,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     const int rA = i0 + lane, rB = i0 + H + lane;
     const bool okA = lane < in, okB = H + lane < in;
     if constexpr (INV) {
-      stage_inverse_block<T>(Lp, Li + (int64_t)I * NB * NB);
+      stage_inverse_block<T, true>(Lp, Li + (int64_t)I * NB * NB);
       if (I + 1 < nblk)
         stage_last_tile<T, true>(Fl, F, lda, piv, (I + 1) * NB, min(NB, n - (I + 1) * NB), i0,
                                  in);
@@ -745,7 +757,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
       }
       __syncthreads();
       T sv[NR];
-      inverse_product<T, NR, true>(Lp, vbot, sv);
+      inverse_product<T, NR>(Lp, vbot, sv);
       const int r = threadIdx.x & (NB - 1);
       if (threadIdx.x >= NB) {
 #pragma unroll
