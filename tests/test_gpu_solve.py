@@ -139,3 +139,19 @@ def test_mixed_many_2x2_ragged(oracle, handle, n):
     info = handle.sysv_mixed(Ad, torch.from_numpy(b).cuda(), X, tol=1e-12, max_iter=30)
     assert info["converged"] and info["iters"] <= 8, info
     assert berr_np(A, b, X.cpu().numpy()).max() <= 1.5e-12
+
+
+def test_mixed_six_rhs_two_chunks(oracle, handle):
+    """Six right-hand sides: the refinement's sweeps run in chunks of four
+    and two, all under one device-side stopping test."""
+    import torch
+    n = 700
+    A = eig_matrix(n, 1e5, 31)
+    B6 = np.random.default_rng(32).standard_normal((n, 6))
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()
+    X = torch.zeros((n, 6), dtype=torch.float64, device="cuda")
+    info = handle.sysv_mixed(Ad, torch.from_numpy(B6).cuda(), X, tol=1e-12)
+    assert info["converged"] and info["berr"] <= 1e-12
+    be = berr_np(A, B6, X.cpu().numpy())
+    assert be.max() <= 1.5e-12 and abs(be.max() - info["berr"]) <= 1e-13
