@@ -142,3 +142,20 @@ def test_reconstruct_matches_dense_product(oracle, handle, n):
     scale = max(1.0, np.abs(M).max())
     assert np.abs(out - R).max() <= 1e-13 * scale * max(1, n)
     assert np.abs(out - A).max() <= 1e-12 * scale * max(1, n)
+
+
+def test_fp32_factorization_residual_at_full_size(handle):
+    """The binary32 form at n = 16384: its factor, widened, reconstructs the
+    binary64 input to binary32 accuracy (checked on the device)."""
+    import torch
+    import blockedbunchkaufmanldlt_amd as B
+    n = 16384
+    lda = B.padded_ld(n)
+    A = handle.gen_synth(n, 41, 1.0, 0, lda)
+    T32 = A.to(torch.float32)
+    perm, piv = handle.sytrf(T32)
+    handle.synchronize()
+    F = T32.to(torch.float64)
+    del T32
+    r = handle.residual(A, F, perm, piv)
+    assert 1e-9 < r < 1e-4, r
