@@ -858,7 +858,6 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
   __shared__ double tile[TB][TB + 1];
   __shared__ double xs_i[TB * kSolveMaxRhs];
   __shared__ double xs_j[TB * kSolveMaxRhs];
-  __shared__ double red[4][TB * (kSolveMaxRhs + 1)];
   int idx = blockIdx.x;
   int J = static_cast<int>((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
   while (J * (J + 1) / 2 > idx) --J;
@@ -892,60 +891,50 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
     xs_j[r * kSolveMaxRhs + q] = (j0 + r < n) ? X[(int64_t)(j0 + r) * ldx + q] : 0.0;
   }
   __syncthreads();
-  // row sums: thread (r = t & 63, part = t >> 6) covers 16 columns
+  // Four lanes of one wavefront per row (row sums) and per column (column
+  // sums): lane part p = t % 4 covers 16 entries, the four partials are added
+  // by two DPP quad exchanges in a fixed order.
+  const int g = t >> 2, p = t & 3;
+  auto quad_sum = [](double x) {
+    x += dpp_f64<kDppQuadXor1, 0xf>(x);
+    x += dpp_f64<kDppQuadXor2, 0xf>(x);
+    return x;
+  };
+  // row sums: row g, columns 16 p .. 16 p + 15
   {
-    const int r = t & 63, part = t >> 6;
-    double s[kSolveMaxRhs] = {0, 0, 0, 0};
+    double sq[kSolveMaxRhs] = {0, 0, 0, 0};
     double sa = 0.0;
-    for (int c = part * 16; c < part * 16 + 16; ++c) {
-      const double a = tile[r][c];
-      for (int q = 0; q < nr; ++q) s[q] += a * xs_j[c * kSolveMaxRhs + q];
+#pragma unroll 4
+    for (int c = 16 * p; c < 16 * p + 16; ++c) {
+      const double a = tile[g][c];
+      for (int q = 0; q < nr; ++q) sq[q] += a * xs_j[c * kSolveMaxRhs + q];
       sa += fabs(a);
     }
-    for (int q = 0; q < nr; ++q) red[part][r * (kSolveMaxRhs + 1) + q] = s[q];
-    red[part][r * (kSolveMaxRhs + 1) + kSolveMaxRhs] = sa;
-  }
-  __syncthreads();
-  if (t < TB && i0 + t < n) {
-    for (int q = 0; q < nr; ++q) {
-      double v = 0.0;
-      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + q];
-      prow[((int64_t)J * n + i0 + t) * nr + q] = v;
-    }
-    if (with_norm) {
-      double v = 0.0;
-      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + kSolveMaxRhs];
-      pabs_row[(int64_t)J * n + i0 + t] = v;
+    for (int q = 0; q < nr; ++q) sq[q] = quad_sum(sq[q]);
+    sa = quad_sum(sa);
+    if (p == 0 && i0 + g < n) {
+      for (int q = 0; q < nr; ++q) prow[((int64_t)J * n + i0 + g) * nr + q] = sq[q];
+      if (with_norm) pabs_row[(int64_t)J * n + i0 + g] = sa;
     }
   }
-  __syncthreads();
-  // column sums over the strictly upper part: thread (c = t & 63, part)
+  // column sums over the strictly upper part: column g, rows 16 p .. 16 p + 15
   {
-    const int c = t & 63, part = t >> 6;
-    const int jj = j0 + c;
-    double s[kSolveMaxRhs] = {0, 0, 0, 0};
+    const int jj = j0 + g;
+    double sq[kSolveMaxRhs] = {0, 0, 0, 0};
     double sa = 0.0;
-    for (int r = part * 16; r < part * 16 + 16; ++r) {
+#pragma unroll 4
+    for (int r = 16 * p; r < 16 * p + 16; ++r) {
       if (i0 + r < jj) {
-        const double a = tile[r][c];
-        for (int q = 0; q < nr; ++q) s[q] += a * xs_i[r * kSolveMaxRhs + q];
+        const double a = tile[r][g];
+        for (int q = 0; q < nr; ++q) sq[q] += a * xs_i[r * kSolveMaxRhs + q];
         sa += fabs(a);
       }
     }
-    for (int q = 0; q < nr; ++q) red[part][c * (kSolveMaxRhs + 1) + q] = s[q];
-    red[part][c * (kSolveMaxRhs + 1) + kSolveMaxRhs] = sa;
-  }
-  __syncthreads();
-  if (t < TB && j0 + t < n) {
-    for (int q = 0; q < nr; ++q) {
-      double v = 0.0;
-      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + q];
-      pcol[((int64_t)I * n + j0 + t) * nr + q] = v;
-    }
-    if (with_norm) {
-      double v = 0.0;
-      for (int p = 0; p < 4; ++p) v += red[p][t * (kSolveMaxRhs + 1) + kSolveMaxRhs];
-      pabs_col[(int64_t)I * n + j0 + t] = v;
+    for (int q = 0; q < nr; ++q) sq[q] = quad_sum(sq[q]);
+    sa = quad_sum(sa);
+    if (p == 0 && jj < n) {
+      for (int q = 0; q < nr; ++q) pcol[((int64_t)I * n + jj) * nr + q] = sq[q];
+      if (with_norm) pabs_col[(int64_t)I * n + jj] = sa;
     }
   }
 }
