@@ -63,6 +63,8 @@ def lib() -> ctypes.CDLL:
     L.bbk_dsyberr.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, vp]
     L.bbk_dresidual.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, ctypes.POINTER(f64)]
     L.bbk_dreconstruct.argtypes = [vp, i32, vp, i64, vp, vp, vp, i64]
+    L.bbk_dsytrf_to_lapack.argtypes = [vp, i32, vp, i64, vp, vp, vp]
+    L.bbk_dsytrf_from_lapack.argtypes = [vp, i32, vp, i64, vp, vp, vp]
     L.bbk_dsampled_residual.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, i32, i32,
                                         ctypes.POINTER(f64)]
     L.bbk_dsysv_mixed.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, f64,
@@ -228,6 +230,27 @@ class Handle:
                                               _ptr(piv), _ptr(out), out.stride(0)),
                        "reconstruct")
         return out
+
+    def to_lapack(self, F, perm, piv):
+        """Convert compact factors in place to LAPACK dsytrf('L') form;
+        returns ipiv (1-based, int32 on F's device)."""
+        n = F.shape[0]
+        ipiv = self._torch.empty(n, dtype=self._torch.int32, device=F.device)
+        with self.ordered():
+            self.check(lib().bbk_dsytrf_to_lapack(self.h, n, _ptr(F), F.stride(0), _ptr(perm),
+                                                  _ptr(piv), _ptr(ipiv)), "to_lapack")
+        return ipiv
+
+    def from_lapack(self, F, ipiv):
+        """Convert a LAPACK dsytrf('L') factor in place to this library's
+        form; returns (perm, piv)."""
+        n = F.shape[0]
+        perm = self._torch.empty(n, dtype=self._torch.int32, device=F.device)
+        piv = self._torch.empty(n, dtype=self._torch.int32, device=F.device)
+        with self.ordered():
+            self.check(lib().bbk_dsytrf_from_lapack(self.h, n, _ptr(F), F.stride(0), _ptr(ipiv),
+                                                    _ptr(perm), _ptr(piv)), "from_lapack")
+        return perm, piv
 
     def residual(self, A, F, perm, piv) -> float:
         """||P L D L^T P^T - A||_F / max(||A||_F, 1) on the device (A: the

@@ -277,6 +277,25 @@ int bbk_dsampled_residual(bbk_handle_t handle, int n, const double* d_A, int64_t
  * `count` entries; host only, no device work. */
 int bbk_residual_samples(int n, int seed, int count, int* rows, int* cols);
 
+/* ---- LAPACK-convention interop ---------------------------------------
+ * The compact factor in place, between this library's form (every
+ * interchange applied to the earlier L columns too, as the reference does,
+ * src/block_ldlt_decomposition.cc:392-587) and LAPACK dsytrf('L')'s form
+ * (A = P(1) L(1) ... P(s) L(s) D (...)^T with 1-based ipiv: ipiv(k) > 0 a
+ * 1x1 pivot with rows/columns k, ipiv(k) interchanged; ipiv(k) = ipiv(k+1)
+ * < 0 a 2x2 pivot with k+1, -ipiv(k) interchanged).  Same bytes: row-major
+ * upper = column-major lower.  d_ipiv: n ints on the device.  Both return
+ * when the conversion is done.
+ *
+ * bbk_dsytrf_to_lapack: after bbk_dsytrf, F and (perm, piv) -> LAPACK's
+ * factor and ipiv (LAPACK dsytrs then solves with it).
+ * bbk_dsytrf_from_lapack: LAPACK's factor and ipiv -> this library's F,
+ * perm and piv (bbk_dsytrs then solves with them). */
+int bbk_dsytrf_to_lapack(bbk_handle_t handle, int n, double* d_F, int64_t ldf,
+                         const int* d_perm, const int* d_piv, int* d_ipiv);
+int bbk_dsytrf_from_lapack(bbk_handle_t handle, int n, double* d_F, int64_t ldf,
+                           const int* d_ipiv, int* d_perm, int* d_piv);
+
 /* ---- synthetic inputs (bench/test plumbing) ---------------------------
  * A(i,j) = A(j,i) = scale * u(seed,i,j), u uniform in [-1,1) from
  * SplitMix64 keyed by (seed, i, j); A(i,i) = 0 when zero_diag_stride > 0
