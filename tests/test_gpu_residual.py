@@ -159,3 +159,21 @@ def test_fp32_factorization_residual_at_full_size(handle):
     del T32
     r = handle.residual(A, F, perm, piv)
     assert 1e-9 < r < 1e-4, r
+
+
+def test_factor_and_residual_at_double_size(handle):
+    """n = 32768 (8.6 GB in binary64, 128 panel workgroups): the device
+    reconstruction residual of the factor, and a valid permutation."""
+    import torch
+    import blockedbunchkaufmanldlt_amd as B
+    n = 32768
+    lda = B.padded_ld(n)
+    A = handle.gen_synth(n, 77, 1.0, 7, lda)
+    T = A.clone()
+    perm, piv = handle.sytrf(T)
+    r = handle.residual(A, T, perm, piv)
+    handle.synchronize()
+    assert r < 1e-11, r
+    assert torch.equal(torch.sort(perm.long())[0], torch.arange(n, device=perm.device))
+    del A, T
+    torch.cuda.empty_cache()
