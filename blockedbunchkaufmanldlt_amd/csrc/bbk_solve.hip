@@ -5,8 +5,9 @@
 // L is stored transposed in the upper triangle (L(j,i) = F(i,j), i < j,
 // except F(i,i+1) of a 2x2 pivot, which is D).  Both triangular sweeps
 // run as one cooperative launch each: 128-row blocks, workgroups take
-// blocks in dependency order and publish each finished half block as
-// tagged records.
+// blocks in dependency order and publish finished unknowns as tagged
+// records (each half block in the substitution form, the whole block in
+// the refinement's inverse form).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -851,7 +852,7 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
     int n, int nr, const double* __restrict__ A, int64_t lda,
     const double* __restrict__ X, int64_t ldx, double* __restrict__ prow,
     double* __restrict__ pcol, double* __restrict__ pabs_row,
-    double* __restrict__ pabs_col, int ntile, int with_norm, const int* __restrict__ stop) {
+    double* __restrict__ pabs_col, int tile_rows, int with_norm, const int* __restrict__ stop) {
   constexpr int TB = kSymvTile;
   if (stop && *stop) return;
   // the tile is read from HBM once, coalesced, and both passes run from LDS
