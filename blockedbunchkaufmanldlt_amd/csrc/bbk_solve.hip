@@ -848,17 +848,64 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
 // with_norm.  Reduced in a fixed order by k_symv_reduce.
 constexpr int kSymvTile = 64;
 
+// One butterfly step of a wavefront reduce-scatter: lanes l and l ^ D
+// exchange the halves of v[0 .. 2H) they do not keep; lane l keeps the upper
+// half when bit D of l is set.  Sums are formed in a fixed order.
+template <int H, int D>
+__device__ __forceinline__ void reduce_scatter_step(double* v, int lane) {
+  const bool hi = (lane & D) != 0;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const double send = hi ? v[k] : v[k + H];
+    const double keep = hi ? v[k + H] : v[k];
+    v[k] = keep + __shfl_xor(send, D);
+  }
+}
+
+// Sum over the 64 lanes of f(u) for the 16 values u = 0 .. 15 of every
+// lane: lane l returns the sum for u = (l >> 2) & 15 (all four lanes of a
+// quad agree).  The first step forms f on the fly, so only eight partial
+// sums are live.
+template <typename F>
+__device__ __forceinline__ double wave_sum16_scatter(F f, int lane) {
+  double v[8];
+  {
+    const bool hi = (lane & 32) != 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double a = f(k), b = f(k + 8);
+      const double send = hi ? a : b;
+      const double keep = hi ? b : a;
+      v[k] = keep + __shfl_xor(send, 32);
+    }
+  }
+  reduce_scatter_step<4, 16>(v, lane);
+  reduce_scatter_step<2, 8>(v, lane);
+  reduce_scatter_step<1, 4>(v, lane);
+  double x = v[0];
+  x += dpp_f64<kDppQuadXor1, 0xf>(x);
+  x += dpp_f64<kDppQuadXor2, 0xf>(x);
+  return x;
+}
+
+// Wavefront w holds rows w + 4 u (u = 0 .. 15) of the tile, lane c column c,
+// in registers: column sums are per-lane sums combined over the four
+// wavefronts through LDS; row sums are a reduce-scatter across the lanes.
+// The tile never passes through LDS, so six workgroups fit on a CU.  NR:
+// right-hand sides (1 .. kSolveMaxRhs), a template argument so that every
+// load is straight-line code and all of them are in flight together.
+template <int NR>
 __global__ __launch_bounds__(256) void k_symv_tiles(
-    int n, int nr, const double* __restrict__ A, int64_t lda,
+    int n, const double* __restrict__ A, int64_t lda,
     const double* __restrict__ X, int64_t ldx, double* __restrict__ prow,
     double* __restrict__ pcol, double* __restrict__ pabs_row,
-    double* __restrict__ pabs_col, int tile_rows, int with_norm, const int* __restrict__ stop) {
+    double* __restrict__ pabs_col, int with_norm, const int* __restrict__ stop) {
   constexpr int TB = kSymvTile;
+  constexpr int PER = TB / 4;
+  static_assert(PER == 16, "wave_sum16_scatter reduces 16 rows per wavefront");
   if (stop && *stop) return;
-  // the tile is read from HBM once, coalesced, and both passes run from LDS
-  __shared__ double tile[TB][TB + 1];
-  __shared__ double xs_i[TB * kSolveMaxRhs];
-  __shared__ double xs_j[TB * kSolveMaxRhs];
+  __shared__ double cpart[4][NR + 1][TB];
+  __shared__ double xs_i[NR][TB];
   int idx = blockIdx.x;
   int J = static_cast<int>((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
   while (J * (J + 1) / 2 > idx) --J;
@@ -866,77 +913,69 @@ __global__ __launch_bounds__(256) void k_symv_tiles(
   const int I = idx - J * (J + 1) / 2;
   const int i0 = I * TB, j0 = J * TB;
   const int t = threadIdx.x;
-  // all 16 loads of a thread in flight together: unconditional loads from
-  // clamped addresses, masked afterwards
-  {
-    static_assert(TB * TB % 256 == 0, "tile elements per thread");
-    constexpr int PER = TB * TB / 256;
-    double v[PER];
+  const int lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int jj = j0 + lane;
+  const int jc = min(jj, n - 1);
+  // every load issued before the first wait: clamped addresses, masked after
+  double xj[NR];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = t + 256 * u;
-      const int i = min(i0 + e / TB, n - 1), jj = min(j0 + e % TB, n - 1);
-      v[u] = A[(int64_t)i * lda + jj];
-    }
+  for (int q = 0; q < NR; ++q) xj[q] = X[(int64_t)jc * ldx + q];
+  double xi = 0.0;
+  const int xr = t / NR, xq = t % NR;
+  if (t < TB * NR) xi = X[(int64_t)min(i0 + xr, n - 1) * ldx + xq];
+  double v[PER];
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = t + 256 * u;
-      const int r = e / TB, c = e % TB;
-      const int i = i0 + r, jj = j0 + c;
-      tile[r][c] = (i < n && jj < n && jj >= i) ? v[u] : 0.0;
-    }
+  for (int u = 0; u < PER; ++u) {
+    const int i = min(i0 + w + 4 * u, n - 1);
+    v[u] = A[(int64_t)i * lda + jc];
   }
-  for (int e = t; e < TB * nr; e += 256) {
-    const int r = e / nr, q = e % nr;
-    xs_i[r * kSolveMaxRhs + q] = (i0 + r < n) ? X[(int64_t)(i0 + r) * ldx + q] : 0.0;
-    xs_j[r * kSolveMaxRhs + q] = (j0 + r < n) ? X[(int64_t)(j0 + r) * ldx + q] : 0.0;
+  if (t < TB * NR) xs_i[xq][xr] = i0 + xr < n ? xi : 0.0;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) xj[q] = jj < n ? xj[q] : 0.0;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int i = i0 + w + 4 * u;
+    if (!(i < n && jj < n && jj >= i)) v[u] = 0.0;
+  }
+  __syncthreads();  // xs_i
+  // column sums over the strictly upper part
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) s += i0 + w + 4 * u < jj ? v[u] * xs_i[q][w + 4 * u] : 0.0;
+    cpart[w][q][lane] = s;
+  }
+  if (with_norm) {
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) s += i0 + w + 4 * u < jj ? fabs(v[u]) : 0.0;
+    cpart[w][NR][lane] = s;
+  }
+  // row sums (diagonal included)
+  const int irow = i0 + w + 4 * ((lane >> 2) & 15);
+  const bool wr = (lane & 3) == 0 && irow < n;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const double x = xj[q];
+    const double s = wave_sum16_scatter([&](int u) { return v[u] * x; }, lane);
+    if (wr) prow[((int64_t)J * n + irow) * NR + q] = s;
+  }
+  if (with_norm) {
+    const double s = wave_sum16_scatter([&](int u) { return fabs(v[u]); }, lane);
+    if (wr) pabs_row[(int64_t)J * n + irow] = s;
   }
   __syncthreads();
-  // Four lanes of one wavefront per row (row sums) and per column (column
-  // sums): lane part p = t % 4 covers 16 entries, the four partials are added
-  // by two DPP quad exchanges in a fixed order.
-  const int g = t >> 2, p = t & 3;
-  auto quad_sum = [](double x) {
-    x += dpp_f64<kDppQuadXor1, 0xf>(x);
-    x += dpp_f64<kDppQuadXor2, 0xf>(x);
-    return x;
-  };
-  // row sums: row g, columns 16 p .. 16 p + 15
-  {
-    double sq[kSolveMaxRhs] = {0, 0, 0, 0};
-    double sa = 0.0;
-#pragma unroll 4
-    for (int c = 16 * p; c < 16 * p + 16; ++c) {
-      const double a = tile[g][c];
-      for (int q = 0; q < nr; ++q) sq[q] += a * xs_j[c * kSolveMaxRhs + q];
-      sa += fabs(a);
-    }
-    for (int q = 0; q < nr; ++q) sq[q] = quad_sum(sq[q]);
-    sa = quad_sum(sa);
-    if (p == 0 && i0 + g < n) {
-      for (int q = 0; q < nr; ++q) prow[((int64_t)J * n + i0 + g) * nr + q] = sq[q];
-      if (with_norm) pabs_row[(int64_t)J * n + i0 + g] = sa;
-    }
-  }
-  // column sums over the strictly upper part: column g, rows 16 p .. 16 p + 15
-  {
-    const int jj = j0 + g;
-    double sq[kSolveMaxRhs] = {0, 0, 0, 0};
-    double sa = 0.0;
-#pragma unroll 4
-    for (int r = 16 * p; r < 16 * p + 16; ++r) {
-      if (i0 + r < jj) {
-        const double a = tile[r][g];
-        for (int q = 0; q < nr; ++q) sq[q] += a * xs_i[r * kSolveMaxRhs + q];
-        sa += fabs(a);
-      }
-    }
-    for (int q = 0; q < nr; ++q) sq[q] = quad_sum(sq[q]);
-    sa = quad_sum(sa);
-    if (p == 0 && jj < n) {
-      for (int q = 0; q < nr; ++q) pcol[((int64_t)I * n + jj) * nr + q] = sq[q];
-      if (with_norm) pabs_col[(int64_t)I * n + jj] = sa;
-    }
+  if (w != 0 || jj >= n) return;
+#pragma unroll
+  for (int q = 0; q <= NR; ++q) {
+    if (q == NR && !with_norm) break;
+    const double c = ((cpart[0][q][lane] + cpart[1][q][lane]) + cpart[2][q][lane]) + cpart[3][q][lane];
+    if (q < NR)
+      pcol[((int64_t)I * n + jj) * NR + q] = c;
+    else
+      pabs_col[(int64_t)I * n + jj] = c;
   }
 }
 
@@ -1212,8 +1251,12 @@ int berr(Handle* h, int n, int nrhs, const double* A, int64_t lda,
     const int nr = std::min(kSolveMaxRhs, nrhs - q0);
     hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(32), 0, h->stream, norms, 16, 0.0);
     const int tiles = T * (T + 1) / 2;
-    hipLaunchKernelGGL(k_symv_tiles, dim3(tiles), dim3(256), 0, h->stream, n, nr, A, lda,
-                       X + q0, ldx, prow, pcol, parow, pacol, T, with_norm, stop);
+    switch (nr) {
+      case 1: hipLaunchKernelGGL(k_symv_tiles<1>, dim3(tiles), dim3(256), 0, h->stream, n, A, lda, X + q0, ldx, prow, pcol, parow, pacol, with_norm, stop); break;
+      case 2: hipLaunchKernelGGL(k_symv_tiles<2>, dim3(tiles), dim3(256), 0, h->stream, n, A, lda, X + q0, ldx, prow, pcol, parow, pacol, with_norm, stop); break;
+      case 3: hipLaunchKernelGGL(k_symv_tiles<3>, dim3(tiles), dim3(256), 0, h->stream, n, A, lda, X + q0, ldx, prow, pcol, parow, pacol, with_norm, stop); break;
+      default: hipLaunchKernelGGL(k_symv_tiles<4>, dim3(tiles), dim3(256), 0, h->stream, n, A, lda, X + q0, ldx, prow, pcol, parow, pacol, with_norm, stop); break;
+    }
     hipLaunchKernelGGL(k_symv_reduce, dim3(blocks_for(n, 64)), dim3(256), 0, h->stream, n, nr,
                        prow, pcol, parow, pacol, B + q0, ldb, X + q0, ldx,
                        R ? R + q0 : nullptr, ldr, norms, with_norm, stop);
