@@ -1080,12 +1080,50 @@ __global__ void k_fill_i32(int* p, int count, int v) {
   if (i < count) p[i] = v;
 }
 
-// Upper-triangle cast binary64 -> binary32 (row-major, leading dims).
-__global__ void k_cast_upper(int n, const double* __restrict__ A, int64_t lda,
-                             float* __restrict__ S, int64_t lds) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (i < n && j < n && j >= i) S[(int64_t)i * lds + j] = (float)A[(int64_t)i * lda + j];
+// Upper triangle (j >= i) of a row-major n x n matrix, converted S -> D
+// (a copy when S = D): one workgroup per row, four consecutive elements per
+// thread and step, as 16- or 32-byte vectors when VEC (leading dimensions
+// multiples of 4, rows aligned to four elements).  The lower triangle of
+// dst is not written.
+template <typename S, typename D, bool VEC>
+__global__ __launch_bounds__(256) void k_copy_upper(int n, const S* __restrict__ src, int64_t lds,
+                                                    D* __restrict__ dst, int64_t ldd) {
+  const int i = blockIdx.x;
+  const S* sr = src + (int64_t)i * lds;
+  D* dr = dst + (int64_t)i * ldd;
+  if constexpr (VEC) {
+    typedef S s4 __attribute__((ext_vector_type(4)));
+    typedef D d4 __attribute__((ext_vector_type(4)));
+    for (int j = (i & ~3) + 4 * (int)threadIdx.x; j < n; j += 4 * 256) {
+      if (j >= i && j + 4 <= n) {
+        const s4 v = *reinterpret_cast<const s4*>(sr + j);
+        d4 o;
+        o.x = static_cast<D>(v.x);
+        o.y = static_cast<D>(v.y);
+        o.z = static_cast<D>(v.z);
+        o.w = static_cast<D>(v.w);
+        *reinterpret_cast<d4*>(dr + j) = o;
+      } else {
+        for (int t = 0; t < 4; ++t)
+          if (j + t >= i && j + t < n) dr[j + t] = static_cast<D>(sr[j + t]);
+      }
+    }
+  } else {
+    for (int j = i + (int)threadIdx.x; j < n; j += 256) dr[j] = static_cast<D>(sr[j]);
+  }
+}
+
+template <typename S, typename D>
+static void copy_upper(Handle* h, int n, const S* src, int64_t lds, D* dst, int64_t ldd) {
+  const bool vec = lds % 4 == 0 && ldd % 4 == 0 &&
+                   reinterpret_cast<uintptr_t>(src) % (4 * sizeof(S)) == 0 &&
+                   reinterpret_cast<uintptr_t>(dst) % (4 * sizeof(D)) == 0;
+  if (vec)
+    hipLaunchKernelGGL((k_copy_upper<S, D, true>), dim3(n), dim3(256), 0, h->stream, n, src, lds,
+                       dst, ldd);
+  else
+    hipLaunchKernelGGL((k_copy_upper<S, D, false>), dim3(n), dim3(256), 0, h->stream, n, src, lds,
+                       dst, ldd);
 }
 
 template <typename S, typename D>
@@ -1345,8 +1383,9 @@ int bbk_dsysv(bbk_handle_t handle, int n, int nrhs, const double* d_A, int64_t l
   int* piv = perm + n;
   double* eta = reinterpret_cast<double*>(base + (size_t)ldf * n * sizeof(double) +
                                           (2 * (size_t)n * sizeof(int) + 255) / 256 * 256);
-  hipError_t e = hipMemcpy2DAsync(F, ldf * sizeof(double), d_A, lda * sizeof(double),
-                                  n * sizeof(double), n, hipMemcpyDeviceToDevice, h->stream);
+  // the factorization reads and writes the upper triangle only
+  bbk::copy_upper<double, double>(h, n, d_A, lda, F, ldf);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return h->hip_fail(e, "copy A");
   rc = bbk::factorize<double>(h, n, 0, F, ldf, perm, piv);
   if (rc != BBK_OK) return rc;
@@ -1395,8 +1434,7 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   int* ctl = reinterpret_cast<int*>(best_dev + 1);
   float* Li = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes + dbytes + ebytes);
   // FP32 copy of the upper triangle, FP32 factorization
-  dim3 cg(bbk::blocks_for(n, 256), n);
-  hipLaunchKernelGGL(bbk::k_cast_upper, cg, dim3(256), 0, h->stream, n, d_A, lda, S, ldf);
+  bbk::copy_upper<double, float>(h, n, d_A, lda, S, ldf);
   rc = bbk::factorize<float>(h, n, 0, S, ldf, perm, piv);
   if (rc != BBK_OK) return rc;
   // inverses of the diagonal blocks of L: the refinement's solves multiply

@@ -155,3 +155,35 @@ def test_mixed_six_rhs_two_chunks(oracle, handle):
     assert info["converged"] and info["berr"] <= 1e-12
     be = berr_np(A, B6, X.cpu().numpy())
     assert be.max() <= 1.5e-12 and abs(be.max() - info["berr"]) <= 1e-13
+
+
+def test_mixed_refines_like_lapack_ssytrf(oracle, handle):
+    """The binary32 factor is as accurate as LAPACK's: classic refinement
+    with it needs no more corrections than the same refinement driven by
+    scipy's ssytrf/ssytrs factor of the same matrix (condition 1e6, where
+    every correction gains only about one decimal digit, so an inaccurate
+    factor shows up as extra corrections)."""
+    import torch
+    from scipy.linalg import lapack
+    n = 3000
+    A = eig_matrix(n, 1e6, 41)
+    b = np.random.default_rng(42).standard_normal(n)
+    lu, ipiv, info = lapack.ssytrf(A.astype(np.float32), lower=1)
+    assert info == 0
+
+    def minv(r):
+        return lapack.ssytrs(lu, ipiv, r.astype(np.float32), lower=1)[0].astype(np.float64)
+
+    x = minv(b)
+    lapack_iters = 0
+    while berr_np(A, b[:, None], x[:, None]).max() > 1e-12 and lapack_iters < 30:
+        x = x + minv(b - A @ x)
+        lapack_iters += 1
+    assert 2 <= lapack_iters < 30
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()
+    X = torch.zeros((n, 1), dtype=torch.float64, device="cuda")
+    res = handle.sysv_mixed(Ad, torch.from_numpy(b[:, None].copy()).cuda(), X, tol=1e-12,
+                            max_iter=30)
+    assert res["converged"]
+    assert res["iters"] <= lapack_iters, (res["iters"], lapack_iters)
