@@ -187,3 +187,37 @@ def test_mixed_refines_like_lapack_ssytrf(oracle, handle):
                             max_iter=30)
     assert res["converged"]
     assert res["iters"] <= lapack_iters, (res["iters"], lapack_iters)
+
+
+@pytest.mark.parametrize("n,nrhs,lda", [(1, 1, 1), (5, 3, 5), (63, 2, 67), (65, 4, 65),
+                                        (130, 1, 131)])
+def test_dsyberr_small_and_ragged(oracle, handle, n, nrhs, lda):
+    """Backward error on orders below, at and just past one 64 x 64 residual
+    tile, with unpadded leading dimensions; right-hand sides 1 to 4 (every
+    instantiation of the tile kernel)."""
+    import torch
+    A = oracle.gen_synth_symmetric(n, 300 + n)
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, nrhs))
+    X = rng.standard_normal((n, nrhs))
+    Ad = to_device(A, lda=lda)
+    eta = handle.syberr(Ad, torch.from_numpy(B).cuda(), torch.from_numpy(X).cuda())
+    handle.synchronize()
+    assert np.allclose(eta.cpu().numpy(), berr_np(A, B, X), rtol=1e-12)
+
+
+@pytest.mark.parametrize("n,lda", [(77, 77), (200, 203), (256, 258)])
+def test_solves_with_odd_leading_dimension(oracle, handle, n, lda):
+    """Leading dimensions that are not multiples of four: the solves copy
+    the input's upper triangle element by element instead of as vectors."""
+    import torch
+    A = eig_matrix(n, 1e4, 50 + n)
+    b = np.random.default_rng(51).standard_normal((n, 1))
+    Ad = to_device(A, lda=lda)
+    bd = torch.from_numpy(b).cuda()
+    X = torch.zeros_like(bd)
+    be64 = handle.sysv(Ad, bd, X)
+    assert be64 < 1e-14 and berr_np(A, b, X.cpu().numpy()).max() < 1e-14
+    X.zero_()
+    info = handle.sysv_mixed(Ad, bd, X, tol=1e-12, max_iter=30)
+    assert info["converged"] and berr_np(A, b, X.cpu().numpy()).max() <= 1.5e-12
