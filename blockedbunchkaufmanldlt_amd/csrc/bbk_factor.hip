@@ -94,7 +94,8 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
 // factorization): binary64 64 x 64 tiles with eight workgroups resident per
 // CU, so one workgroup's C traffic overlaps another's MFMAs (42.6 ms for all
 // updates; C added after the K loop at five per CU 47.5 ms, operands two
-// chunks ahead at seven per CU 44.8 ms); binary32 128 x 128 tiles, five per
+// chunks ahead at seven per CU 44.8 ms, K chunks of 12 at six per CU
+// 45.5 ms, K chunks of 16 at four per CU 51.0 ms); binary32 128 x 128 tiles, five per
 // CU by registers (27.3 ms against 27.8 ms for 64 x 64 and 28.4 ms with K
 // chunks of 16).  K chunks of 8; XCD-contiguous supertile order.
 template <typename T>
