@@ -525,8 +525,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
           const T v = sv[q] + part[0][r][q];
-          Z[(int64_t)(j0 + r) * ldz + q] = v;
           xput(rec + (int64_t)(j0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(v));
+          Z[(int64_t)(j0 + r) * ldz + q] = v;
         }
       }
       __syncthreads();  // Lp, part, ztop are reused by the next block
@@ -624,6 +624,15 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     const int in = min(NB, n - i0);
     const int rA = i0 + lane, rB = i0 + H + lane;
     const bool okA = lane < in, okB = H + lane < in;
+    // permutation entries of the rows this thread publishes, read now so
+    // that no global load sits between the block's solution and its records
+    int pub_perm = 0;
+    if constexpr (INV) {
+      if (threadIdx.x < in) pub_perm = perm[i0 + threadIdx.x];
+    } else {
+      if (w == 0 && okA) pub_perm = perm[rA];
+      if (w == 1 && okB) pub_perm = perm[rB];
+    }
     if constexpr (INV) {
       stage_inverse_block<T, true>(Lp, Li + (int64_t)I * NB * NB);
       if (I + 1 < nblk)
@@ -766,12 +775,11 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
       }
       __syncthreads();
       if (threadIdx.x < NB && r < in) {
-        const int pi = perm[i0 + r];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
           const T v = sv[q] + part[0][r][q];
-          X[(int64_t)pi * ldx + q] = v;
           xput(rec + (int64_t)(i0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(v));
+          X[(int64_t)pub_perm * ldx + q] = v;
         }
       }
       __syncthreads();  // Lp, part, vbot are reused by the next block
@@ -791,11 +799,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
 #pragma unroll
       for (int q = 0; q < NR; ++q) vbot[lane][q] = okB ? v[q] : T(0);
       if (okB) {
-        const int pi = perm[rB];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-          X[(int64_t)pi * ldx + q] = v[q];
           xput(rec + (int64_t)rB * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+          X[(int64_t)pub_perm * ldx + q] = v[q];
         }
       }
     }
@@ -829,11 +836,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
         return min(c * (c - 1) / 2 + lane, NPK - 1);
       });
       if (okA) {
-        const int pi = perm[rA];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-          X[(int64_t)pi * ldx + q] = v[q];
           xput(rec + (int64_t)rA * kSolveMaxRhs + q, tag, q, static_cast<double>(v[q]));
+          X[(int64_t)pub_perm * ldx + q] = v[q];
         }
       }
     }
