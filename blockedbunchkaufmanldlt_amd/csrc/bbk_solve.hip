@@ -218,6 +218,97 @@ __global__ __launch_bounds__(kSolveBlock) void k_diag_inv(int n, const T* __rest
   for (int i = 0; i < NB; ++i) out[(int64_t)i * NB + c] = (i < bn && c < bn) ? Xs[i][c] : T(0);
 }
 
+// G_J = Li_J L_{J,J-1} for J >= 1 (binary32 refinement solves): the
+// forward sweep forms u_J = Li_J (b_J - sum_{I < J-1} L_JI z_I) before block
+// J-1 is solved and then z_J = u_J - G_J z_{J-1}, one product after block
+// J-1's records arrive.  L_{J,J-1}(r, c) = F(j0 - 128 + c, j0 + r), zero for
+// rows past n and for the D entry of a 2x2 pivot across the block edge,
+// F(j0 - 1, j0).  Stored like Li: G_J row-major at G + J 128^2 (G_0 unused).
+// Thread c forms column c of G_J; Li_J and L_{J,J-1} staged in LDS.
+template <typename T>
+__global__ __launch_bounds__(kSolveBlock) void k_next_block_fwd(int n, const T* __restrict__ F,
+                                                                int64_t lda,
+                                                                const int* __restrict__ piv,
+                                                                const T* __restrict__ Li,
+                                                                T* __restrict__ G) {
+  constexpr int NB = kSolveBlock;
+  __shared__ T Ls[NB][NB + 1];  // Li_J(i, k)
+  __shared__ T Lt[NB][NB + 1];  // L_{J,J-1}(r, c)
+  const int J = blockIdx.x + 1;
+  const int j0 = J * NB, jn = min(NB, n - j0), p0 = j0 - NB;
+  const int c = threadIdx.x;
+  const T* Lj = Li + (int64_t)J * NB * NB;
+  const bool edge = piv[j0 - 1] == 2;
+#pragma unroll 8
+  for (int k = 0; k < NB; ++k) {
+    Ls[k][c] = Lj[(int64_t)k * NB + c];
+    // row p0 + k of F, column j0 + c: L(j0 + c, p0 + k) = L_{J,J-1}(c, k)
+    T x = c < jn ? F[(int64_t)(p0 + k) * lda + j0 + c] : T(0);
+    if (k == NB - 1 && c == 0 && edge) x = T(0);
+    Lt[c][k] = x;
+  }
+  __syncthreads();
+  T* out = G + (int64_t)J * NB * NB;
+  for (int i = 0; i < NB; ++i) {
+    // Li_J is unit lower triangular: k <= i
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+    int k = 0;
+    for (; k + 4 <= i + 1; k += 4) {
+      a0 += Ls[i][k] * Lt[k][c];
+      a1 += Ls[i][k + 1] * Lt[k + 1][c];
+      a2 += Ls[i][k + 2] * Lt[k + 2][c];
+      a3 += Ls[i][k + 3] * Lt[k + 3][c];
+    }
+    for (; k <= i; ++k) a0 += Ls[i][k] * Lt[k][c];
+    out[(int64_t)i * NB + c] = (a0 + a1) + (a2 + a3);
+  }
+}
+
+// H_I = Li_I^T L_{I+1,I}^T for I < nblk - 1 (binary32 refinement solves):
+// the backward sweep forms u_I = Li_I^T (D^-1 z_I - sum_{J > I+1} L_JI^T v_J)
+// before block I+1 is solved and then v_I = u_I - H_I v_{I+1}.
+// L_{I+1,I}(c, k) = F(i0 + k, i0 + 128 + c), zero for rows past n and for
+// the D entry of a 2x2 pivot across the block edge, F(i0 + 127, i0 + 128).
+// Stored like Li: H_I row-major at H + I 128^2.  Thread c forms column c:
+// H_I(r, c) = sum_{k >= r} Li_I(k, r) L_{I+1,I}(c, k).
+template <typename T>
+__global__ __launch_bounds__(kSolveBlock) void k_next_block_bwd(int n, const T* __restrict__ F,
+                                                                int64_t lda,
+                                                                const int* __restrict__ piv,
+                                                                const T* __restrict__ Li,
+                                                                T* __restrict__ Hb) {
+  constexpr int NB = kSolveBlock;
+  __shared__ T Ls[NB][NB + 1];  // Li_I(k, r)
+  __shared__ T Lb[NB][NB + 1];  // Lb[k][c] = L_{I+1,I}(c, k)
+  const int I = blockIdx.x;
+  const int i0 = I * NB, i1 = i0 + NB, cn = min(NB, n - i1);
+  const int c = threadIdx.x;
+  const T* Lj = Li + (int64_t)I * NB * NB;
+  const bool edge = piv[i0 + NB - 1] == 2;
+#pragma unroll 8
+  for (int k = 0; k < NB; ++k) {
+    Ls[k][c] = Lj[(int64_t)k * NB + c];
+    T x = c < cn ? F[(int64_t)(i0 + k) * lda + i1 + c] : T(0);
+    if (k == NB - 1 && c == 0 && edge) x = T(0);
+    Lb[k][c] = x;
+  }
+  __syncthreads();
+  T* out = Hb + (int64_t)I * NB * NB;
+  for (int r = 0; r < NB; ++r) {
+    // Li_I is unit lower triangular: k >= r
+    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+    int k = r;
+    for (; k + 4 <= NB; k += 4) {
+      a0 += Ls[k][r] * Lb[k][c];
+      a1 += Ls[k + 1][r] * Lb[k + 1][c];
+      a2 += Ls[k + 2][r] * Lb[k + 2][c];
+      a3 += Ls[k + 3][r] * Lb[k + 3][c];
+    }
+    for (; k < NB; ++k) a0 += Ls[k][r] * Lb[k][c];
+    out[(int64_t)r * NB + c] = (a0 + a1) + (a2 + a3);
+  }
+}
+
 // Strictly lower triangle of F := mirror of the strictly upper one (F(j, i)
 // = F(i, j), i < j), through 64 x 64 LDS tiles so that reads and writes
 // both run along rows.  Tile t of the upper triangle: (I, J), I <= J.
@@ -241,44 +332,6 @@ __global__ __launch_bounds__(256) void k_mirror_upper(int n, T* __restrict__ F, 
   for (int r = r0; r < TB; r += 4) {
     const int j = j0 + r, i = i0 + c;
     if (j < n && i < j) F[(int64_t)j * lda + i] = tile[c][r];
-  }
-}
-
-// One 128 x 128 factor tile into LDS (row-major, stride 128), read along
-// rows of F: Fl[a][b] = F(p0 + a, q0 + b) for a < pn and b < qn, else 0
-// (outside the matrix).  Forward sweep (MIRROR false): rows of block J-1,
-// columns of block J, i.e. L(q0 + b, p0 + a); the D entry of a 2x2 pivot
-// across the block edge, F(q0 - 1, q0) at (127, 0), reads as 0.  Backward
-// sweep (MIRROR true): F's mirrored lower triangle, rows of block I+1 and
-// columns of block I, i.e. L(p0 + a, q0 + b); the edge entry is (0, 127)
-// when the 2x2 pivot is (q0 + 127, q0 + 128).
-template <typename T, bool MIRROR>
-__device__ __forceinline__ void stage_last_tile(T* Fl, const T* __restrict__ F, int64_t lda,
-                                                const int* __restrict__ piv, int p0, int pn,
-                                                int q0, int qn) {
-  constexpr int NB = kSolveBlock;
-  constexpr int PER = NB * NB / kSolveThreads / 2;
-  const int b = threadIdx.x & (NB - 1), a0 = threadIdx.x >> 7;
-  const bool ok = b < qn;
-  bool edge;
-  if (MIRROR)
-    edge = qn == NB && b == NB - 1 && piv[q0 + NB - 1] == 2;  // Fl[0][127]
-  else
-    edge = b == 0 && piv[q0 - 1] == 2;                         // Fl[127][0]
-#pragma unroll
-  for (int hh = 0; hh < 2; ++hh) {
-    T v[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int a = a0 + 2 * (u + PER * hh);
-      v[u] = (ok && a < pn) ? F[(int64_t)(p0 + a) * lda + q0 + b] : T(0);
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int a = a0 + 2 * (u + PER * hh);
-      const bool zero = edge && (MIRROR ? a == 0 : a == NB - 1);
-      Fl[a * NB + b] = zero ? T(0) : v[u];
-    }
   }
 }
 
@@ -377,9 +430,11 @@ __device__ __forceinline__ void serial64(T (&v)[NR], const T* Lp, int lane, Idx 
 // applies them to the bottom 64 and solves those.
 //
 // INV: the diagonal block's inverse Li (k_diag_inv) replaces the serial
-// substitution: t = b_J - sum_I L_JI z_I, then z_J = Li t as a product
-// (thread (r, h) sums row r over columns 64h .. 64h+63), so no serial chain
-// remains on the path from one block to the next.
+// substitution, and G_J = Li_J L_{J,J-1} (k_next_block_fwd) the last
+// block's contribution: u_J = Li_J (b_J - sum_{I < J-1} L_JI z_I) is formed
+// before block J-1 is solved, then z_J = u_J - G_J z_{J-1}, so one product
+// (thread (r, h) sums row r over columns 64h .. 64h+63) separates block
+// J-1's records from block J's.
 template <typename T, int NR, bool INV>
 __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
@@ -391,11 +446,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
   __shared__ T part[4][NB][NR];
   __shared__ T ztop[INV ? NB : H][NR];
-  // INV: the tile L_{J,J-1} (block J-1's rows, this block's columns), the
-  // last contribution before the diagonal product, staged when the
-  // workgroup starts so that no HBM load sits between block J-1's records
-  // arriving and block J's product
-  __shared__ T Fl[INV ? NB * NB : 1];
+  // INV: G_J = Li_J L_{J,J-1} (k_next_block_fwd, stored after the nblk
+  // inverse blocks), staged when the workgroup starts: once block J-1's
+  // records arrive, z_J = u_J - G_J z_{J-1} is one product away
+  __shared__ __attribute__((aligned(16))) T Gs[INV ? NB * kInvLd : 4];
   if (stop && *stop) return;  // every workgroup reads the same flag
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
@@ -406,8 +460,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     const bool okA = lane < jn, okB = H + lane < jn;
     if constexpr (INV) {
       stage_inverse_block<T, false>(Lp, Li + (int64_t)J * NB * NB);
-      if (J > 0) stage_last_tile<T, false>(Fl, F, lda, piv, (J - 1) * NB, NB, j0, jn);
-      __syncthreads();  // Fl is read, by other threads, before the next barrier
+      if (J > 0) stage_inverse_block<T, false>(Gs, Li + (int64_t)(nblk + J) * NB * NB);
     } else {
       stage_diag_block<T>(Lp, F, lda, piv, j0, jn, n);
     }
@@ -474,27 +527,6 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
           }
         }
     }
-    if constexpr (INV) {
-      if (J > 0) {  // block J-1 from the staged tile
-        const int rb = (J - 1) * NB + 32 * w;
-        const int row0[2] = {rb, rb + 16};
-        double zv[2];
-        poll_rows_multi<NR, 2>(rec, row0, n, tag, zv);
-#pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-          for (int ii = 0; ii < 16; ++ii) {
-            const int r = 32 * w + 16 * k + ii;
-            const T a = Fl[r * NB + lane], b = Fl[r * NB + H + lane];
-#pragma unroll
-            for (int q = 0; q < NR; ++q) {
-              const T z = static_cast<T>(bcast(zv[k], ii * 4 + q));
-              acc[0][q] += a * z;
-              acc[1][q] += b * z;
-            }
-          }
-      }
-    }
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       part[w][lane][q] = acc[0][q];
@@ -502,7 +534,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     }
     __syncthreads();
     if constexpr (INV) {
-      // t into ztop (rows 0..127), then z = Li t
+      // t' into ztop (rows 0..127), then u = Li t'
       if (w < 2) {
         const int r = w * H + lane;
         const bool ok = w == 0 ? okA : okB;
@@ -521,15 +553,40 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
         for (int q = 0; q < NR; ++q) part[0][r][q] = sv[q];
       }
       __syncthreads();
+      T u[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) u[q] = sv[q] + part[0][r][q];
+      if (J > 0) {
+        // block J-1's solution: wavefront w polls rows 32w .. 32w+31 into
+        // ztop (free again), then u -= G_J z_{J-1}
+        const int rb = (J - 1) * NB + 32 * w;
+        const int row0[2] = {rb, rb + 16};
+        double zv[2];
+        poll_rows_multi<NR, 2>(rec, row0, n, tag, zv);
+        if ((lane & 3) < NR) {
+#pragma unroll
+          for (int k = 0; k < 2; ++k)
+            ztop[32 * w + 16 * k + (lane >> 2)][lane & 3] = static_cast<T>(zv[k]);
+        }
+        __syncthreads();
+        T gv[NR];
+        inverse_product<T, NR>(Gs, ztop, gv);
+        if (threadIdx.x >= NB) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) part[1][r][q] = gv[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NR; ++q) u[q] -= gv[q] + part[1][r][q];
+      }
       if (threadIdx.x < NB && r < jn) {
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-          const T v = sv[q] + part[0][r][q];
-          xput(rec + (int64_t)(j0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(v));
-          Z[(int64_t)(j0 + r) * ldz + q] = v;
+          xput(rec + (int64_t)(j0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(u[q]));
+          Z[(int64_t)(j0 + r) * ldz + q] = u[q];
         }
       }
-      __syncthreads();  // Lp, part, ztop are reused by the next block
+      __syncthreads();  // Lp, Gs, part, ztop are reused by the next block
       continue;
     }
     if (w == 0) {
@@ -595,9 +652,11 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
 // bottom 64 unknowns are solved first (wavefront 1) and published, then
 // applied to the top 64, which wavefront 0 solves.
 //
-// INV (refinement solves): v_I = Li^T s with Li the diagonal block's inverse
-// (k_diag_inv), s = D^-1 z_I - sum_J L_JI^T v_J; thread (r, h) sums column
-// r of Li over rows 64h .. 64h+63.  The strictly lower triangle of F then
+// INV (refinement solves): u_I = Li_I^T s' with Li the diagonal block's
+// inverse (k_diag_inv), s' = D^-1 z_I - sum_{J > I+1} L_JI^T v_J, formed
+// before block I+1 is solved; then v_I = u_I - H_I v_{I+1} with H_I =
+// Li_I^T L_{I+1,I}^T (k_next_block_bwd).  Thread (r, h) sums row r of the
+// staged Li^T or H_I over columns 64h .. 64h+63.  The strictly lower triangle of F then
 // holds the mirror of the upper one (k_mirror_upper), and L_JI^T is read
 // from it: element F(i, j), i in I, as F[j lda + i], consecutive lanes on
 // consecutive i (coalesced, where the upper rows give every lane its own
@@ -613,9 +672,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
   __shared__ T part[4][NB][NR];
   __shared__ T vbot[INV ? NB : H][NR];
-  // INV: the tile L_{I+1,I}^T (rows of block I, columns of block I+1; from
-  // the mirrored lower triangle), staged when the workgroup starts
-  __shared__ T Fl[INV ? NB * NB : 1];
+  // INV: H_I = Li_I^T L_{I+1,I}^T (k_next_block_bwd), staged when the
+  // workgroup starts: once block I+1's records arrive, v_I = u_I - H_I
+  // v_{I+1} is one product away
+  __shared__ __attribute__((aligned(16))) T Hs[INV ? NB * kInvLd : 4];
   if (stop && *stop) return;  // every workgroup reads the same flag
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
@@ -635,10 +695,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     }
     if constexpr (INV) {
       stage_inverse_block<T, true>(Lp, Li + (int64_t)I * NB * NB);
-      if (I + 1 < nblk)
-        stage_last_tile<T, true>(Fl, F, lda, piv, (I + 1) * NB, min(NB, n - (I + 1) * NB), i0,
-                                 in);
-      __syncthreads();  // Fl is read, by other threads, before the next barrier
+      if (I + 1 < nblk) stage_inverse_block<T, false>(Hs, Li + (int64_t)(2 * nblk + I) * NB * NB);
     } else {
       stage_diag_block<T>(Lp, F, lda, piv, i0, in, n);
     }
@@ -728,27 +785,6 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
           }
         }
     }
-    if constexpr (INV) {
-      if (I + 1 < nblk) {  // block I+1 from the staged tile
-        const int cb = (I + 1) * NB + 32 * w;
-        const int row0[2] = {cb, cb + 16};
-        double vv[2];
-        poll_rows_multi<NR, 2>(rec, row0, n, tag, vv);
-#pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-          for (int cc = 0; cc < 16; ++cc) {
-            const int c = 32 * w + 16 * k + cc;
-            const T a = Fl[c * NB + lane], b = Fl[c * NB + H + lane];
-#pragma unroll
-            for (int q = 0; q < NR; ++q) {
-              const T v = static_cast<T>(bcast(vv[k], cc * 4 + q));
-              acc[0][q] += a * v;
-              acc[1][q] += b * v;
-            }
-          }
-      }
-    }
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
       part[w][lane][q] = acc[0][q];
@@ -756,6 +792,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     }
     __syncthreads();
     if constexpr (INV) {
+      // s' into vbot (rows 0..127), then u = Li^T s'
       if (w < 2) {
         const int r = w * H + lane;
         const bool ok = w == 0 ? okA : okB;
@@ -774,15 +811,40 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
         for (int q = 0; q < NR; ++q) part[0][r][q] = sv[q];
       }
       __syncthreads();
+      T u[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) u[q] = sv[q] + part[0][r][q];
+      if (I + 1 < nblk) {
+        // block I+1's solution: wavefront w polls rows 32w .. 32w+31 (0
+        // past n) into vbot (free again), then u -= H_I v_{I+1}
+        const int cb = (I + 1) * NB + 32 * w;
+        const int row0[2] = {cb, cb + 16};
+        double vv[2];
+        poll_rows_multi<NR, 2>(rec, row0, n, tag, vv);
+        if ((lane & 3) < NR) {
+#pragma unroll
+          for (int k = 0; k < 2; ++k)
+            vbot[32 * w + 16 * k + (lane >> 2)][lane & 3] = static_cast<T>(vv[k]);
+        }
+        __syncthreads();
+        T gv[NR];
+        inverse_product<T, NR>(Hs, vbot, gv);
+        if (threadIdx.x >= NB) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) part[1][r][q] = gv[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NR; ++q) u[q] -= gv[q] + part[1][r][q];
+      }
       if (threadIdx.x < NB && r < in) {
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-          const T v = sv[q] + part[0][r][q];
-          xput(rec + (int64_t)(i0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(v));
-          X[(int64_t)pub_perm * ldx + q] = v;
+          xput(rec + (int64_t)(i0 + r) * kSolveMaxRhs + q, tag, q, static_cast<double>(u[q]));
+          X[(int64_t)pub_perm * ldx + q] = u[q];
         }
       }
-      __syncthreads();  // Lp, part, vbot are reused by the next block
+      __syncthreads();  // Lp, Hs, part, vbot are reused by the next block
       continue;
     }
     if (w == 1) {
@@ -1216,8 +1278,10 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
   return BBK_OK;
 }
 
-// Li: the diagonal blocks' inverses from k_diag_inv (binary32 only), or
-// NULL for substitution.
+// Li: the diagonal blocks' inverses from k_diag_inv followed by the
+// forward and backward sweeps' next-block products (k_next_block_fwd,
+// k_next_block_bwd): 3 nblk blocks of 128 x 128, binary32 only; or NULL
+// for substitution.
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
           const int* piv, T* B, int64_t ldb, const T* Li, const int* stop) {
@@ -1424,7 +1488,8 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   const size_t rbytes = ((size_t)n * nrhs * sizeof(double) + 255) / 256 * 256;
   const size_t dbytes = ((size_t)n * nrhs * sizeof(float) + 255) / 256 * 256;
   const int nblk = (n + bbk::kSolveBlock - 1) / bbk::kSolveBlock;
-  const size_t lbytes = (size_t)nblk * bbk::kSolveBlock * bbk::kSolveBlock * sizeof(float);
+  // the diagonal blocks' inverses, then the two sweeps' next-block products
+  const size_t lbytes = 3 * (size_t)nblk * bbk::kSolveBlock * bbk::kSolveBlock * sizeof(float);
   // eta (nrhs), the best backward error and the stop flags
   const size_t ebytes = ((size_t)nrhs * sizeof(double) + 64 + 255) / 256 * 256;
   rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + ebytes + lbytes);
@@ -1448,6 +1513,13 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   // mirrored (otherwise unused) lower triangle of the binary32 factor
   hipLaunchKernelGGL(bbk::k_diag_inv<float>, dim3(nblk), dim3(bbk::kSolveBlock), 0, h->stream, n,
                      S, ldf, piv, Li);
+  if (nblk > 1) {
+    const size_t blk = (size_t)bbk::kSolveBlock * bbk::kSolveBlock;
+    hipLaunchKernelGGL(bbk::k_next_block_fwd<float>, dim3(nblk - 1), dim3(bbk::kSolveBlock), 0,
+                       h->stream, n, S, ldf, piv, Li, Li + nblk * blk);
+    hipLaunchKernelGGL(bbk::k_next_block_bwd<float>, dim3(nblk - 1), dim3(bbk::kSolveBlock), 0,
+                       h->stream, n, S, ldf, piv, Li, Li + 2 * nblk * blk);
+  }
   {
     const unsigned tb = static_cast<unsigned>((n + 63) / 64);
     hipLaunchKernelGGL(bbk::k_mirror_upper<float>, dim3(tb * (tb + 1) / 2), dim3(256), 0,
