@@ -218,94 +218,76 @@ __global__ __launch_bounds__(kSolveBlock) void k_diag_inv(int n, const T* __rest
   for (int i = 0; i < NB; ++i) out[(int64_t)i * NB + c] = (i < bn && c < bn) ? Xs[i][c] : T(0);
 }
 
-// G_J = Li_J L_{J,J-1} for J >= 1 (binary32 refinement solves): the
-// forward sweep forms u_J = Li_J (b_J - sum_{I < J-1} L_JI z_I) before block
-// J-1 is solved and then z_J = u_J - G_J z_{J-1}, one product after block
-// J-1's records arrive.  L_{J,J-1}(r, c) = F(j0 - 128 + c, j0 + r), zero for
-// rows past n and for the D entry of a 2x2 pivot across the block edge,
-// F(j0 - 1, j0).  Stored like Li: G_J row-major at G + J 128^2 (G_0 unused).
-// Thread c forms column c of G_J; Li_J and L_{J,J-1} staged in LDS.
+// Next-block products of the binary32 refinement solves, both in one
+// launch (blockIdx.y 0: forward, 1: backward; blockIdx.x = b < nblk - 1):
+//
+// forward, J = b + 1: G_J = Li_J L_{J,J-1}.  The forward sweep forms u_J =
+// Li_J (b_J - sum_{I < J-1} L_JI z_I) before block J-1 is solved, then z_J
+// = u_J - G_J z_{J-1}.  L_{J,J-1}(r, c) = F(j0 - 128 + c, j0 + r).
+// Stored row-major at Li + (nblk + J) 128^2.
+//
+// backward, I = b: H_I = Li_I^T L_{I+1,I}^T.  The backward sweep forms u_I =
+// Li_I^T (D^-1 z_I - sum_{J > I+1} L_JI^T v_J), then v_I = u_I - H_I v_{I+1}.
+// L_{I+1,I}(c, k) = F(i0 + k, i0 + 128 + c).  Stored row-major at Li +
+// (2 nblk + I) 128^2.
+//
+// The tiles read as 0 for rows past n and for the D entry of a 2x2 pivot
+// across the block edge (F(j0 - 1, j0), F(i0 + 127, i0 + 128)).  Thread
+// (c, h) forms column c of the product for rows 32h .. 32h+31; Li's block
+// and the tile staged in LDS.
 template <typename T>
-__global__ __launch_bounds__(kSolveBlock) void k_next_block_fwd(int n, const T* __restrict__ F,
+__global__ __launch_bounds__(4 * kSolveBlock) void k_next_block(int n, const T* __restrict__ F,
                                                                 int64_t lda,
                                                                 const int* __restrict__ piv,
-                                                                const T* __restrict__ Li,
-                                                                T* __restrict__ G) {
+                                                                T* __restrict__ Li, int nblk) {
   constexpr int NB = kSolveBlock;
-  __shared__ T Ls[NB][NB + 1];  // Li_J(i, k)
-  __shared__ T Lt[NB][NB + 1];  // L_{J,J-1}(r, c)
-  const int J = blockIdx.x + 1;
-  const int j0 = J * NB, jn = min(NB, n - j0), p0 = j0 - NB;
-  const int c = threadIdx.x;
-  const T* Lj = Li + (int64_t)J * NB * NB;
-  const bool edge = piv[j0 - 1] == 2;
+  __shared__ T Ls[NB][NB + 1];  // the diagonal block's inverse, Li(i, k)
+  __shared__ T Lt[NB][NB + 1];  // the tile: F(p0 + k, q0 + c) at Lt[k][c]
+  const bool fwd = blockIdx.y == 0;
+  const int b = blockIdx.x;
+  const int D = fwd ? b + 1 : b;       // block whose inverse is used
+  const int p0 = b * NB, q0 = p0 + NB;  // tile rows (block b), columns (block b+1)
+  const int qn = min(NB, n - q0);
+  const int c = threadIdx.x & (NB - 1), h = threadIdx.x >> 7;
+  const T* Lj = Li + (int64_t)D * NB * NB;
+  const bool edge = piv[q0 - 1] == 2;
 #pragma unroll 8
-  for (int k = 0; k < NB; ++k) {
+  for (int kk = 0; kk < NB / 4; ++kk) {
+    const int k = 32 * h + kk;
     Ls[k][c] = Lj[(int64_t)k * NB + c];
-    // row p0 + k of F, column j0 + c: L(j0 + c, p0 + k) = L_{J,J-1}(c, k)
-    T x = c < jn ? F[(int64_t)(p0 + k) * lda + j0 + c] : T(0);
+    T x = c < qn ? F[(int64_t)(p0 + k) * lda + q0 + c] : T(0);
     if (k == NB - 1 && c == 0 && edge) x = T(0);
-    Lt[c][k] = x;
+    Lt[k][c] = x;
   }
   __syncthreads();
-  T* out = G + (int64_t)J * NB * NB;
-  for (int i = 0; i < NB; ++i) {
-    // Li_J is unit lower triangular: k <= i
+  T* out = Li + (int64_t)((fwd ? nblk : 2 * nblk) + D) * NB * NB;
+  for (int ii = 0; ii < NB / 4; ++ii) {
+    const int i = 32 * h + ii;
     T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
-    int k = 0;
-    for (; k + 4 <= i + 1; k += 4) {
-      a0 += Ls[i][k] * Lt[k][c];
-      a1 += Ls[i][k + 1] * Lt[k + 1][c];
-      a2 += Ls[i][k + 2] * Lt[k + 2][c];
-      a3 += Ls[i][k + 3] * Lt[k + 3][c];
+    if (fwd) {
+      // G(i, c) = sum_{k <= i} Li(i, k) L_{J,J-1}(k, c), L_{J,J-1}(k, c) =
+      // F(p0 + c, q0 + k) = Lt[c][k]
+      int k = 0;
+      for (; k + 4 <= i + 1; k += 4) {
+        a0 += Ls[i][k] * Lt[c][k];
+        a1 += Ls[i][k + 1] * Lt[c][k + 1];
+        a2 += Ls[i][k + 2] * Lt[c][k + 2];
+        a3 += Ls[i][k + 3] * Lt[c][k + 3];
+      }
+      for (; k <= i; ++k) a0 += Ls[i][k] * Lt[c][k];
+    } else {
+      // H(i, c) = sum_{k >= i} Li(k, i) L_{I+1,I}(c, k), L_{I+1,I}(c, k) =
+      // F(p0 + k, q0 + c) = Lt[k][c]
+      int k = i;
+      for (; k + 4 <= NB; k += 4) {
+        a0 += Ls[k][i] * Lt[k][c];
+        a1 += Ls[k + 1][i] * Lt[k + 1][c];
+        a2 += Ls[k + 2][i] * Lt[k + 2][c];
+        a3 += Ls[k + 3][i] * Lt[k + 3][c];
+      }
+      for (; k < NB; ++k) a0 += Ls[k][i] * Lt[k][c];
     }
-    for (; k <= i; ++k) a0 += Ls[i][k] * Lt[k][c];
     out[(int64_t)i * NB + c] = (a0 + a1) + (a2 + a3);
-  }
-}
-
-// H_I = Li_I^T L_{I+1,I}^T for I < nblk - 1 (binary32 refinement solves):
-// the backward sweep forms u_I = Li_I^T (D^-1 z_I - sum_{J > I+1} L_JI^T v_J)
-// before block I+1 is solved and then v_I = u_I - H_I v_{I+1}.
-// L_{I+1,I}(c, k) = F(i0 + k, i0 + 128 + c), zero for rows past n and for
-// the D entry of a 2x2 pivot across the block edge, F(i0 + 127, i0 + 128).
-// Stored like Li: H_I row-major at H + I 128^2.  Thread c forms column c:
-// H_I(r, c) = sum_{k >= r} Li_I(k, r) L_{I+1,I}(c, k).
-template <typename T>
-__global__ __launch_bounds__(kSolveBlock) void k_next_block_bwd(int n, const T* __restrict__ F,
-                                                                int64_t lda,
-                                                                const int* __restrict__ piv,
-                                                                const T* __restrict__ Li,
-                                                                T* __restrict__ Hb) {
-  constexpr int NB = kSolveBlock;
-  __shared__ T Ls[NB][NB + 1];  // Li_I(k, r)
-  __shared__ T Lb[NB][NB + 1];  // Lb[k][c] = L_{I+1,I}(c, k)
-  const int I = blockIdx.x;
-  const int i0 = I * NB, i1 = i0 + NB, cn = min(NB, n - i1);
-  const int c = threadIdx.x;
-  const T* Lj = Li + (int64_t)I * NB * NB;
-  const bool edge = piv[i0 + NB - 1] == 2;
-#pragma unroll 8
-  for (int k = 0; k < NB; ++k) {
-    Ls[k][c] = Lj[(int64_t)k * NB + c];
-    T x = c < cn ? F[(int64_t)(i0 + k) * lda + i1 + c] : T(0);
-    if (k == NB - 1 && c == 0 && edge) x = T(0);
-    Lb[k][c] = x;
-  }
-  __syncthreads();
-  T* out = Hb + (int64_t)I * NB * NB;
-  for (int r = 0; r < NB; ++r) {
-    // Li_I is unit lower triangular: k >= r
-    T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
-    int k = r;
-    for (; k + 4 <= NB; k += 4) {
-      a0 += Ls[k][r] * Lb[k][c];
-      a1 += Ls[k + 1][r] * Lb[k + 1][c];
-      a2 += Ls[k + 2][r] * Lb[k + 2][c];
-      a3 += Ls[k + 3][r] * Lb[k + 3][c];
-    }
-    for (; k < NB; ++k) a0 += Ls[k][r] * Lb[k][c];
-    out[(int64_t)r * NB + c] = (a0 + a1) + (a2 + a3);
   }
 }
 
@@ -430,7 +412,7 @@ __device__ __forceinline__ void serial64(T (&v)[NR], const T* Lp, int lane, Idx 
 // applies them to the bottom 64 and solves those.
 //
 // INV: the diagonal block's inverse Li (k_diag_inv) replaces the serial
-// substitution, and G_J = Li_J L_{J,J-1} (k_next_block_fwd) the last
+// substitution, and G_J = Li_J L_{J,J-1} (k_next_block) the last
 // block's contribution: u_J = Li_J (b_J - sum_{I < J-1} L_JI z_I) is formed
 // before block J-1 is solved, then z_J = u_J - G_J z_{J-1}, so one product
 // (thread (r, h) sums row r over columns 64h .. 64h+63) separates block
@@ -446,7 +428,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
   __shared__ T part[4][NB][NR];
   __shared__ T ztop[INV ? NB : H][NR];
-  // INV: G_J = Li_J L_{J,J-1} (k_next_block_fwd, stored after the nblk
+  // INV: G_J = Li_J L_{J,J-1} (k_next_block, stored after the nblk
   // inverse blocks), staged when the workgroup starts: once block J-1's
   // records arrive, z_J = u_J - G_J z_{J-1} is one product away
   __shared__ __attribute__((aligned(16))) T Gs[INV ? NB * kInvLd : 4];
@@ -655,7 +637,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
 // INV (refinement solves): u_I = Li_I^T s' with Li the diagonal block's
 // inverse (k_diag_inv), s' = D^-1 z_I - sum_{J > I+1} L_JI^T v_J, formed
 // before block I+1 is solved; then v_I = u_I - H_I v_{I+1} with H_I =
-// Li_I^T L_{I+1,I}^T (k_next_block_bwd).  Thread (r, h) sums row r of the
+// Li_I^T L_{I+1,I}^T (k_next_block).  Thread (r, h) sums row r of the
 // staged Li^T or H_I over columns 64h .. 64h+63.  The strictly lower triangle of F then
 // holds the mirror of the upper one (k_mirror_upper), and L_JI^T is read
 // from it: element F(i, j), i in I, as F[j lda + i], consecutive lanes on
@@ -672,7 +654,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
   __shared__ T part[4][NB][NR];
   __shared__ T vbot[INV ? NB : H][NR];
-  // INV: H_I = Li_I^T L_{I+1,I}^T (k_next_block_bwd), staged when the
+  // INV: H_I = Li_I^T L_{I+1,I}^T (k_next_block), staged when the
   // workgroup starts: once block I+1's records arrive, v_I = u_I - H_I
   // v_{I+1} is one product away
   __shared__ __attribute__((aligned(16))) T Hs[INV ? NB * kInvLd : 4];
@@ -1279,9 +1261,8 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
 }
 
 // Li: the diagonal blocks' inverses from k_diag_inv followed by the
-// forward and backward sweeps' next-block products (k_next_block_fwd,
-// k_next_block_bwd): 3 nblk blocks of 128 x 128, binary32 only; or NULL
-// for substitution.
+// forward and backward sweeps' next-block products (k_next_block): 3 nblk
+// blocks of 128 x 128, binary32 only; or NULL for substitution.
 template <typename T>
 int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
           const int* piv, T* B, int64_t ldb, const T* Li, const int* stop) {
@@ -1513,13 +1494,9 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   // mirrored (otherwise unused) lower triangle of the binary32 factor
   hipLaunchKernelGGL(bbk::k_diag_inv<float>, dim3(nblk), dim3(bbk::kSolveBlock), 0, h->stream, n,
                      S, ldf, piv, Li);
-  if (nblk > 1) {
-    const size_t blk = (size_t)bbk::kSolveBlock * bbk::kSolveBlock;
-    hipLaunchKernelGGL(bbk::k_next_block_fwd<float>, dim3(nblk - 1), dim3(bbk::kSolveBlock), 0,
-                       h->stream, n, S, ldf, piv, Li, Li + nblk * blk);
-    hipLaunchKernelGGL(bbk::k_next_block_bwd<float>, dim3(nblk - 1), dim3(bbk::kSolveBlock), 0,
-                       h->stream, n, S, ldf, piv, Li, Li + 2 * nblk * blk);
-  }
+  if (nblk > 1)
+    hipLaunchKernelGGL(bbk::k_next_block<float>, dim3(nblk - 1, 2), dim3(4 * bbk::kSolveBlock), 0,
+                       h->stream, n, S, ldf, piv, Li, nblk);
   {
     const unsigned tb = static_cast<unsigned>((n + 63) / 64);
     hipLaunchKernelGGL(bbk::k_mirror_upper<float>, dim3(tb * (tb + 1) / 2), dim3(256), 0,
