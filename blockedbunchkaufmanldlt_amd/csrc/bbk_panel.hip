@@ -373,6 +373,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   constexpr int RR = PanelCfg<T>::kRegRows;
   constexpr int LR = PanelCfg<T>::kLdsRows;
   constexpr int NR = RR > 0 ? RR : 1;
+  // rows [0, RR], RR + 1 of them, sit in Wl until they move to registers
+  static_assert(LR >= RR, "the LDS rows must hold the register rows before they move");
   constexpr int WS = NT;  // LDS row stride of W
   // W(q, own column): q < base (= 0 or RR) in registers wr[q], the rest in
   // LDS row q - base
