@@ -2,6 +2,8 @@
 // synthetic-input generator.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "../../include/bbk_ldlt.h"
@@ -75,15 +77,18 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 // u = 2 * (key >> 11) * 2^-53 - 1, key = splitmix64(splitmix64(seed) ^ (i << 32 | j)).
 __global__ void k_gen_synth(int n, uint64_t seed, double scale, int zstride,
                             double* A, int64_t lda) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)n * n) return;
-  const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
-  const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const uint64_t key = splitmix64(splitmix64(seed) ^
-                                  ((uint64_t)lo << 32 | (uint32_t)hi));
-  double v = scale * (2.0 * (static_cast<double>(key >> 11) * 0x1.0p-53) - 1.0);
-  if (i == j && zstride > 0 && i % zstride == 0) v = 0.0;
-  A[(int64_t)i * lda + j] = v;
+  // grid-stride: n^2 passes the 2^32 work-items of one launch at n = 65536
+  const int64_t total = (int64_t)n * n;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += step) {
+    const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const uint64_t key = splitmix64(splitmix64(seed) ^
+                                    ((uint64_t)lo << 32 | (uint32_t)hi));
+    double v = scale * (2.0 * (static_cast<double>(key >> 11) * 0x1.0p-53) - 1.0);
+    if (i == j && zstride > 0 && i % zstride == 0) v = 0.0;
+    A[(int64_t)i * lda + j] = v;
+  }
 }
 
 }  // namespace bbk
@@ -143,9 +148,9 @@ int bbk_dgen_synth(bbk_handle_t handle, int n, uint64_t seed, double scale,
   if (lda < n) return -7;
   Handle* h = reinterpret_cast<Handle*>(handle);
   const int64_t total = (int64_t)n * n;
-  hipLaunchKernelGGL(bbk::k_gen_synth, dim3((unsigned)((total + 255) / 256)),
-                     dim3(256), 0, h->stream, n, seed, scale, zero_diag_stride,
-                     d_A, lda);
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, int64_t(1) << 20);
+  hipLaunchKernelGGL(bbk::k_gen_synth, dim3((unsigned)blocks), dim3(256), 0, h->stream, n, seed,
+                     scale, zero_diag_stride, d_A, lda);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? BBK_OK : h->hip_fail(e, "gen_synth");
 }
