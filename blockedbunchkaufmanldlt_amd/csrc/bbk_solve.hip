@@ -1100,7 +1100,8 @@ __global__ void k_berr(int nr, const double* __restrict__ norms,
 }
 
 // Refinement stopping test on the device, after correction number it:
-// stop (ctl[0] = 1, ctl[1] = it, *best = max_q eta_q) when the largest
+// *best is the smallest max_q eta_q so far, ctl[2] = 1 when this correction
+// improved it (else 0); stop (ctl[0] = 1, ctl[1] = it) when the largest
 // backward error is <= tol, is NaN, or it reached max_iter.  Once set, the
 // flag stays: later checks and the kernels that take it do nothing.
 __global__ void k_refine_check(const double* __restrict__ eta, int nr, double tol, int it,
@@ -1113,12 +1114,24 @@ __global__ void k_refine_check(const double* __restrict__ eta, int nr, double to
     if (!(e == e)) nan = true;
     m = e > m ? e : m;
   }
-  if (nan) m = __builtin_nan("");
+  const bool improved = !nan && m < *best;
+  if (improved) *best = m;
+  ctl[2] = improved ? 1 : 0;
   if (nan || m <= tol || it >= max_iter) {
     ctl[0] = 1;
     ctl[1] = it;
-    *best = m;
   }
+}
+
+// Xb := X (n x nr, Xb with leading dimension nr) when the last stopping
+// test recorded an improvement, so that Xb holds the best iterate.
+__global__ void k_keep_best(int n, int nr, const double* __restrict__ X, int64_t ldx,
+                            double* __restrict__ Xb, const int* __restrict__ ctl) {
+  if (!ctl[2]) return;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * nr) return;
+  const int64_t i = e / nr, q = e % nr;
+  Xb[i * nr + q] = X[i * ldx + q];
 }
 
 __global__ void k_fill_f64(double* p, int count, double v) {
@@ -1473,7 +1486,9 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   const size_t lbytes = 3 * (size_t)nblk * bbk::kSolveBlock * bbk::kSolveBlock * sizeof(float);
   // eta (nrhs), the best backward error and the stop flags
   const size_t ebytes = ((size_t)nrhs * sizeof(double) + 64 + 255) / 256 * 256;
-  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + ebytes + lbytes);
+  // the best iterate so far (n x nrhs)
+  const size_t xbytes = rbytes;
+  rc = h->reserve_mixed(fbytes + ibytes + rbytes + dbytes + ebytes + lbytes + xbytes);
   if (rc != BBK_OK) return rc;
   char* base = reinterpret_cast<char*>(h->mixed_ws);
   float* S = reinterpret_cast<float*>(base);
@@ -1485,6 +1500,7 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   double* best_dev = eta + nrhs;
   int* ctl = reinterpret_cast<int*>(best_dev + 1);
   float* Li = reinterpret_cast<float*>(base + fbytes + ibytes + rbytes + dbytes + ebytes);
+  double* Xb = reinterpret_cast<double*>(base + fbytes + ibytes + rbytes + dbytes + ebytes + lbytes);
   // FP32 copy of the upper triangle, FP32 factorization
   bbk::copy_upper<double, float>(h, n, d_A, lda, S, ldf);
   rc = bbk::factorize<float>(h, n, 0, S, ldf, perm, piv);
@@ -1530,9 +1546,15 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
     // Corrections d = solve32(r), x += d, then r and eta again, queued a few
     // at a time with the stopping test on the device (k_refine_check): after
     // the test has stopped, the queued kernels return at once, so x is the
-    // iterate the test accepted and the host waits once per chunk.
+    // iterate the test accepted and the host waits once per chunk.  The
+    // best iterate is kept (k_keep_best) and returned when the test stops
+    // short of tol (max_iter reached, or a diverging correction).
     constexpr int kChunk = 4;
-    hipLaunchKernelGGL(bbk::k_fill_i32, dim3(1), dim3(32), 0, h->stream, ctl, 2, 0);
+    hipLaunchKernelGGL(bbk::k_fill_i32, dim3(1), dim3(32), 0, h->stream, ctl, 3, 0);
+    // x0 is the best iterate so far
+    hipLaunchKernelGGL(bbk::k_fill_f64, dim3(1), dim3(32), 0, h->stream, best_dev, 1, best);
+    hipLaunchKernelGGL((bbk::k_copy_cols<double, double>), dim3(eb), dim3(256), 0, h->stream, n,
+                       nrhs, d_X, ldx, Xb, (int64_t)nrhs);
     int queued = 0;
     for (;;) {
       for (int c = 0; c < kChunk && queued < max_iter; ++c) {
@@ -1547,6 +1569,8 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
         if (rc != BBK_OK) return rc;
         hipLaunchKernelGGL(bbk::k_refine_check, dim3(1), dim3(32), 0, h->stream, eta, nrhs, tol,
                            queued, max_iter, ctl, best_dev);
+        hipLaunchKernelGGL(bbk::k_keep_best, dim3(eb), dim3(256), 0, h->stream, n, nrhs, d_X, ldx,
+                           Xb, ctl);
       }
       int hc[2] = {0, 0};
       e = hipMemcpyAsync(hc, ctl, sizeof(hc), hipMemcpyDeviceToHost, h->stream);
@@ -1558,6 +1582,13 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
         it = hc[1];
         break;
       }
+    }
+    if (!(best <= tol)) {
+      // not converged: return the best iterate, whose backward error is best
+      hipLaunchKernelGGL((bbk::k_copy_cols<double, double>), dim3(eb), dim3(256), 0, h->stream,
+                         n, nrhs, Xb, (int64_t)nrhs, d_X, ldx);
+      e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) return h->hip_fail(e, "restore best iterate");
     }
   }
   if (iters) *iters = it;

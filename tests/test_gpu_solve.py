@@ -105,6 +105,35 @@ def test_mixed_reports_non_convergence(oracle, handle):
     X = torch.zeros((n, 1), dtype=torch.float64, device="cuda")
     info = handle.sysv_mixed(Ad, torch.from_numpy(b).cuda(), X, tol=1e-15, max_iter=3)
     assert not info["converged"] and info["iters"] == 3
+    # X holds the best iterate, whose backward error is the one reported
+    Xh = X.cpu().numpy()
+    assert np.isfinite(Xh).all()
+    be = berr_np(A, b, Xh).max()
+    assert abs(be - info["berr"]) <= 1e-6 * be, (be, info["berr"])
+
+
+def test_mixed_returns_best_iterate_when_refinement_fails(oracle, handle):
+    """Condition 1e13: binary32 corrections cannot reach tol and may make x
+    worse.  The driver must return the best iterate (never worse than x0)
+    with its backward error, not the last one."""
+    import torch
+    n = 300
+    A = eig_matrix(n, 1e13, 17)
+    b = np.random.default_rng(18).standard_normal((n, 1))
+    Ad = to_device(A)
+    Ad[:, :n] = torch.from_numpy(A).cuda()
+    bd = torch.from_numpy(b).cuda()
+    X0 = torch.zeros((n, 1), dtype=torch.float64, device="cuda")
+    info0 = handle.sysv_mixed(Ad, bd, X0, tol=1e-15, max_iter=0)
+    be0 = berr_np(A, b, X0.cpu().numpy()).max()
+    X = torch.zeros((n, 1), dtype=torch.float64, device="cuda")
+    info = handle.sysv_mixed(Ad, bd, X, tol=1e-15, max_iter=12)
+    assert not info["converged"]
+    Xh = X.cpu().numpy()
+    assert np.isfinite(Xh).all()
+    be = berr_np(A, b, Xh).max()
+    assert abs(be - info["berr"]) <= 1e-6 * be, (be, info["berr"])
+    assert be <= be0 * (1 + 1e-9), (be, be0, info0["berr"])
 
 
 def test_multi_rhs_mixed(oracle, handle):
