@@ -233,3 +233,42 @@ def test_kernels_shapes(oracle, handle):
         handle.kernel_scale_2x2(10, a, b, c, t1, 3, t2, 3, o1, 3, o2, 3)
         handle.synchronize()
         assert np.array_equal(o1.cpu().numpy(), w1) and np.array_equal(o2.cpu().numpy(), w2)
+
+
+def test_kernels_host_forms(oracle, handle):
+    """The host-pointer forms that the drop-in's naive_block_ldlt::kernels::*
+    call (operands copied to the device and back) against the oracle's
+    restatement of src/kernels.cc, with strides and an exact-zero pivot."""
+    import ctypes
+    import blockedbunchkaufmanldlt_amd as B
+    lib = B.lib()
+    vp, i32, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.bbk_kernel_scale_1x1_host.argtypes = [vp, i32, f64, vp, i32, vp, i32]
+    lib.bbk_kernel_scale_2x2_host.argtypes = [vp, i32, f64, f64, f64, vp, i32, vp, i32, vp, i32,
+                                              vp, i32]
+    lib.bbk_kernel_update_trailing_host.argtypes = [vp, i32, i32, i32, vp, i32, vp, i32, vp, i32]
+    L = oracle.lib()
+    d = oracle._d
+    rng = np.random.default_rng(5)
+    for (rows, cols, inner) in [(1, 1, 1), (19, 7, 3), (70, 130, 128)]:
+        ls, ws, ts = rows + 2, cols + 3, cols + 1
+        left = rng.standard_normal(inner * ls)
+        work = rng.standard_normal(inner * ws)
+        tgt = rng.standard_normal(rows * ts)
+        want, got = tgt.copy(), tgt.copy()
+        L.bko_kernel_update_trailing(rows, cols, inner, d(left), ls, d(work), ws, d(want), ts)
+        assert lib.bbk_kernel_update_trailing_host(handle.h, rows, cols, inner, d(left), ls,
+                                                   d(work), ws, d(got), ts) == 0
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    for pivot in (0.0, 1.75):
+        s = rng.standard_normal(33)
+        want, got = np.zeros(33), np.zeros(33)
+        L.bko_kernel_scale_1x1(11, pivot, d(s), 3, d(want), 3)
+        assert lib.bbk_kernel_scale_1x1_host(handle.h, 11, pivot, d(s), 3, d(got), 3) == 0
+        assert np.array_equal(got, want)
+    x1, x2 = rng.standard_normal(24), rng.standard_normal(24)
+    w1, w2, g1, g2 = np.zeros(24), np.zeros(24), np.zeros(24), np.zeros(24)
+    L.bko_kernel_scale_2x2(12, -1.5, 4.0, 0.25, d(x1), 2, d(x2), 2, d(w1), 2, d(w2), 2)
+    assert lib.bbk_kernel_scale_2x2_host(handle.h, 12, -1.5, 4.0, 0.25, d(x1), 2, d(x2), 2,
+                                         d(g1), 2, d(g2), 2) == 0
+    assert np.array_equal(g1, w1) and np.array_equal(g2, w2)
