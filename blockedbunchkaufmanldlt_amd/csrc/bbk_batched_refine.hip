@@ -8,9 +8,10 @@
 //   (||A||_inf ||x_q||_inf + ||b_q||_inf);  stop when max_q eta_q <= tol or
 //   after max_iter corrections;  x += solve32(r) }
 // with the right-hand sides (up to four) kept in LDS: binary32 copies for
-// the triangular sweeps, binary64 accumulators for A x.  The sweeps use
-// 64-row blocks: one wavefront solves a diagonal block serially (readlane
-// broadcast), the whole workgroup applies the block to the rest.  The 2x2
+// the triangular sweeps, binary64 accumulators for A x.  Both sweeps are
+// right-looking over 64-row blocks: one wavefront solves a diagonal block
+// serially (readlane broadcast), the whole workgroup applies the block to
+// the rest, one row of the factor per thread.  The 2x2
 // D blocks use LAPACK dsytrs's scaled formula in binary64.  A x reads each
 // stored row of the upper triangle once: the row part is a wavefront dot
 // product, the mirrored column part is added into LDS accumulators with
@@ -152,31 +153,12 @@ __device__ void solve32(int n, const float* __restrict__ F, int64_t ldf,
   __syncthreads();
   for (int i = t; i < n * NR; i += kRT) z[i] = part[i];
   __syncthreads();
-  // backward: z := L^-T z; block I first takes dot products with the
-  // finished rows below it, then wavefront 0 solves the diagonal block
+  // backward: z := L^-T z, right-looking over 64-row blocks from the last:
+  // wavefront 0 solves the diagonal block, then every row j above it takes
+  // z_j -= sum_{i in block} L(i, j) z_i, L(i, j) = F(j, i): one row of F
+  // per thread, 64 independent loads
   for (int I = nb - 1; I >= 0; --I) {
     const int i0 = I * kRB, in = min(kRB, n - i0);
-    for (int rr = w; rr < in; rr += 4) {
-      const int i = i0 + rr;
-      // column i + 1 (first after the block when rr = in - 1) is 0 in L
-      // when (i, i + 1) is a 2x2 block
-      const int jskip = (rr == in - 1 && i + 1 < n && piv[i] == 2) ? i + 1 : -1;
-      double acc[NR];
-#pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] = 0.0;
-#pragma unroll 4
-      for (int j = i0 + in + lane; j < n; j += 64) {
-        const float l = j == jskip ? 0.0f : F[(int64_t)i * ldf + j];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] += (double)l * (double)z[j * NR + q];
-      }
-#pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        const double s = wave_sum(acc[q]);
-        if (lane == 0) part[rr * NR + q] = (float)s;
-      }
-    }
-    __syncthreads();
     if (w == 0) {
       const int i = i0 + lane;
       const bool ok = lane < in;
@@ -191,7 +173,7 @@ __device__ void solve32(int n, const float* __restrict__ F, int64_t ldf,
       }
       float v[NR];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) v[q] = ok ? z[i * NR + q] - part[lane * NR + q] : 0.0f;
+      for (int q = 0; q < NR; ++q) v[q] = ok ? z[i * NR + q] : 0.0f;
 #pragma unroll
       for (int jj = kRB - 1; jj > 0; --jj) {
         if (jj < in) {
@@ -202,6 +184,33 @@ __device__ void solve32(int n, const float* __restrict__ F, int64_t ldf,
       if (ok)
 #pragma unroll
         for (int q = 0; q < NR; ++q) z[i * NR + q] = v[q];
+    }
+    __syncthreads();
+    // row i0 - 1 couples to column i0 only through L, which is 0 when a
+    // 2x2 block spans the two
+    const bool cut = i0 > 0 && piv[i0 - 1] == 2;
+    for (int j = t; j < i0; j += kRT) {
+      const int ifirst = (cut && j == i0 - 1) ? 1 : 0;
+      float acc[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] = 0.0f;
+      // the row's 64 entries in two halves of 32 loads in flight
+#pragma unroll
+      for (int h = 0; h < kRB; h += kRB / 2) {
+        float lv[kRB / 2];
+#pragma unroll
+        for (int u = 0; u < kRB / 2; ++u) {
+          const int ii = h + u;
+          lv[u] = (ii >= ifirst && ii < in) ? F[(int64_t)j * ldf + i0 + ii] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kRB / 2; ++u) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) acc[q] += lv[u] * z[(i0 + h + u) * NR + q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NR; ++q) z[j * NR + q] -= acc[q];
     }
     __syncthreads();
   }
