@@ -238,9 +238,25 @@ __device__ __forceinline__ void team_symv(int n, const double* __restrict__ A, i
         racc[r][q] = 0.0;
         xj[r][q] = ABS ? 1.0 : (jb + r < n ? X[(int64_t)(jb + r) * ldx + q] : 0.0);
       }
+    // the chunk's RB entries are loaded one chunk ahead
+    double an[RB];
+    {
+      const int i = jb + lane;
+#pragma unroll
+      for (int r = 0; r < RB; ++r) an[r] = (i < n && i >= jb + r) ? A[(int64_t)(jb + r) * lda + i] : 0.0;
+    }
     for (int ib = jb; ib < n; ib += 64) {
       const int i = ib + lane;
       const bool vi = i < n;
+      double ac[RB];
+#pragma unroll
+      for (int r = 0; r < RB; ++r) ac[r] = an[r];
+      {
+        const int i2 = i + 64;
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+          an[r] = (i2 < n && i2 >= jb + r) ? A[(int64_t)(jb + r) * lda + i2] : 0.0;
+      }
       double xi[QN], cacc[QN];
 #pragma unroll
       for (int q = 0; q < QN; ++q) {
@@ -250,7 +266,7 @@ __device__ __forceinline__ void team_symv(int n, const double* __restrict__ A, i
 #pragma unroll
       for (int r = 0; r < RB; ++r) {
         const int j = jb + r;
-        double a = (vi && i >= j) ? A[(int64_t)j * lda + i] : 0.0;  // i >= j implies j < n
+        double a = ac[r];  // (vi && i >= j) ? A(j, i) : 0; i >= j implies j < n
         if (ABS) a = fabs(a);
 #pragma unroll
         for (int q = 0; q < QN; ++q) racc[r][q] += a * xi[q];
