@@ -112,12 +112,35 @@ __device__ __forceinline__ T ucol(const ScaleTab& st, const T* W, int n, int p, 
                         st.c1[p] * w);
 }
 
+// sum_{p < kk} l[p] W(p, j) (row stride n).  ACC4: four interleaved
+// partial sums, so that four loads of W are in flight per step (W in global
+// memory, and binary64 in LDS: measured faster); else one running sum
+// (binary32 with W in LDS: measured faster).
+template <typename T, bool ACC4>
+__device__ __forceinline__ T panel_rows_dot(const T* l, const T* W, int n, int kk, int j) {
+  if constexpr (!ACC4) {
+    T acc = T(0);
+    for (int p = 0; p < kk; ++p) acc += l[p] * W[(int64_t)p * n + j];
+    return acc;
+  }
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  int p = 0;
+  for (; p + 4 <= kk; p += 4) {
+    a0 += l[p] * W[(int64_t)p * n + j];
+    a1 += l[p + 1] * W[(int64_t)(p + 1) * n + j];
+    a2 += l[p + 2] * W[(int64_t)(p + 2) * n + j];
+    a3 += l[p + 3] * W[(int64_t)(p + 3) * n + j];
+  }
+  for (; p < kk; ++p) a0 += l[p] * W[(int64_t)p * n + j];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // Panel P of one matrix, whole team of NT threads, starting at column s
 // (one column early when the previous panel deferred a 2x2): Bunch-Kaufman
 // steps with the panel rows W (row stride n; rows 0..bw = panel rows, row
 // bw + 1 = the candidate row C).  Returns the panel's end e (the nominal
 // end, or the column of a deferred 2x2).
-template <typename T, int NT>
+template <typename T, int NT, bool ACC4>
 __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
                             int* __restrict__ piv, int bw, int P, int s, T* W, T* Lk, T* Lr,
                             ScaleTab& st, double* rv, int* ri) {
@@ -134,9 +157,7 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
     int arg = n;
     T* Wk = W + (int64_t)kk * n;
     for (int j = k + tid; j < n; j += NT) {
-      T acc = T(0);
-      for (int p = 0; p < kk; ++p) acc += Lk[p] * W[(int64_t)p * n + j];
-      const T w = F[(int64_t)k * lda + j] - acc;
+      const T w = F[(int64_t)k * lda + j] - panel_rows_dot<T, ACC4>(Lk, W, n, kk, j);
       Wk[j] = w;
       const double a = fabs(static_cast<double>(w));
       if (j > k && a > best) {
@@ -158,9 +179,7 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
       double sig = 0.0;
       for (int i = k + tid; i < n; i += NT) {
         const T base = i < r ? F[(int64_t)i * lda + r] : F[(int64_t)r * lda + i];
-        T acc = T(0);
-        for (int p = 0; p < kk; ++p) acc += Lr[p] * W[(int64_t)p * n + i];
-        const T cv = base - acc;
+        const T cv = base - panel_rows_dot<T, ACC4>(Lr, W, n, kk, i);
         C[i] = cv;
         const double a = fabs(static_cast<double>(cv));
         if (i != r) sig = (sig < a) ? a : sig;
@@ -335,7 +354,10 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
   __syncthreads();
   int s = 0;
   for (int P = 0; s < n; ++P) {
-    const int e = factor_panel<T, NT>(F, lda, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri);
+    // persistent teams: the matrix's W in LDS unless it is large (binary64
+    // past about 280): four partial sums for binary64 only
+    const int e = factor_panel<T, NT, sizeof(T) == 8>(F, lda, n, perm, piv, bw, P, s, W, Lk, Lr,
+                                                       st, rv, ri);
     simt_update<T, NT>(F, lda, n, s, e, W);
     __syncthreads();
     s = e;
@@ -417,7 +439,7 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   }
   const int s = P == 0 ? 0 : S[0];
   T* W = a.ws + a.wsoff[m];
-  const int e = factor_panel<T, 256>(a.A[m], a.lda[m], n, perm, piv, bw, P, s, W, Lk, Lr, stab,
+  const int e = factor_panel<T, 256, true>(a.A[m], a.lda[m], n, perm, piv, bw, P, s, W, Lk, Lr, stab,
                                      rv, ri);
   if (threadIdx.x == 0) {
     S[0] = e;
