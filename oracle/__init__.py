@@ -105,6 +105,16 @@ def ref():
             [_dp, ctypes.c_int] * 4
         R.ref_kernel_update_trailing.argtypes = [ctypes.c_int] * 3 + \
             [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int]
+        # helpers of the reference's own test programs (oracle/ref_tests_*.cc)
+        R.ref_random_symmetric_matrix.argtypes = [ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_double, _dp]
+        R.ref_random_symmetric_block_matrix.argtypes = [ctypes.c_int, ctypes.c_int,
+                                                        ctypes.c_int, ctypes.c_double, _dp]
+        R.ref_residual_samples.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip]
+        R.ref_residual_samples.restype = ctypes.c_int
+        R.ref_stress_case.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_double, _dp, _ip, _ip, _dp]
+        R.ref_stress_case.restype = ctypes.c_int
         _ref = R
     return _ref
 
@@ -197,6 +207,42 @@ def ref_time_factorize(A: np.ndarray, bs: int, want_factors: bool = False):
     if want_factors:
         return float(sec[0]), Factorization(n, bs, compact, perm, piv)
     return float(sec[0])
+
+
+# The reference's own test helpers (checker library only).
+REF_SAMPLE_COUNT = 4096  # kResidualSampleCount, tests/block_ldlt_large_stress_test.cc:25
+
+
+def ref_random_symmetric_matrix(n: int, seed: int, scale: float = 1.0) -> np.ndarray:
+    out = np.zeros((n, n))
+    ref().ref_random_symmetric_matrix(n, seed, scale, _d(out))
+    return out
+
+
+def ref_random_symmetric_block_matrix(n: int, bs: int, seed: int, scale: float = 1.0) -> np.ndarray:
+    out = np.zeros((n, n))
+    ref().ref_random_symmetric_block_matrix(n, bs, seed, scale, _d(out))
+    return out
+
+
+def ref_residual_samples(n: int, seed: int):
+    rows = np.zeros(REF_SAMPLE_COUNT, dtype=np.int32)
+    cols = np.zeros(REF_SAMPLE_COUNT, dtype=np.int32)
+    cnt = ref().ref_residual_samples(n, seed, _i(rows), _i(cols))
+    return rows[:cnt], cols[:cnt]
+
+
+def ref_stress_case(n: int, bs: int, seed: int, scale: float = 1.0):
+    """The large stress test's case: its generator, a compact-only
+    Factorize() and its SampledRelativeResidual.  Returns
+    (Factorization, sampled residual)."""
+    compact = np.zeros((n, n))
+    perm = np.zeros(n, dtype=np.int32)
+    piv = np.zeros(n, dtype=np.int32)
+    res = np.zeros(1)
+    if ref().ref_stress_case(n, bs, seed, scale, _d(compact), _i(perm), _i(piv), _d(res)) != 0:
+        raise RuntimeError("reference threw")
+    return Factorization(n, bs, compact, perm, piv), float(res[0])
 
 
 def relative_residual(A: np.ndarray, f: Factorization) -> float:

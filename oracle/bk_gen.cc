@@ -5,6 +5,12 @@
 // same libstdc++.  The synthetic workload generator (SplitMix64 keyed by
 // (seed, i, j)) is an independent restatement of the formula the HIP
 // library's generator kernel implements.
+// Pins (tests/test_oracle.py): the two reference-matrix generators and the
+// residual-sample positions are bitwise equal to the reference's own
+// RandomSymmetricMatrix, RandomSymmetricBlockMatrix and ResidualSamples,
+// compiled from its test sources (oracle/ref_tests_*.cc), and to fixtures
+// those wrote; the SplitMix64 formula against an independent Python
+// evaluation.
 #include <cstddef>
 #include <cstdint>
 #include <random>

@@ -7,11 +7,26 @@
  * same order with the same association as the reference, so with
  * -ffp-contract=off the results are bitwise equal to the reference build.
  *
- * Parity pins (tests/test_oracle_*.py): bitwise equality with the
- * reference compiled from its own sources (oracle/_ref), the reference's
- * hand-checked 4x4 compact factors, LAPACK dsytrf pivots on
- * well-separated matrices, exact rational BK on tiny matrices, and the
- * unpivoted closed form on diagonally dominant matrices.
+ * Parity pins (tests/test_oracle.py), per exported function:
+ *   bko_dsytrf        bitwise equal to the reference compiled from its own
+ *                     sources (oracle/_ref): factors, perm, piv, workspace,
+ *                     block geometry; golden vectors; the reference's 4x4
+ *                     KAT; LAPACK dsytrf pivots; exact rational BK on tiny
+ *                     matrices; the unpivoted closed form.
+ *   bko_dsytrf_omp    bitwise equal to the reference's OpenMP build at the
+ *                     extended stress sizes (n = 400..1000) and on the
+ *                     golden stress case.
+ *   bko_relative_residual   equal to the reference's RelativeError of
+ *                     ReconstructOriginal() on the same cases.
+ *   bko_sampled_residual, bko_reconstructed_entry   equal to the stress
+ *                     test's SampledRelativeResidual (compiled from the
+ *                     reference's test source) on the same factors, and to
+ *                     the exact formula on planted errors.
+ *   bko_extract_ld    through the full product L D L^T against the pinned
+ *                     sampled residual; closed form on dominant matrices.
+ *   bko_dsytrs        numpy's solve and the backward-error bound.
+ *   bko_kernel_*      bitwise equal to the reference's kernels::* on
+ *                     strided shapes, exact-zero pivots and empty sizes.
  */
 #include "bk_oracle.h"
 

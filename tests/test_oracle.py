@@ -100,18 +100,48 @@ def test_golden_vectors(oracle):
 
 
 def test_golden_generators(oracle):
-    """The libstdc++-based generators reproduce the reference's inputs."""
+    """The libstdc++-based generators reproduce the inputs the reference's
+    own generators drew (fixtures written by RandomSymmetricMatrix,
+    RandomSymmetricBlockMatrix and ResidualSamples compiled from the
+    reference's test sources, tests/golden/make_golden.py)."""
     g = np.load(GOLDEN)
     meta = g["meta"]
     kinds = g["kinds"]
+    checked = 0
     for idx, (n, bs, seed, scale) in enumerate(meta):
-        if kinds[idx] == "fixed0" or str(kinds[idx]).startswith("fixed"):
+        if str(kinds[idx]).startswith("fixed"):
             continue
         n = int(n)
         A = oracle.gen_reference_symmetric(n, int(seed), float(scale))
         assert np.array_equal(A[np.triu_indices(n)], g["A_%d" % idx])
-    rows, cols = oracle.residual_samples(5000, 931300, 64)
-    assert np.array_equal(np.stack([rows, cols]), g["samples_5000_931300"])
+        checked += 1
+    assert checked >= 30
+    for t, (n, bs, seed, scale) in enumerate(g["block_meta"]):
+        n = int(n)
+        A = oracle.gen_stress_matrix(n, int(bs), int(seed), float(scale))
+        assert np.array_equal(A[np.triu_indices(n)], g["block_A_%d" % t])
+    expected = g["samples_5000_931300"]
+    rows, cols = oracle.residual_samples(5000, 931300, expected.shape[1])
+    assert expected.shape[1] == 4096
+    assert np.array_equal(np.stack([rows, cols]), expected)
+
+
+def test_golden_stress_case(oracle):
+    """One whole large-stress case as the reference's own test code ran it
+    (tests/block_ldlt_large_stress_test.cc:220-265): the oracle's compact
+    factors of the same input are bitwise the reference's, and its sampled
+    residual on them equals SampledRelativeResidual's value."""
+    g = np.load(GOLDEN)
+    n, bs, seed, scale = g["stress_meta"]
+    n, bs, seed = int(n), int(bs), int(seed)
+    A = oracle.gen_stress_matrix(n, bs, seed, float(scale))
+    iu = np.triu_indices(n)
+    for omp in (False, True):
+        f = oracle.dsytrf(A, bs, omp=omp)
+        assert np.array_equal(f.compact[iu], g["stress_U"])
+        assert np.array_equal(f.perm, g["stress_perm"])
+        assert np.array_equal(f.piv, g["stress_piv"])
+        assert oracle.sampled_residual(A, f, seed) == g["stress_sampled_residual"][0]
 
 
 def test_bitwise_against_reference_build(ref):
@@ -133,6 +163,138 @@ def test_bitwise_against_reference_build(ref):
         assert np.array_equal(f.perm, r.perm) and np.array_equal(f.piv, r.piv), (n, bs)
         assert np.array_equal(np.triu(f.workspace), np.triu(r.workspace)), (n, bs)
         assert ref.relative_residual(A, f) == r.residual
+
+
+@pytest.mark.parametrize("n,bs", [(400, 64), (500, 8), (500, 128), (1000, 128)])
+def test_omp_oracle_bitwise_against_reference_build(ref, n, bs):
+    """The OpenMP form of the oracle (used for n >= 400) against the
+    reference's OpenMP build at the extended stress sizes and block sizes
+    of tests/block_ldlt_decomposition_test.cc:45-58 (seeds as its
+    BuildStressCases, :67-89, base 920000); both parallelize element-wise
+    loops whose per-element summation order is fixed, so results are
+    bitwise equal."""
+    si = (n // 100 + bs) % 9
+    scales = [1e-12, 1e-9, 1e-6, 1e-3, 1.0, 1e3, 1e6, 1e9, 1e12]
+    seed = 920000 + 10000 * n + 100 * bs + 10 * si
+    A = ref.ref_random_symmetric_matrix(n, seed, scales[si])
+    _, r = ref.ref_time_factorize(A, bs, want_factors=True)
+    f = ref.dsytrf(A, bs, omp=True)
+    assert np.array_equal(np.triu(f.compact), np.triu(r.compact))
+    assert np.array_equal(f.perm, r.perm) and np.array_equal(f.piv, r.piv)
+    assert ref.relative_residual(A, f) < TOL
+
+
+def test_generators_against_reference_build(ref):
+    """The oracle's generators against the reference's own test helpers,
+    compiled from its test sources: RandomSymmetricMatrix
+    (tests/block_ldlt_decomposition_test.cc:148-161),
+    RandomSymmetricBlockMatrix and ResidualSamples
+    (tests/block_ldlt_large_stress_test.cc:116-136,204-218)."""
+    for n, seed, scale in ((1, 3, 1.0), (17, 920007, 1e-6), (64, 5, 1e12), (131, 99, 1.0)):
+        assert np.array_equal(ref.gen_reference_symmetric(n, seed, scale),
+                              ref.ref_random_symmetric_matrix(n, seed, scale))
+    for n, bs, seed, scale in ((1, 1, 0, 1.0), (50, 7, 931001, 1e3), (129, 128, 931002, 1.0),
+                               (200, 64, 5, 1e-9)):
+        assert np.array_equal(ref.gen_stress_matrix(n, bs, seed, scale),
+                              ref.ref_random_symmetric_block_matrix(n, bs, seed, scale))
+    for n, seed in ((1, 0), (2, 7), (5000, 931300), (10000, 940000), (65536, 12)):
+        rows, cols = ref.ref_residual_samples(n, seed)
+        r2, c2 = ref.residual_samples(n, seed, len(rows))
+        assert np.array_equal(rows, r2) and np.array_equal(cols, c2)
+
+
+@pytest.mark.parametrize("n,bs,scale", [(96, 16, 1.0), (160, 32, 1e6), (250, 64, 1e-6)])
+def test_sampled_residual_against_reference_build(ref, n, bs, scale):
+    """bko_sampled_residual equals the stress test's SampledRelativeResidual
+    (tests/block_ldlt_large_stress_test.cc:160-244) on the same factors,
+    bit for bit; also with a planted error in the factors, where the two
+    must still agree (the reference's value comes from the same factors
+    only when they are unmodified, so the planted case compares against
+    the exact formula evaluated with numpy)."""
+    seed = 931000 + n
+    f_ref, sres = ref.ref_stress_case(n, bs, seed, scale)
+    A = ref.gen_stress_matrix(n, bs, seed, scale)
+    f = ref.dsytrf(A, bs)
+    assert np.array_equal(np.triu(f.compact), np.triu(f_ref.compact))
+    assert np.array_equal(f.perm, f_ref.perm) and np.array_equal(f.piv, f_ref.piv)
+    assert ref.sampled_residual(A, f, seed) == sres
+    # a planted error in an L entry of a sampled row moves both the same way
+    rows, cols = ref.residual_samples(n, seed)
+    inv = np.argsort(f.perm)
+    i = int(inv[rows[10]])
+    p = max(0, i - 1)
+    if p < i and f.piv[p] != 2:
+        bad = ref.Factorization(n, bs, f.compact.copy(), f.perm, f.piv)
+        bad.compact[p, i] += 0.25 * max(1.0, abs(bad.compact[p, i]))
+        L, D = ref.extract_ld(bad)
+        R = L @ D @ L.T
+        rec = R[inv[rows], inv[cols]]
+        exp = A[rows, cols]
+        want = np.linalg.norm(rec - exp) / max(np.linalg.norm(exp), 1.0)
+        got = ref.sampled_residual(A, bad, seed)
+        assert got > 1e-6 and abs(got - want) <= 1e-12 * max(1.0, want)
+
+
+KERNEL_SHAPES = [(1, 1, 1), (7, 5, 3), (16, 16, 16), (33, 9, 65), (5, 40, 128), (0, 4, 4),
+                 (4, 0, 4), (4, 4, 0)]
+
+
+@pytest.mark.parametrize("rows,cols,inner", KERNEL_SHAPES)
+def test_kernel_update_trailing_against_reference_build(ref, rows, cols, inner):
+    """bko_kernel_update_trailing against kernels::UpdateTrailingBlock
+    (src/kernels.cc:67-91, non-MKL) on strided operands, bit for bit;
+    non-positive sizes leave the target untouched."""
+    rng = np.random.default_rng(rows * 10000 + cols * 100 + inner)
+    ls, ws, ts = max(rows, 1) + 3, max(cols, 1) + 5, max(cols, 1) + 2
+    left = rng.standard_normal(max(inner, 1) * ls)
+    work = rng.standard_normal(max(inner, 1) * ws)
+    left[::7] = 0.0
+    t0 = rng.standard_normal(max(rows, 1) * ts)
+    a, b = t0.copy(), t0.copy()
+    ref.lib().bko_kernel_update_trailing(rows, cols, inner, ref._d(left), ls, ref._d(work), ws,
+                                         ref._d(a), ts)
+    ref.ref().ref_kernel_update_trailing(rows, cols, inner, ref._d(left), ls, ref._d(work), ws,
+                                         ref._d(b), ts)
+    assert np.array_equal(a, b)
+    if min(rows, cols, inner) <= 0:
+        assert np.array_equal(a, t0)
+
+
+@pytest.mark.parametrize("pivot", [2.5, -1e-300, 0.0, -0.0, 1e-17, np.inf])
+@pytest.mark.parametrize("count,ss,ds", [(1, 1, 1), (9, 3, 2), (64, 1, 5), (0, 1, 1), (-2, 1, 1)])
+def test_kernel_scale_1x1_against_reference_build(ref, pivot, count, ss, ds):
+    """bko_kernel_scale_1x1 against kernels::ScaleOneByOnePivot
+    (src/kernels.cc:11-31): division, the exact-zero pivot (zeros written),
+    strides, non-positive counts (no-op); bit for bit."""
+    rng = np.random.default_rng(count + 17)
+    src = rng.standard_normal(max(count, 1) * ss)
+    src[::4] = 0.0
+    a = np.full(max(count, 1) * ds, 7.0)
+    b = a.copy()
+    ref.lib().bko_kernel_scale_1x1(count, pivot, ref._d(src), ss, ref._d(a), ds)
+    ref.ref().ref_kernel_scale_1x1(count, pivot, ref._d(src), ss, ref._d(b), ds)
+    assert np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("abc", [(2.0, 1.0, 3.0), (0.0, 1.0, 0.0), (1.0, 1.0, 1.0), (4.0, 2.0, 1.0),
+                                 (1e-200, 1e-200, 1e-200), (-3.0, 0.5, 7.0)])
+@pytest.mark.parametrize("count,s1,s2,d1,d2", [(1, 1, 1, 1, 1), (13, 2, 3, 1, 4), (0, 1, 1, 1, 1)])
+def test_kernel_scale_2x2_against_reference_build(ref, abc, count, s1, s2, d1, d2):
+    """bko_kernel_scale_2x2 against kernels::ScaleTwoByTwoPivot
+    (src/kernels.cc:33-65): the 2x2 solve, an exactly singular block
+    (det = 0: zeros written, including the underflowing det of 1e-200
+    entries), strides and an empty count; bit for bit."""
+    a_, b_, c_ = abc
+    rng = np.random.default_rng(count * 3 + 1)
+    x1 = rng.standard_normal(max(count, 1) * s1)
+    x2 = rng.standard_normal(max(count, 1) * s2)
+    A1, A2 = np.full(max(count, 1) * d1, -5.0), np.full(max(count, 1) * d2, -6.0)
+    B1, B2 = A1.copy(), A2.copy()
+    ref.lib().bko_kernel_scale_2x2(count, a_, b_, c_, ref._d(x1), s1, ref._d(x2), s2,
+                                   ref._d(A1), d1, ref._d(A2), d2)
+    ref.ref().ref_kernel_scale_2x2(count, a_, b_, c_, ref._d(x1), s1, ref._d(x2), s2,
+                                   ref._d(B1), d1, ref._d(B2), d2)
+    assert np.array_equal(A1, B1, equal_nan=True) and np.array_equal(A2, B2, equal_nan=True)
 
 
 def test_reference_lower_input(ref):
