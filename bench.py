@@ -27,6 +27,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from blockedbunchkaufmanldlt_amd import workloads  # noqa: E402
+
 PEAK_FP64 = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
 PEAK_FP32 = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
 PEAK_HBM = 8.0e12     # bytes/s
@@ -34,27 +36,6 @@ PEAK_HBM = 8.0e12     # bytes/s
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
-
-
-def c3_matrix(torch, n, seed, lda, kappa=1e6):
-    """A = Q diag(lambda) Q^T, Q from QR of a seeded Gaussian (on the GPU)."""
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    G = torch.randn((n, n), generator=gen, dtype=torch.float64, device="cuda")
-    Q, _ = torch.linalg.qr(G)
-    del G
-    i = torch.arange(n, dtype=torch.float64, device="cuda")
-    lam = torch.pow(torch.tensor(10.0, dtype=torch.float64, device="cuda"),
-                    -math.log10(kappa) * i / max(n - 1, 1))
-    sgn = torch.randint(0, 2, (n,), generator=gen, device="cuda").to(torch.float64) * 2 - 1
-    A = (Q * (lam * sgn)) @ Q.T.contiguous()
-    del Q
-    A = 0.5 * (A + A.T)
-    out = torch.zeros((n, lda), dtype=torch.float64, device="cuda")
-    out[:, :n] = A
-    del A
-    b = torch.randn((n, 1), generator=gen, dtype=torch.float64, device="cuda")
-    return out, b
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "factor_fp64_n16384_pmc_traffic.md")
@@ -71,30 +52,11 @@ def measured_traffic():
         return {}
 
 
-def householder_eig(torch, n, kappa, gen, reflectors=4):
-    """A = U diag(lambda) U^T with U a product of random Householder
-    reflectors (as LAPACK's dlagsy builds test matrices): exact eigenvalues
-    |lambda| log-spaced over [1/kappa, 1] with random signs."""
-    lam = torch.pow(torch.tensor(10.0, dtype=torch.float64, device="cuda"),
-                    -math.log10(kappa) * torch.arange(n, dtype=torch.float64, device="cuda")
-                    / max(n - 1, 1))
-    sgn = torch.randint(0, 2, (n,), generator=gen, device="cuda").to(torch.float64) * 2 - 1
-    A = torch.diag(lam * sgn)
-    for _ in range(reflectors):
-        v = torch.randn(n, generator=gen, dtype=torch.float64, device="cuda")
-        v = v / torch.linalg.norm(v)
-        Av = A @ v
-        A = A - 2 * torch.outer(v, Av) - 2 * torch.outer(Av, v) + 4 * torch.dot(v, Av) * torch.outer(v, v)
-    return 0.5 * (A + A.T)
-
-
 def bench_batched_small(torch, B, h, reps=5, seed=4096):
     """4096 matrices, n uniform in [16, 256], entries uniform [-1, 1]: one
     persistent launch replayed from a HIP graph, FP64 and FP32; panel
     widths / team sizes from the autotuner (tuning time reported apart)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    ns = [int(x) for x in rng.integers(16, 257, size=4096)]
+    ns = workloads.c4_sizes(seed)
     flops = sum(n ** 3 / 3.0 for n in ns)
     out = {"workload": "4096 matrices, n uniform in [16, 256], entries uniform [-1, 1]",
            "flops": flops}
@@ -138,18 +100,7 @@ def bench_batched_mixed(torch, B, h, reps=2, seed=1024):
     """1024 matrices, n uniform in [256, 2048], condition 10^U[4,7]
     (Householder eigen-construction), 4 right-hand sides N(0, 1): binary32
     batched factorization + binary64 refinement to backward error 1e-12."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    ns = [int(x) for x in rng.integers(256, 2049, size=1024)]
-    kap = 10.0 ** rng.uniform(4, 7, size=1024)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    mats = []
-    for n, k in zip(ns, kap):
-        T = torch.empty((n, B.padded_ld(n)), dtype=torch.float64, device="cuda")
-        T[:, :n] = householder_eig(torch, n, k, gen)
-        mats.append(T)
-    Bs = [torch.randn((n, 4), generator=gen, dtype=torch.float64, device="cuda") for n in ns]
+    ns, mats, Bs = workloads.c5_batch(torch, B.padded_ld, seed)
     Xs = [torch.zeros((n, 4), dtype=torch.float64, device="cuda") for n in ns]
     torch.cuda.synchronize()
     best = None
@@ -226,7 +177,7 @@ def main():
     lda = B.padded_ld(n)
     h = B.Handle()
     log(f"generating n={n} eigenvalue-constructed matrix")
-    A0, b = c3_matrix(torch, n, args.seed, lda)
+    A0, b = workloads.c3_matrix(torch, n, args.seed, lda)
     torch.cuda.synchronize()
 
     # ---- timed FP64 factorizations on resident copies -------------------

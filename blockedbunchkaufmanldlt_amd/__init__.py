@@ -77,6 +77,7 @@ def lib() -> ctypes.CDLL:
     L.bbk_batched_get_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_set_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_tune.argtypes = [vp, i32, i32, ip]
+    L.bbk_batched_set_grid_limit.argtypes = [vp, i32]
     L.bbk_dsysv_mixed_batched.argtypes = [vp, i32, ip, i32, vp, vp, vp, vp, vp, vp, f64, i32,
                                           vp, vp]
     L.bbk_batched_get_team_sizes.argtypes = [vp, i32, ip]
@@ -389,6 +390,11 @@ class Handle:
         arr = (ctypes.c_int * 8)(*sizes)
         self.check(lib().bbk_batched_set_team_sizes(self.h, eb, arr), "batched_set_team_sizes")
 
+    def set_batched_grid_limit(self, max_workgroups: int):
+        """Cap the persistent batched kernels' workgroups (0 = no cap)."""
+        self.check(lib().bbk_batched_set_grid_limit(self.h, max_workgroups),
+                   "batched_set_grid_limit")
+
     def batched_tune(self, dtype, max_n: int):
         eb = 8 if dtype == self._torch.float64 else 4
         arr = (ctypes.c_int * 8)()
@@ -401,7 +407,9 @@ class Handle:
         dtype float32); returns a list of (n, n + pad) tensors."""
         torch = self._torch
         dtype = dtype or torch.float64
-        mats = [torch.empty((n, n + pad), dtype=dtype, device="cuda") for n in ns]
+        # zero-filled, so the padding columns and the unused lower triangle
+        # hold defined values (the generator writes the n x n block only)
+        mats = [torch.zeros((n, n + pad), dtype=dtype, device="cuda") for n in ns]
         dn = torch.tensor(list(ns), dtype=torch.int32, device="cuda")
         aptr = torch.tensor([m.data_ptr() for m in mats], dtype=torch.int64, device="cuda")
         lds = torch.tensor([m.stride(0) for m in mats], dtype=torch.int64, device="cuda")

@@ -621,7 +621,8 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
     if (need > lds_elems) stride = std::max(stride, need);
   }
   stride = (stride + 63) / 64 * 64;
-  const int grid = batched_grid(h, std::max(count, 1), elem_bytes, nt);
+  int grid = batched_grid(h, std::max(count, 1), elem_bytes, nt);
+  if (h->batch_grid_limit > 0) grid = std::min(grid, h->batch_grid_limit);
   const size_t ints = 2 * static_cast<size_t>(count) + 64;
   int rc = h->reserve_buf(&p.buf, &p.buf_bytes, ints * sizeof(int), "batch schedule");
   if (rc != BBK_OK) return rc;
@@ -658,9 +659,8 @@ static bool plan_matches(const BatchPlan& p, int count, const int* h_n, int elem
 
 // Queue reset (fill kernel: graph-capture safe) + the persistent launch.
 template <typename T>
-static int lockstep_launch(Handle* h, T* const* A, const int64_t* lda, int* const* perm,
-                           int* const* piv) {
-  const BatchPlan& p = h->batch;
+static int lockstep_launch(Handle* h, const BatchPlan& p, T* const* A, const int64_t* lda,
+                           int* const* perm, int* const* piv) {
   int* base = reinterpret_cast<int*>(p.buf);
   const size_t ints = p.count + 4 * (size_t)p.count + (size_t)p.steps * p.step_stride + 64;
   StepArgs<T> a;
@@ -690,11 +690,10 @@ static int lockstep_launch(Handle* h, T* const* A, const int64_t* lda, int* cons
 }
 
 template <typename T>
-int batched_launch(Handle* h, T* const* A, const int64_t* lda, int* const* perm,
-                   int* const* piv) {
-  const BatchPlan& p = h->batch;
+int batched_launch(Handle* h, const BatchPlan& p, T* const* A, const int64_t* lda,
+                   int* const* perm, int* const* piv) {
   if (p.count == 0) return BBK_OK;
-  if (p.lockstep) return lockstep_launch<T>(h, A, lda, perm, piv);
+  if (p.lockstep) return lockstep_launch<T>(h, p, A, lda, perm, piv);
   int* base = reinterpret_cast<int*>(p.buf);
   BatchArgs<T> a;
   a.count = p.count;
@@ -735,7 +734,7 @@ int batched_sytrf(Handle* h, int count, const int* h_n, T* const* A, const int64
     int rc = batched_prepare(h, count, h_n, eb);
     if (rc != BBK_OK) return rc;
   }
-  return batched_launch<T>(h, A, lda, perm, piv);
+  return batched_launch<T>(h, h->batch, A, lda, perm, piv);
 }
 
 template int batched_sytrf<double>(Handle*, int, const int*, double* const*, const int64_t*,
@@ -846,7 +845,7 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
         e = hipMemcpyAsync(work, src, elems * eb, hipMemcpyDeviceToDevice, h->stream);
         if (e != hipSuccess) { rc = h->hip_fail(e, "tuner copy"); break; }
         hipEventRecord(e0, h->stream);
-        rc = batched_launch<T>(h, reinterpret_cast<T* const*>(dA), dL,
+        rc = batched_launch<T>(h, h->batch, reinterpret_cast<T* const*>(dA), dL,
                                reinterpret_cast<int* const*>(dP), reinterpret_cast<int* const*>(dV));
         hipEventRecord(e1, h->stream);
         e = hipEventSynchronize(e1);
@@ -875,11 +874,46 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
   return rc;
 }
 
-// Capture one batched factorization (queue reset + persistent kernel) on
-// the handle's stream into a graph.
+// A private copy of the handle's current plan for a graph: its own
+// schedule buffer (copied from the handle's) and its own W slices, so
+// later batched calls, re-plans or tuning on the handle cannot change what
+// a captured graph reads.  Freed by release_plan.
+static int clone_plan(Handle* h, BatchPlan* out) {
+  const BatchPlan& p = h->batch;
+  *out = p;
+  out->buf = nullptr;
+  out->ws = nullptr;
+  hipError_t e = hipSuccess;
+  if (p.buf_bytes > 0) {
+    e = hipMalloc(&out->buf, p.buf_bytes);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out->buf, p.buf, p.buf_bytes, hipMemcpyDeviceToDevice, h->stream);
+  }
+  if (e == hipSuccess && p.ws_bytes > 0) e = hipMalloc(&out->ws, p.ws_bytes);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    release_plan(out);
+    return h->hip_fail(e, "graph plan copy");
+  }
+  return BBK_OK;
+}
+
+void release_plan(BatchPlan* p) {
+  if (p->buf) (void)hipFree(p->buf);
+  if (p->ws) (void)hipFree(p->ws);
+  p->buf = nullptr;
+  p->ws = nullptr;
+  p->buf_bytes = p->ws_bytes = 0;
+}
+
+// Capture one batched factorization (queue reset + persistent kernel, or
+// the lockstep launches) on the handle's stream into a graph that reads
+// only the private plan `gp` (filled here; the caller keeps it alive as
+// long as the graph and releases it after).
 template <typename T>
 int batched_graph(Handle* h, int count, const int* h_n, T* const* A, const int64_t* lda,
-                  int* const* perm, int* const* piv, hipGraph_t* g, hipGraphExec_t* x) {
+                  int* const* perm, int* const* piv, hipGraph_t* g, hipGraphExec_t* x,
+                  BatchPlan* gp) {
   const int eb = sizeof(T);
   if (count < 0) return -2;
   if (count > 0 && (!h_n || !A || !lda || !perm || !piv)) return -3;
@@ -887,20 +921,26 @@ int batched_graph(Handle* h, int count, const int* h_n, T* const* A, const int64
     int rc = batched_prepare(h, count, h_n, eb);
     if (rc != BBK_OK) return rc;
   }
+  int rc = clone_plan(h, gp);
+  if (rc != BBK_OK) return rc;
   hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return h->hip_fail(e, "begin capture");
-  int rc = batched_launch<T>(h, A, lda, perm, piv);
+  if (e != hipSuccess) {
+    release_plan(gp);
+    return h->hip_fail(e, "begin capture");
+  }
+  rc = batched_launch<T>(h, *gp, A, lda, perm, piv);
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(h->stream, &graph);
-  if (rc != BBK_OK) {
+  if (rc != BBK_OK || e != hipSuccess) {
     if (graph) hipGraphDestroy(graph);
-    return rc;
+    release_plan(gp);
+    return rc != BBK_OK ? rc : h->hip_fail(e, "end capture");
   }
-  if (e != hipSuccess) return h->hip_fail(e, "end capture");
   hipGraphExec_t exec = nullptr;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     hipGraphDestroy(graph);
+    release_plan(gp);
     return h->hip_fail(e, "graph instantiate");
   }
   *g = graph;
@@ -910,10 +950,12 @@ int batched_graph(Handle* h, int count, const int* h_n, T* const* A, const int64
 
 }  // namespace bbk
 
+// A graph owns its copy of the batch plan (schedule and W slices).
 struct bbk_graph {
   hipGraph_t graph;
   hipGraphExec_t exec;
   hipStream_t stream;
+  bbk::BatchPlan plan;
 };
 
 using bbk::Handle;
@@ -985,6 +1027,15 @@ int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes, const int* t
   return BBK_OK;
 }
 
+int bbk_batched_set_grid_limit(bbk_handle_t handle, int max_workgroups) {
+  if (!handle) return -1;
+  if (max_workgroups < 0) return -2;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  h->batch_grid_limit = max_workgroups;
+  h->batch.count = -1;  // the next batched call plans again
+  return BBK_OK;
+}
+
 int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n, int* widths_out) {
   if (!handle) return -1;
   if (elem_bytes != 4 && elem_bytes != 8) return -2;
@@ -1003,9 +1054,10 @@ int bbk_graph_create_dsytrf_batched(bbk_handle_t handle, int count, const int* h
   Handle* h = reinterpret_cast<Handle*>(handle);
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
-  int rc = bbk::batched_graph<double>(h, count, h_n, d_A, d_lda, d_perm, d_piv, &g, &x);
+  bbk::BatchPlan plan;
+  int rc = bbk::batched_graph<double>(h, count, h_n, d_A, d_lda, d_perm, d_piv, &g, &x, &plan);
   if (rc != BBK_OK) return rc;
-  *graph = new bbk_graph{g, x, h->stream};
+  *graph = new bbk_graph{g, x, h->stream, plan};
   return BBK_OK;
 }
 
@@ -1018,9 +1070,10 @@ int bbk_graph_create_ssytrf_batched(bbk_handle_t handle, int count, const int* h
   Handle* h = reinterpret_cast<Handle*>(handle);
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
-  int rc = bbk::batched_graph<float>(h, count, h_n, d_A, d_lda, d_perm, d_piv, &g, &x);
+  bbk::BatchPlan plan;
+  int rc = bbk::batched_graph<float>(h, count, h_n, d_A, d_lda, d_perm, d_piv, &g, &x, &plan);
   if (rc != BBK_OK) return rc;
-  *graph = new bbk_graph{g, x, h->stream};
+  *graph = new bbk_graph{g, x, h->stream, plan};
   return BBK_OK;
 }
 
@@ -1031,8 +1084,11 @@ int bbk_graph_launch(bbk_graph_t graph) {
 
 int bbk_graph_destroy(bbk_graph_t graph) {
   if (!graph) return -1;
+  // a replay still queued must finish before its plan's buffers go
+  (void)hipStreamSynchronize(graph->stream);
   if (graph->exec) hipGraphExecDestroy(graph->exec);
   if (graph->graph) hipGraphDestroy(graph->graph);
+  bbk::release_plan(&graph->plan);
   delete graph;
   return BBK_OK;
 }

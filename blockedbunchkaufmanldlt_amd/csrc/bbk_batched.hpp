@@ -62,4 +62,7 @@ struct BatchPlan {
   std::vector<int> step_tiles;  // lockstep: trailing tiles per step
 };
 
+// Free the device buffers of a plan copy owned by a graph.
+void release_plan(BatchPlan* p);
+
 }  // namespace bbk

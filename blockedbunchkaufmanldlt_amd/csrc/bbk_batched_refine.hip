@@ -315,7 +315,14 @@ __global__ __launch_bounds__(kRT) void k_batched_refine(RefineArgs a) {
     if (qpos >= a.count) break;
     const int m = a.order[qpos];
     const int n = a.ns[m];
-    if (n <= 0) continue;
+    if (n <= 0) {
+      // an empty system is solved exactly: no correction, zero error
+      if (threadIdx.x == 0) {
+        a.iters[m] = 0;
+        a.berr[m] = 0.0;
+      }
+      continue;
+    }
     const double* A = a.A[m];
     const int64_t lda = a.lda[m];
     const float* F = a.F[m];
@@ -548,7 +555,8 @@ int batched_sysv_mixed(Handle* h, int count, const int* h_n, int nrhs,
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kRT, lds_bytes) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
-  const int grid = std::max(1, std::min(count, per_cu * cus));
+  int grid = std::max(1, std::min(count, per_cu * cus));
+  if (h->batch_grid_limit > 0) grid = std::min(grid, h->batch_grid_limit);
   switch (nrhs) {
     case 1:
       hipLaunchKernelGGL(k_batched_refine<1>, dim3(grid), dim3(kRT), lds_bytes, h->stream, ra);

@@ -44,10 +44,11 @@ int Handle::reserve(size_t bytes) {
     return hip_fail(e, "hipMalloc(workspace)");
   }
   ws_bytes = bytes;
+  ++ws_gen;
   return BBK_OK;
 }
 
-int Handle::reserve_buf(void** p, size_t* cur, size_t want, const char* what) {
+int Handle::reserve_buf(void** p, size_t* cur, size_t want, const char* what, unsigned* gen) {
   if (want <= *cur) return BBK_OK;
   if (*p) {
     hipError_t e = hipStreamSynchronize(stream);
@@ -62,6 +63,7 @@ int Handle::reserve_buf(void** p, size_t* cur, size_t want, const char* what) {
     return hip_fail(e, what);
   }
   *cur = want;
+  if (gen) ++*gen;
   return BBK_OK;
 }
 
@@ -125,18 +127,21 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
                    "matrix order exceeds the co-resident panel grid");
   FactorScratch fs;
   const size_t bytes = factor_scratch_bytes<T>(n, nullptr, nullptr);
-  void* old_ws = h->ws;
   int rc = h->reserve(bytes);
   if (rc != BBK_OK) return rc;
   factor_scratch_bytes<T>(n, &fs, reinterpret_cast<char*>(h->ws));
 
   hipStream_t st = h->stream;
   // Exchange-record tags must never repeat within the live record area:
-  // each launch takes a fresh range of 1024 tags; fresh or wrapped areas
-  // are cleared first.
-  if (h->ws != old_ws || h->epoch > (1 << 30)) {
-    e = hipMemsetAsync(h->ws, 0, h->ws_bytes, st);
-    if (e != hipSuccess) return h->hip_fail(e, "clear exchange area");
+  // each launch takes a fresh range of 1024 tags; a (re)allocated area, an
+  // area laid out for another order or type, or a wrapped tag range is
+  // cleared first (fill kernel).
+  const long layout = static_cast<long>(n) * 16 + static_cast<long>(sizeof(T));
+  if (h->ws_gen != h->ws_clear_gen || h->ws_layout != layout || h->epoch > (1 << 30)) {
+    launch_zero_u32(reinterpret_cast<unsigned*>(fs.xrec),
+                    static_cast<int>(fs.xrec_bytes / sizeof(unsigned)), st);
+    h->ws_clear_gen = h->ws_gen;
+    h->ws_layout = layout;
     h->epoch = 1024;
   }
   launch_init(n, perm, piv, fs.state, st);

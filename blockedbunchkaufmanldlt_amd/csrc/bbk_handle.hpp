@@ -20,9 +20,17 @@ struct Handle {
   // Device scratch regions, each grown on demand and kept for reuse.
   void* ws = nullptr;        // factorization: exchange area, W rows
   size_t ws_bytes = 0;
+  // Record areas are cleared whenever their buffer was (re)allocated or
+  // their layout (order, element size) changed since the last clearing:
+  // an allocation can return the same address with unrelated contents.
+  unsigned ws_gen = 0;             // bumped by every (re)allocation of ws
+  unsigned ws_clear_gen = ~0u;     // ws_gen when the exchange area was cleared
+  long ws_layout = -1;             // layout the exchange area was cleared for
   void* solve_ws = nullptr;  // triangular sweeps: flags, Z, V
   size_t solve_bytes = 0;
-  void* solve_tagged = nullptr;  // solve_ws whose record area was zeroed
+  unsigned solve_gen = 0;
+  unsigned solve_clear_gen = ~0u;
+  long solve_layout = -1;
   int solve_tag = 0;             // last record tag used by a sweep launch
   void* berr_ws = nullptr;   // residual partial sums and norms
   size_t berr_bytes = 0;
@@ -35,14 +43,18 @@ struct Handle {
   int panels = 0;
   std::vector<hipEvent_t> marks[kGroups];
   BatchPlan batch;                 // last prepared batched schedule
+  int batch_grid_limit = 0;        // > 0: cap on persistent batched workgroups
   BatchWidths batch_widths[2] = {kDefaultBatchWidthsF64, kDefaultBatchWidthsF32};
 
   ~Handle();
   int fail(int code, const char* what);
   int hip_fail(hipError_t e, const char* what);
   int reserve(size_t bytes);
-  int reserve_buf(void** p, size_t* cur, size_t want, const char* what);
-  int reserve_solve(size_t b) { return reserve_buf(&solve_ws, &solve_bytes, b, "solve"); }
+  int reserve_buf(void** p, size_t* cur, size_t want, const char* what,
+                  unsigned* gen = nullptr);
+  int reserve_solve(size_t b) {
+    return reserve_buf(&solve_ws, &solve_bytes, b, "solve", &solve_gen);
+  }
   int reserve_berr(size_t b) { return reserve_buf(&berr_ws, &berr_bytes, b, "residual"); }
   int reserve_mixed(size_t b) { return reserve_buf(&mixed_ws, &mixed_bytes, b, "refinement"); }
   int next_solve_tag() { return ++solve_tag; }

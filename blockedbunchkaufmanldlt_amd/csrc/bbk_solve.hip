@@ -1297,11 +1297,15 @@ int solve(Handle* h, int n, int nrhs, const T* F, int64_t lda, const int* perm,
   XRec* rf = reinterpret_cast<XRec*>(base);
   XRec* rb = reinterpret_cast<XRec*>(base + rec_bytes);
   T* Z = reinterpret_cast<T*>(base + 2 * rec_bytes);
-  if (h->solve_tagged != h->solve_ws || h->solve_tag > (1 << 30)) {
+  // Record areas are cleared when the buffer was (re)allocated (the same
+  // address may come back with other contents), when the order changes
+  // the layout (rb and Z move with n), or when the tags would wrap.
+  if (h->solve_gen != h->solve_clear_gen || h->solve_layout != n || h->solve_tag > (1 << 30)) {
     const int words = static_cast<int>(2 * rec_bytes / sizeof(int));
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(words, 256)), dim3(256), 0, h->stream,
                        reinterpret_cast<int*>(base), words, 0);
-    h->solve_tagged = h->solve_ws;
+    h->solve_clear_gen = h->solve_gen;
+    h->solve_layout = n;
     h->solve_tag = 0;
   }
   for (int q0 = 0; q0 < nrhs; q0 += kSolveMaxRhs) {

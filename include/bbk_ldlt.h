@@ -151,6 +151,13 @@ int bbk_batched_get_team_sizes(bbk_handle_t handle, int elem_bytes,
                                int* team_sizes);
 int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes,
                                const int* team_sizes);
+/* Cap on the number of persistent workgroups of the batched kernels
+ * (factorization and refinement); 0 (the default) = as many as fit the
+ * GPU.  Each workgroup takes matrices from the work queue until it is
+ * empty, so a cap below the batch size makes workgroups factor several
+ * matrices in turn (used by the tests to exercise that path; also a way to
+ * leave CUs to other work). */
+int bbk_batched_set_grid_limit(bbk_handle_t handle, int max_workgroups);
 /* Autotuner: for every size class up to max_n, times candidate (panel
  * width, team size) pairs on a synthetic batch of that class and keeps the
  * fastest in the handle's table (widths written to widths_out when not
@@ -160,7 +167,14 @@ int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n,
 
 /* HIP graph of one batched factorization (the queue reset and the
  * persistent kernel), captured on the handle's stream for fixed
- * arguments; bbk_graph_launch replays it on the same stream. */
+ * arguments; bbk_graph_launch replays it on the same stream.  The graph
+ * owns a private copy of the batch schedule and of the W slices it reads,
+ * so later batched calls, bbk_batched_tune or re-planning on the same
+ * handle do not affect its replays.  The device arrays passed at creation
+ * (d_A, d_lda, d_perm, d_piv and the matrices they point to) must stay
+ * valid while the graph lives; bbk_graph_destroy waits for queued replays
+ * before freeing the graph's copy.  Replays of one graph, and a replay and
+ * a direct batched call on the same matrices, must not overlap. */
 typedef struct bbk_graph* bbk_graph_t;
 int bbk_graph_create_dsytrf_batched(bbk_handle_t handle, int count,
                                     const int* h_n, double* const* d_A,
@@ -184,7 +198,10 @@ int bbk_graph_destroy(bbk_graph_t graph);
  * arrays; d_iters / d_berr: device arrays of count entries receiving the
  * corrections applied and the final backward error.  n * nrhs must be at
  * most 9600 (the right-hand sides live in LDS).  Synchronous; returns
- * BBK_ERR_NOT_CONVERGED when some matrix stayed above tol. */
+ * BBK_ERR_NOT_CONVERGED when some matrix stayed above tol.  The loop of a
+ * matrix stops at tol, after max_iter corrections or at a non-finite
+ * error; X_m then holds the last iterate and d_berr[m] its backward error
+ * (an order-0 matrix gets d_iters[m] = 0, d_berr[m] = 0). */
 int bbk_dsysv_mixed_batched(bbk_handle_t handle, int count, const int* h_n,
                             int nrhs, const double* const* d_A,
                             const int64_t* d_lda, const double* const* d_B,
