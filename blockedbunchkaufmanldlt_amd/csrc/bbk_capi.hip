@@ -67,38 +67,6 @@ int Handle::reserve_buf(void** p, size_t* cur, size_t want, const char* what, un
   return BBK_OK;
 }
 
-static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// Workspace carve-up for one factorization of order n in type T.
-struct FactorScratch {
-  XRec* xrec;
-  size_t xrec_bytes;
-  int* state;
-  int* swaps;
-  void* Wg;
-  int64_t ldw;
-};
-
-template <typename T>
-static size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off = align_up(off + bytes, 256);
-    return base ? base + o : nullptr;
-  };
-  const int64_t ldw = align_up(static_cast<size_t>(n), 64);
-  const int groups = (n + kPanelThreads - 1) / kPanelThreads;
-  FactorScratch f;
-  f.xrec_bytes = sizeof(XRec) * panel_exchange_records(groups, sizeof(T));
-  f.xrec = reinterpret_cast<XRec*>(take(f.xrec_bytes));
-  f.state = reinterpret_cast<int*>(take(sizeof(int) * PanelState::kInts));
-  f.swaps = reinterpret_cast<int*>(take(sizeof(int) * 2 * (kPanelMaxWidth + 1)));
-  f.Wg = take(sizeof(T) * (kPanelMaxWidth + 1) * static_cast<size_t>(ldw));
-  f.ldw = ldw;
-  if (fs) *fs = f;
-  return off;
-}
 
 int default_panel_width(int n, int elem_bytes) {
   (void)elem_bytes;
@@ -106,6 +74,59 @@ int default_panel_width(int n, int elem_bytes) {
   if (n <= 1024) return 64;
   return 128;
 }
+
+// The panel loop shared by every factorization entry point: queues the
+// init kernel, then per panel the cooperative panel kernel and the trailing
+// update (with the interchanges of the rows above the panel) on `st`.
+// fs: the carved scratch; *epoch: the next exchange-tag range (advanced by
+// 1024 per launch).  h (may be NULL) receives timing marks and the panel
+// count.
+template <typename T>
+int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
+                int max_groups, int* epoch, hipStream_t st, Handle* h) {
+  launch_init(n, perm, piv, fs.state, st);
+  int panels = 0;
+  for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
+    const int groups = (n - s_nom + kPanelThreads - 1) / kPanelThreads;
+    PanelArgs pa;
+    pa.n = n;
+    pa.b = b;
+    pa.lda = lda;
+    pa.state = fs.state;
+    pa.swaps = fs.swaps;
+    pa.perm = perm;
+    pa.piv = piv;
+    pa.xrec = fs.xrec;
+    pa.epoch = *epoch;
+    *epoch += 1024;
+    pa.Wg = fs.Wg;
+    pa.ldw = fs.ldw;
+    // Small panels run on one XCD: the grid is spread over the 8 XCDs and
+    // only every 8th workgroup works, when all of them fit co-resident.
+    pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
+    if (h) h->timing_mark(Handle::kPanel, true);
+    hipError_t e = launch_panel<T>(A, pa, groups * pa.xcd_stride, st);
+    if (e != hipSuccess) return h ? h->hip_fail(e, "panel launch") : BBK_ERR_HIP;
+    if (h) h->timing_mark(Handle::kPanel, false);
+    // Rows above this panel: at most s_nom + panels (each earlier panel
+    // can widen by one column); their interchanges run in the same launch
+    // as the trailing update.
+    const int rows_bound = s_nom > 0 ? std::min(n, s_nom + panels) : 0;
+    if (h) h->timing_mark(Handle::kUpdate, true);
+    launch_trailing_update<T>(A, lda, reinterpret_cast<const T*>(fs.Wg), fs.ldw, fs.state,
+                              fs.swaps, n, n - s_nom - b, rows_bound, st);
+    if (h) h->timing_mark(Handle::kUpdate, false);
+  }
+  if (h && h->timing) h->panels += panels;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return h ? h->hip_fail(e, "kernel launch") : BBK_ERR_HIP;
+  return BBK_OK;
+}
+
+template int factor_loop<double>(int, int, double*, int64_t, int*, int*, const FactorScratch&,
+                                 int, int*, hipStream_t, Handle*);
+template int factor_loop<float>(int, int, float*, int64_t, int*, int*, const FactorScratch&,
+                                int, int*, hipStream_t, Handle*);
 
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
@@ -144,43 +165,7 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     h->ws_layout = layout;
     h->epoch = 1024;
   }
-  launch_init(n, perm, piv, fs.state, st);
-  int panels = 0;
-  for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
-    const int groups = (n - s_nom + kPanelThreads - 1) / kPanelThreads;
-    PanelArgs pa;
-    pa.n = n;
-    pa.b = b;
-    pa.lda = lda;
-    pa.state = fs.state;
-    pa.swaps = fs.swaps;
-    pa.perm = perm;
-    pa.piv = piv;
-    pa.xrec = fs.xrec;
-    pa.epoch = h->epoch;
-    h->epoch += 1024;
-    pa.Wg = fs.Wg;
-    pa.ldw = fs.ldw;
-    // Small panels run on one XCD: the grid is spread over the 8 XCDs and
-    // only every 8th workgroup works, when all of them fit co-resident.
-    pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
-    h->timing_mark(Handle::kPanel, true);
-    e = launch_panel<T>(A, pa, groups * pa.xcd_stride, st);
-    if (e != hipSuccess) return h->hip_fail(e, "panel launch");
-    h->timing_mark(Handle::kPanel, false);
-    // Rows above this panel: at most s_nom + panels (each earlier panel
-    // can widen by one column); their interchanges run in the same launch
-    // as the trailing update.
-    const int rows_bound = s_nom > 0 ? std::min(n, s_nom + panels) : 0;
-    h->timing_mark(Handle::kUpdate, true);
-    launch_trailing_update<T>(A, lda, reinterpret_cast<const T*>(fs.Wg), fs.ldw,
-                              fs.state, fs.swaps, n, n - s_nom - b, rows_bound, st);
-    h->timing_mark(Handle::kUpdate, false);
-  }
-  if (h->timing) h->panels += panels;
-  e = hipGetLastError();
-  if (e != hipSuccess) return h->hip_fail(e, "kernel launch");
-  return BBK_OK;
+  return factor_loop<T>(n, b, A, lda, perm, piv, fs, max_groups, &h->epoch, st, h);
 }
 
 template int factorize<double>(Handle*, int, int, double*, int64_t, int*, int*);

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "bbk_batched.hpp"
+#include "bbk_kernels.hpp"
 
 namespace bbk {
 
@@ -68,6 +69,11 @@ int default_panel_width(int n, int elem_bytes);
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
               int* piv);
+// The factorization's launch sequence on stream st with carved scratch fs
+// (h: timing and error text, may be NULL).
+template <typename T>
+int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
+                int max_groups, int* epoch, hipStream_t st, Handle* h);
 // Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
 // refinement's solves instead of substitution, with F's strictly lower
 // triangle holding the mirror of the upper one; NULL substitutes (F upper

@@ -36,6 +36,40 @@ void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                             const int* state, const int* swaps, int n, int m_bound,
                             int rows_bound, hipStream_t stream);
 void launch_init(int n, int* perm, int* piv, int* state, hipStream_t stream);
+
+// Scratch carve-up for one factorization of order n in type T: exchange
+// records, panel state, the interchange list and the flushed W rows.  A
+// carve for order n serves every order <= n.
+struct FactorScratch {
+  XRec* xrec;
+  size_t xrec_bytes;
+  int* state;
+  int* swaps;
+  void* Wg;
+  int64_t ldw;
+};
+
+template <typename T>
+inline size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = (off + bytes + 255) / 256 * 256;
+    return base ? base + o : nullptr;
+  };
+  const int64_t ldw = (static_cast<int64_t>(n) + 63) / 64 * 64;
+  const int groups = (n + kPanelThreads - 1) / kPanelThreads;
+  FactorScratch f;
+  f.xrec_bytes = sizeof(XRec) * panel_exchange_records(groups, sizeof(T));
+  f.xrec = reinterpret_cast<XRec*>(take(f.xrec_bytes));
+  f.state = reinterpret_cast<int*>(take(sizeof(int) * PanelState::kInts));
+  f.swaps = reinterpret_cast<int*>(take(sizeof(int) * 2 * (kPanelMaxWidth + 1)));
+  f.Wg = take(sizeof(T) * (kPanelMaxWidth + 1) * static_cast<size_t>(ldw));
+  f.ldw = ldw;
+  if (fs) *fs = f;
+  return off;
+}
+
 void launch_zero_u32(unsigned* p, int count, hipStream_t stream);
 
 }  // namespace bbk

@@ -81,6 +81,22 @@ int bbk_ssytrf(bbk_handle_t handle, int n, int nb, float* d_A, int64_t lda,
 int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
                     int64_t lda, int* perm, int* piv);
 
+/* ---- factorization that allocates nothing -----------------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): bbk_dsytrf's factorization (same
+ * kernels, same results) with its scratch in a fixed device array of
+ * 16.1 MiB that every GPU the library runs on holds until the process
+ * ends; calls no hipMalloc/hipFree and needs no handle.  n <= 16384
+ * (larger orders return BBK_ERR_UNSUPPORTED).  stream: a non-null stream
+ * that does not synchronize with the null stream (cooperative launches);
+ * the call is asynchronous on it.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race and must be ordered by the
+ * caller.  Returns BBK_OK, -i for an invalid argument i (1-based: n = 1),
+ * BBK_ERR_UNSUPPORTED or BBK_ERR_HIP. */
+int bbk_dsytrf_noalloc(int n, int nb, double* d_A, int64_t lda, int* d_perm,
+                       int* d_piv, void* stream);
+
 /* ---- solves (new; no reference counterpart) -------------------------
  * Solve A X = B with compact factors from bbk_?sytrf.  B: n x nrhs,
  * row-major with leading dimension ldb >= nrhs; overwritten by X. */
