@@ -90,6 +90,7 @@ def bench_batched_small(torch, B, h, reps=5, seed=4096):
                      "hbm_GB/s": round(alg_bytes / ms / 1e6, 1),
                      "frac_hbm": round(alg_bytes / (ms * 1e-3) / PEAK_HBM, 5),
                      "widths": h.batched_widths(dt), "team_sizes": h.batched_team_sizes(dt),
+                     "update_tiles": h.batched_tiles(dt),
                      "tune_s": round(tune_s, 3)}
         del src, work, perms, pivs
     torch.cuda.empty_cache()

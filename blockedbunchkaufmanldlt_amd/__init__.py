@@ -78,6 +78,8 @@ def lib() -> ctypes.CDLL:
     L.bbk_batched_set_widths.argtypes = [vp, i32, ip]
     L.bbk_batched_tune.argtypes = [vp, i32, i32, ip]
     L.bbk_batched_set_grid_limit.argtypes = [vp, i32]
+    L.bbk_batched_get_tiles.argtypes = [vp, i32, ip]
+    L.bbk_batched_set_tiles.argtypes = [vp, i32, ip]
     L.bbk_dsysv_mixed_batched.argtypes = [vp, i32, ip, i32, vp, vp, vp, vp, vp, vp, f64, i32,
                                           vp, vp]
     L.bbk_batched_get_team_sizes.argtypes = [vp, i32, ip]
@@ -389,6 +391,17 @@ class Handle:
         eb = 8 if dtype == self._torch.float64 else 4
         arr = (ctypes.c_int * 8)(*sizes)
         self.check(lib().bbk_batched_set_team_sizes(self.h, eb, arr), "batched_set_team_sizes")
+
+    def batched_tiles(self, dtype):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)()
+        self.check(lib().bbk_batched_get_tiles(self.h, eb, arr), "batched_get_tiles")
+        return list(arr)
+
+    def set_batched_tiles(self, dtype, tiles: Sequence[int]):
+        eb = 8 if dtype == self._torch.float64 else 4
+        arr = (ctypes.c_int * 8)(*tiles)
+        self.check(lib().bbk_batched_set_tiles(self.h, eb, arr), "batched_set_tiles")
 
     def set_batched_grid_limit(self, max_workgroups: int):
         """Cap the persistent batched kernels' workgroups (0 = no cap)."""

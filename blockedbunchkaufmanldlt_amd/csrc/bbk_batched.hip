@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bbk_ldlt.h"
@@ -51,6 +52,7 @@ struct BatchArgs {
   T* ws;                // global W slices (gridDim.x of ws_stride elements)
   int64_t ws_stride;
   int lds_elems;        // dynamic LDS capacity in elements of T
+  int lds_bytes;        // resident kernel: dynamic LDS arena bytes
   BatchWidths widths;
 };
 
@@ -135,13 +137,69 @@ __device__ __forceinline__ T panel_rows_dot(const T* l, const T* W, int n, int k
   return (a0 + a1) + (a2 + a3);
 }
 
+// Storage views of one matrix for the batched factorization; element
+// (i, j) with i <= j (upper triangle).
+//   GlobalMat  row-major in global memory (leading dimension lda).
+//   LdsMat     packed in LDS: row i holds columns i..n-1 from offset
+//              i n - i (i - 1) / 2.
+//   HybridMat  rows >= R0 packed in LDS as above (the trailing square
+//              [R0, n)^2 on chip), rows < R0 in global memory.  In a panel
+//              that starts at or after R0, the interchanges that reach the
+//              rows < R0 are recorded (defer) and applied after the last
+//              panel: those rows belong to finished panels, which nothing
+//              reads again (L rows come from W, the trailing update reads
+//              only the current panel's rows).  first_immediate_row(s): the
+//              first row an interchange in the panel starting at s updates
+//              at once.
+template <typename T>
+struct GlobalMat {
+  T* g;
+  int64_t lda;
+  __device__ __forceinline__ T& at(int i, int j) const { return g[(int64_t)i * lda + j]; }
+  __device__ __forceinline__ int first_immediate_row(int) const { return 0; }
+  __device__ __forceinline__ void defer(int, int) const {}
+};
+template <typename T>
+struct LdsMat {
+  T* s;
+  int n;
+  __device__ __forceinline__ T& at(int i, int j) const {
+    return s[i * n - ((i * (i - 1)) >> 1) + (j - i)];
+  }
+  __device__ __forceinline__ int first_immediate_row(int) const { return 0; }
+  __device__ __forceinline__ void defer(int, int) const {}
+};
+template <typename T>
+struct HybridMat {
+  T* g;
+  int64_t lda;
+  T* s;
+  int n, R0;
+  int* dlist;   // deferred interchanges (a, b), in order
+  int* dcount;
+  __device__ __forceinline__ T& at(int i, int j) const {
+    if (i < R0) return g[(int64_t)i * lda + j];
+    const int r = i - R0, m = n - R0;
+    return s[r * m - ((r * (r - 1)) >> 1) + (j - i)];
+  }
+  __device__ __forceinline__ int first_immediate_row(int s) const { return s >= R0 ? R0 : 0; }
+  // called by one thread
+  __device__ __forceinline__ void defer(int a, int b) const {
+    const int c = *dcount;
+    dlist[2 * c] = a;
+    dlist[2 * c + 1] = b;
+    *dcount = c + 1;
+  }
+};
+
 // Panel P of one matrix, whole team of NT threads, starting at column s
 // (one column early when the previous panel deferred a 2x2): Bunch-Kaufman
 // steps with the panel rows W (row stride n; rows 0..bw = panel rows, row
-// bw + 1 = the candidate row C).  Returns the panel's end e (the nominal
-// end, or the column of a deferred 2x2).
-template <typename T, int NT, bool ACC4>
-__device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
+// bw + 1 = the candidate row C).  F: the matrix's storage view (GlobalMat,
+// LdsMat or HybridMat); perm/piv in global memory or LDS.  Returns the
+// panel's end e (the nominal end, or the column of a deferred 2x2).
+template <typename T, int NT, bool ACC4, typename Mat>
+__device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
                             int* __restrict__ piv, int bw, int P, int s, T* W, T* Lk, T* Lr,
                             ScaleTab& st, double* rv, int* ri) {
   const int tid = threadIdx.x;
@@ -157,7 +215,7 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
     int arg = n;
     T* Wk = W + (int64_t)kk * n;
     for (int j = k + tid; j < n; j += NT) {
-      const T w = F[(int64_t)k * lda + j] - panel_rows_dot<T, ACC4>(Lk, W, n, kk, j);
+      const T w = F.at(k, j) - panel_rows_dot<T, ACC4>(Lk, W, n, kk, j);
       Wk[j] = w;
       const double a = fabs(static_cast<double>(w));
       if (j > k && a > best) {
@@ -178,7 +236,7 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
       __syncthreads();
       double sig = 0.0;
       for (int i = k + tid; i < n; i += NT) {
-        const T base = i < r ? F[(int64_t)i * lda + r] : F[(int64_t)r * lda + i];
+        const T base = i < r ? F.at(i, r) : F.at(r, i);
         const T cv = base - panel_rows_dot<T, ACC4>(Lr, W, n, kk, i);
         C[i] = cv;
         const double a = fabs(static_cast<double>(cv));
@@ -200,18 +258,18 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
       __syncthreads();
       if (kind == -1) {
         // interchange k <-> r (:414-498); row k of W becomes the candidate
-        for (int j = r + 1 + tid; j < n; j += NT)
-          swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)r * lda + j]);
-        for (int j = k + 1 + tid; j < r; j += NT)
-          swap_elems(&F[(int64_t)k * lda + j], &F[(int64_t)j * lda + r]);
-        for (int i = tid; i < k; i += NT)
-          swap_elems(&F[(int64_t)i * lda + k], &F[(int64_t)i * lda + r]);
+        for (int j = r + 1 + tid; j < n; j += NT) swap_elems(&F.at(k, j), &F.at(r, j));
+        for (int j = k + 1 + tid; j < r; j += NT) swap_elems(&F.at(k, j), &F.at(j, r));
+        // rows above k: rows the view holds off chip wait until the end
+        const int i0 = F.first_immediate_row(s);
+        for (int i = i0 + tid; i < k; i += NT) swap_elems(&F.at(i, k), &F.at(i, r));
+        if (i0 > 0 && tid == 0) F.defer(k, r);
         for (int p = tid; p < kk; p += NT)
           swap_elems(&W[(int64_t)p * n + k], &W[(int64_t)p * n + r]);
         for (int j = k + 1 + tid; j < n; j += NT)
           if (j != r) Wk[j] = C[j];
         if (tid == 0) {
-          swap_elems(&F[(int64_t)k * lda + k], &F[(int64_t)r * lda + r]);
+          swap_elems(&F.at(k, k), &F.at(r, r));
           Wk[k] = C[r];
           const int t = perm[k];
           perm[k] = perm[r];
@@ -222,12 +280,11 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
         const int k1 = k + 1;
         T* Wk1 = W + (int64_t)(kk + 1) * n;
         if (r != k1) {
-          for (int j = r + 1 + tid; j < n; j += NT)
-            swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)r * lda + j]);
-          for (int j = k + 2 + tid; j < r; j += NT)
-            swap_elems(&F[(int64_t)k1 * lda + j], &F[(int64_t)j * lda + r]);
-          for (int i = tid; i < k; i += NT)
-            swap_elems(&F[(int64_t)i * lda + k1], &F[(int64_t)i * lda + r]);
+          for (int j = r + 1 + tid; j < n; j += NT) swap_elems(&F.at(k1, j), &F.at(r, j));
+          for (int j = k + 2 + tid; j < r; j += NT) swap_elems(&F.at(k1, j), &F.at(j, r));
+          const int i0 = F.first_immediate_row(s);
+          for (int i = i0 + tid; i < k; i += NT) swap_elems(&F.at(i, k1), &F.at(i, r));
+          if (i0 > 0 && tid == 0) F.defer(k1, r);
           for (int p = tid; p < kk; p += NT)
             swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
         }
@@ -235,8 +292,8 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
           if (j != r) Wk1[j] = C[j];
         if (tid == 0) {
           if (r != k1) {
-            swap_elems(&F[(int64_t)k1 * lda + k1], &F[(int64_t)r * lda + r]);
-            swap_elems(&F[(int64_t)k * lda + k1], &F[(int64_t)k * lda + r]);
+            swap_elems(&F.at(k1, k1), &F.at(r, r));
+            swap_elems(&F.at(k, k1), &F.at(k, r));
             swap_elems(&Wk[k1], &Wk[r]);
             Wk1[r] = C[k1];
             const int t = perm[k1];
@@ -253,9 +310,9 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
       const double d = static_cast<double>(Wk[k]);
       const double den = safe_den(d);
       for (int j = k + 1 + tid; j < n; j += NT)
-        F[(int64_t)k * lda + j] = static_cast<T>(div_or_zero(static_cast<double>(Wk[j]), den));
+        F.at(k, j) = static_cast<T>(div_or_zero(static_cast<double>(Wk[j]), den));
       if (tid == 0) {
-        F[(int64_t)k * lda + k] = Wk[k];
+        F.at(k, k) = Wk[k];
         piv[k] = 1;
         st.ty[kk] = 1;
         st.c0[kk] = den;
@@ -274,13 +331,13 @@ __device__ int factor_panel(T* __restrict__ F, int64_t lda, int n, int* __restri
         double u0, u1;
         apply_2x2_nc(itl, ibr, ioff, static_cast<double>(Wk[j]), static_cast<double>(Wk1[j]),
                      &u0, &u1);
-        F[(int64_t)k * lda + j] = static_cast<T>(u0);
-        F[(int64_t)k1 * lda + j] = static_cast<T>(u1);
+        F.at(k, j) = static_cast<T>(u0);
+        F.at(k1, j) = static_cast<T>(u1);
       }
       if (tid == 0) {
-        F[(int64_t)k * lda + k] = Wk[k];
-        F[(int64_t)k * lda + k1] = Wk[k1];
-        F[(int64_t)k1 * lda + k1] = Wk1[k1];
+        F.at(k, k) = Wk[k];
+        F.at(k, k1) = Wk[k1];
+        F.at(k1, k1) = Wk1[k1];
         piv[k] = 2;
         piv[k1] = -2;
         st.ty[kk] = 2;
@@ -342,11 +399,67 @@ __device__ void simt_update(T* __restrict__ F, int64_t lda, int n, int s, int e,
   }
 }
 
-// One matrix, whole team: every panel, then its trailing update.
+// MFMA trailing update C(i, j) -= sum_{p < kp} U(p, i) W(p, j), e <= i <= j < n,
+// U(p, i) = F(s + p, i), kp = e - s (src/block_ldlt_decomposition.cc:706-744,
+// src/kernels.cc:67-91) on MFMA: each wavefront takes TS x TS output tiles
+// of the upper triangle (TS = 16: one 16x16x4 block; 32: 2 x 2 blocks
+// sharing their operands), K in steps of 4.  U is negated as it is loaded,
+// so the accumulators started from C hold C - U^T W.
+template <typename T, int NT, int TS, typename Mat>
+__device__ void mfma_update(const Mat& F, int n, int s, int e, const T* W) {
+  constexpr int TB = TS / 16;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = n - e, kp = e - s;
+  if (m <= 0 || kp <= 0) return;
+  const int nt = (m + TS - 1) / TS, tiles = nt * (nt + 1) / 2;
+  for (int t = wave; t < tiles; t += NT / 64) {
+    int I, J;
+    tri_tile(t, &I, &J);
+    const int i0 = e + I * TS, j0 = e + J * TS;
+    acc_t acc[TB][TB];
+#pragma unroll
+    for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = i0 + 16 * ti + Mfma<T>::row(lane, v), j = j0 + 16 * tj + (lane & 15);
+          acc[ti][tj][v] = (i <= j && j < n) ? F.at(i, j) : T(0);
+        }
+    const int q = lane >> 4, c = lane & 15;
+    for (int k4 = 0; k4 < kp; k4 += 4) {
+      const int p = k4 + q;
+      T a[TB], b[TB];
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) {
+        const int ia = i0 + 16 * tb + c, jb = j0 + 16 * tb + c;
+        a[tb] = (p < kp && ia < n) ? -F.at(s + p, ia) : T(0);
+        b[tb] = (p < kp && jb < n) ? W[p * n + jb] : T(0);
+      }
+#pragma unroll
+      for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TB; ++tj) acc[ti][tj] = Mfma<T>::mma(a[ti], b[tj], acc[ti][tj]);
+    }
+#pragma unroll
+    for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = i0 + 16 * ti + Mfma<T>::row(lane, v), j = j0 + 16 * tj + (lane & 15);
+          if (i <= j && j < n) F.at(i, j) = acc[ti][tj][v];
+        }
+  }
+}
+
+// One matrix, whole team: every panel, then its trailing update (ts: 0 =
+// SIMT 4 x 4 register tiles, 16 or 32 = MFMA tiles per wavefront).
 template <typename T, int NT>
 __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restrict__ perm,
-                           int* __restrict__ piv, int bw, T* W, T* Lk, T* Lr, ScaleTab& st,
-                           double* rv, int* ri) {
+                           int* __restrict__ piv, int bw, int ts, T* W, T* Lk, T* Lr,
+                           ScaleTab& st, double* rv, int* ri) {
   for (int i = threadIdx.x; i < n; i += NT) {
     perm[i] = i;
     piv[i] = 1;
@@ -356,9 +469,14 @@ __device__ void factor_one(T* __restrict__ F, int64_t lda, int n, int* __restric
   for (int P = 0; s < n; ++P) {
     // persistent teams: the matrix's W in LDS unless it is large (binary64
     // past about 280): four partial sums for binary64 only
-    const int e = factor_panel<T, NT, sizeof(T) == 8>(F, lda, n, perm, piv, bw, P, s, W, Lk, Lr,
-                                                       st, rv, ri);
-    simt_update<T, NT>(F, lda, n, s, e, W);
+    const int e = factor_panel<T, NT, sizeof(T) == 8>(GlobalMat<T>{F, lda}, n, perm, piv, bw, P,
+                                                       s, W, Lk, Lr, st, rv, ri);
+    if (ts == 16)
+      mfma_update<T, NT, 16>(GlobalMat<T>{F, lda}, n, s, e, W);
+    else if (ts == 32)
+      mfma_update<T, NT, 32>(GlobalMat<T>{F, lda}, n, s, e, W);
+    else
+      simt_update<T, NT>(F, lda, n, s, e, W);
     __syncthreads();
     s = e;
   }
@@ -382,10 +500,413 @@ __global__ __launch_bounds__(NT) void k_batched_sytrf(BatchArgs<T> a) {
     const int m = a.order[q];
     const int n = a.ns[m];
     if (n <= 0) continue;
-    const int bw = a.widths.w[batch_class(n)];
+    const int c = batch_class(n);
+    const int bw = a.widths.w[c];
     const int64_t need = (int64_t)(bw + 2) * n;
     T* W = need <= a.lds_elems ? lds : a.ws + (int64_t)blockIdx.x * a.ws_stride;
-    factor_one<T, NT>(a.A[m], a.lda[m], n, a.perm[m], a.piv[m], bw, W, Lk, Lr, st, rv, ri);
+    factor_one<T, NT>(a.A[m], a.lda[m], n, a.perm[m], a.piv[m], bw, a.widths.ts[c], W, Lk, Lr, st,
+                      rv, ri);
+  }
+}
+
+// Panel P of one matrix held whole in LDS (packed upper triangle S, order
+// n <= 256 = NT: thread j owns column j for the whole factorization).  The
+// same Bunch-Kaufman steps and arithmetic as factor_panel
+// (src/block_ldlt_decomposition.cc:80-120,122-704), fused so that a pivot
+// meets at most three barriers:
+//   A  row k: W(k, j) = S(k, j) - sum_p L(k, p) W(p, j), the first argmax
+//      of |W(k, j)|, j > k (one reduction);
+//   B  (when the test needs it) candidate row r and its maximum sigma over
+//      i != r (one reduction);
+//   C  interchange and scaling in one pass: the interchange's writes into
+//      the pivot row(s) are dead (the scaling overwrites them), so only the
+//      moves into row/column r, the earlier rows' column swaps and the W
+//      column swaps remain, each element touched by one thread.
+// L(k, s + p) is read straight from the finished rows, S(s + p, k): the
+// stored U values are the scaled W values the reference uses.  Values a
+// thread needs from another column (W(k, k) with its sign, C(k), C(k+1),
+// C(r), the old W(k, k+1) and W(k, r)) go through LDS slots written before
+// the reduction barrier that precedes their reads.
+struct PivotSlots {
+  double wkk, ck, ck1, crr;
+};
+
+template <typename T>
+__device__ __forceinline__ int pk(int n, int i, int j) {
+  return i * n - ((i * (i - 1)) >> 1) + (j - i);
+}
+
+// sum_{p < kk} S(s + p, col) W(p, j), four interleaved partial sums for
+// binary64 (as panel_rows_dot), one running sum for binary32.
+template <typename T, bool ACC4>
+__device__ __forceinline__ T lds_rows_dot(const T* S, int n, int s, int col, const T* W, int kk,
+                                          int j) {
+  int o = pk<T>(n, s, col);
+  int step = n - s - 1;  // pk(i + 1, col) - pk(i, col) = n - i - 1
+  if constexpr (!ACC4) {
+    T acc = T(0);
+    for (int p = 0; p < kk; ++p) {
+      acc += S[o] * W[p * n + j];
+      o += step;
+      --step;
+    }
+    return acc;
+  }
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  int p = 0;
+  for (; p + 4 <= kk; p += 4) {
+    const int o1 = o + step, o2 = o1 + step - 1, o3 = o2 + step - 2;
+    a0 += S[o] * W[p * n + j];
+    a1 += S[o1] * W[(p + 1) * n + j];
+    a2 += S[o2] * W[(p + 2) * n + j];
+    a3 += S[o3] * W[(p + 3) * n + j];
+    o = o3 + step - 3;
+    step -= 4;
+  }
+  for (; p < kk; ++p) {
+    a0 += S[o] * W[p * n + j];
+    o += step;
+    --step;
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
+template <typename T, bool ACC4>
+__device__ int factor_panel_lds(T* S, int n, int* perm, int* piv, int bw, int P, int s, T* W,
+                                PivotSlots& ps, double* rv, int* ri, int& par) {
+  const int j = threadIdx.x;  // owned column
+  T* C = W + (bw + 1) * n;
+  int e = min((P + 1) * bw, n);
+  int kk = 0;
+  while (s + kk < e) {
+    const int k = s + kk;
+    T* Wk = W + kk * n;
+    // ---- A: row k -------------------------------------------------------
+    T w = T(0);
+    double v = 0.0;
+    int vi = INT_MAX;
+    if (j >= k && j < n) {
+      w = S[pk<T>(n, k, j)] - lds_rows_dot<T, ACC4>(S, n, s, k, W, kk, j);
+      Wk[j] = w;
+      const double a = fabs(static_cast<double>(w));
+      if (j > k && a > 0.0) {  // NaN never wins (strict '>' scan, :164-182)
+        v = a;
+        vi = j;
+      }
+      if (j == k) ps.wkk = static_cast<double>(w);
+    }
+    block_argmax1(v, vi, rv, ri, par);
+    const double omega = v;
+    const int r = vi;
+    const double wkk = ps.wkk;
+    const double akk = fabs(wkk);
+    int kind = 1;
+    double ck = 0.0, ck1 = 0.0, crr = 0.0;
+    T c = T(0);
+    T wk_k1 = T(0), wk_r = T(0);
+    if (!(akk >= kAlpha * omega || omega == 0.0)) {
+      // ---- B: candidate row r (:184-367) ---------------------------------
+      wk_r = Wk[r];
+      if (k + 1 < n) wk_k1 = Wk[k + 1];
+      double sv = 0.0;
+      if (j >= k && j < n) {
+        const T base = j < r ? S[pk<T>(n, j, r)] : S[pk<T>(n, r, j)];
+        c = base - lds_rows_dot<T, ACC4>(S, n, s, r, W, kk, j);
+        C[j] = c;
+        const double a = fabs(static_cast<double>(c));
+        if (j != r) sv = (a >= 0.0) ? a : 0.0;  // NaN ignored, as std::max does
+        if (j == k) ps.ck = static_cast<double>(c);
+        if (j == k + 1) ps.ck1 = static_cast<double>(c);
+        if (j == r) ps.crr = static_cast<double>(c);
+      }
+      const double sigma = block_max1(sv, rv, par);
+      crr = ps.crr;
+      ck = ps.ck;
+      ck1 = ps.ck1;
+      kind = select_pivot_nc(akk, omega, sigma, fabs(crr));
+      if (kind == 2 && kk == e - s - 1 && e < n) {
+        e = k;  // 2x2 across the panel edge: deferred to the next panel (:589-631)
+        break;
+      }
+    }
+    // ---- C: interchange + scaling -----------------------------------------
+    if (kind == 1 || kind == 3) {
+      // ScaleOneByOnePivot (:633-656); after a candidate row that kept the
+      // pivot, W(k, r) is the candidate's value at k (:184-247)
+      const double den = safe_den(wkk);
+      if (j > k && j < n) {
+        T wj = w;
+        if (kind == 3 && j == r) {
+          wj = static_cast<T>(ck);
+          Wk[j] = wj;
+        }
+        S[pk<T>(n, k, j)] = static_cast<T>(div_or_zero(static_cast<double>(wj), den));
+      }
+      if (j == k) {
+        S[pk<T>(n, k, k)] = w;
+        piv[k] = 1;
+      }
+      kk += 1;
+    } else if (kind == -1) {
+      // interchange k <-> r (:414-498), then the candidate row is row k
+      const double den = safe_den(crr);
+      if (j < k) {
+        swap_elems(&S[pk<T>(n, j, k)], &S[pk<T>(n, j, r)]);
+        if (j < kk) swap_elems(&W[j * n + k], &W[j * n + r]);
+      } else if (j == k) {
+        S[pk<T>(n, r, r)] = S[pk<T>(n, k, k)];
+        S[pk<T>(n, k, k)] = static_cast<T>(crr);
+        Wk[k] = static_cast<T>(crr);
+        piv[k] = 1;
+        const int t = perm[k];
+        perm[k] = perm[r];
+        perm[r] = t;
+      } else if (j < n) {
+        const T old = S[pk<T>(n, k, j)];
+        if (j > r)
+          S[pk<T>(n, r, j)] = old;
+        else if (j < r)
+          S[pk<T>(n, j, r)] = old;
+        const T wj = j == r ? static_cast<T>(ck) : c;
+        Wk[j] = wj;
+        S[pk<T>(n, k, j)] = static_cast<T>(div_or_zero(static_cast<double>(wj), den));
+      }
+      kk += 1;
+    } else {
+      // 2x2 pivot at (k, k+1) after the interchange k+1 <-> r (:500-587),
+      // ScaleTwoByTwoPivot (:658-704)
+      const int k1 = k + 1;
+      T* Wk1 = W + (kk + 1) * n;
+      const bool sw = r != k1;
+      // W(k, k+1) after the interchange, and the D block
+      const double a = wkk;
+      const double b = static_cast<double>(sw ? wk_r : wk_k1);
+      const double cc = crr;
+      double itl, ibr, ioff;
+      inverse_2x2_nc(a, b, cc, &itl, &ibr, &ioff);
+      if (j < k) {
+        if (sw) {
+          swap_elems(&S[pk<T>(n, j, k1)], &S[pk<T>(n, j, r)]);
+          if (j < kk) swap_elems(&W[j * n + k1], &W[j * n + r]);
+        }
+      } else if (j == k) {
+        S[pk<T>(n, k, k)] = static_cast<T>(a);
+        S[pk<T>(n, k, k1)] = static_cast<T>(b);
+        Wk[k1] = static_cast<T>(b);
+        piv[k] = 2;
+      } else if (j == k1) {
+        if (sw) {
+          S[pk<T>(n, r, r)] = S[pk<T>(n, k1, k1)];
+          const int t = perm[k1];
+          perm[k1] = perm[r];
+          perm[r] = t;
+        }
+        S[pk<T>(n, k1, k1)] = static_cast<T>(cc);
+        Wk1[k1] = static_cast<T>(cc);
+        piv[k1] = -2;
+      } else if (j < n) {
+        if (sw) {
+          const T old = S[pk<T>(n, k1, j)];
+          if (j > r)
+            S[pk<T>(n, r, j)] = old;
+          else if (j < r)
+            S[pk<T>(n, j, r)] = old;
+        }
+        const T w0 = (sw && j == r) ? wk_k1 : w;
+        const T w1 = (sw && j == r) ? static_cast<T>(ck1) : c;
+        Wk[j] = w0;
+        Wk1[j] = w1;
+        double u0, u1;
+        apply_2x2_nc(itl, ibr, ioff, static_cast<double>(w0), static_cast<double>(w1), &u0, &u1);
+        S[pk<T>(n, k, j)] = static_cast<T>(u0);
+        S[pk<T>(n, k1, j)] = static_cast<T>(u1);
+      }
+      kk += 2;
+    }
+    __syncthreads();
+  }
+  return e;
+}
+
+// ---- resident mode (batches of matrices up to kBatchResidentN) -------------
+// Every panel of one matrix, then its trailing update (the reference's
+// blocked loop, src/block_ldlt_decomposition.cc:50-59), on a storage view:
+// the fused on-chip panel for a matrix held whole in LDS, factor_panel for
+// one that keeps its first rows in global memory.
+template <typename T, typename Mat>
+__device__ void factor_resident(const Mat& F, int n, int* perm, int* piv, int bw, int ts, T* W,
+                                T* Lk, T* Lr, ScaleTab& st, double* rv, int* ri,
+                                PivotSlots& ps) {
+  constexpr bool kOnChip = std::is_same<Mat, LdsMat<T>>::value;
+  int s = 0;
+  int par = 0;
+  for (int P = 0; s < n; ++P) {
+    int e;
+    if constexpr (kOnChip)
+      e = factor_panel_lds<T, sizeof(T) == 8>(F.s, n, perm, piv, bw, P, s, W, ps, rv, ri, par);
+    else
+      e = factor_panel<T, 256, sizeof(T) == 8>(F, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri);
+    if (ts == 32)
+      mfma_update<T, 256, 32>(F, n, s, e, W);
+    else
+      mfma_update<T, 256, 16>(F, n, s, e, W);
+    __syncthreads();
+    s = e;
+  }
+}
+
+// Packed index of element (r, c), r <= c, of an order-m upper triangle.
+__device__ __forceinline__ int packed_at(int m, int r, int c) {
+  return r * m - ((r * (r - 1)) >> 1) + (c - r);
+}
+
+// Copy the upper triangle of rows/columns [R0, n) between A (row-major,
+// lda) and the packed LDS array S (m = n - R0): 16 elements per thread in
+// flight, consecutive threads on consecutive columns of a row.
+template <typename T, bool TO_LDS>
+__device__ void packed_copy(T* __restrict__ A, int64_t lda, int R0, int m, T* S) {
+  const int total = m * (m + 1) / 2;
+  // thread-local walk over the packed index idx = tid + 256 u: row r holds
+  // [start, start + m - r)
+  int r = 0, start = 0;
+  for (int base = 0; base < total; base += 256 * 16) {
+    int rr[16], cc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = base + 256 * u + static_cast<int>(threadIdx.x);
+      if (idx < total) {
+        while (idx >= start + (m - r)) {
+          start += m - r;
+          ++r;
+        }
+        rr[u] = r;
+        cc[u] = r + (idx - start);
+      } else {
+        rr[u] = -1;
+        cc[u] = 0;
+      }
+    }
+    if constexpr (TO_LDS) {
+      T v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        v[u] = rr[u] >= 0 ? A[(int64_t)(R0 + rr[u]) * lda + R0 + cc[u]] : T(0);
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (rr[u] >= 0) S[packed_at(m, rr[u], cc[u])] = v[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (rr[u] >= 0) A[(int64_t)(R0 + rr[u]) * lda + R0 + cc[u]] = S[packed_at(m, rr[u], cc[u])];
+    }
+  }
+}
+
+// Interchanges deferred by a HybridMat (dlist: ns pairs, in order) applied
+// to rows [0, R0) of A, composed into one gather per row (compose_swaps;
+// slot and content: LDS scratch of 2 ns ints each).  Eight rows per pass,
+// every read of a pass before its writes.
+template <typename T>
+__device__ void apply_deferred(T* __restrict__ A, int64_t lda, int R0, const int* dlist, int ns,
+                               int* slot, int* content) {
+  if (ns == 0 || R0 == 0) return;
+  compose_swaps(dlist, ns, slot, content);
+  const int ne = 2 * ns;
+  const int t0 = threadIdx.x, t1 = threadIdx.x + 256;
+  const int d0 = t0 < ne ? dlist[t0] : 0, s0 = t0 < ne ? content[slot[t0]] : 0;
+  const int d1 = t1 < ne ? dlist[t1] : 0, s1 = t1 < ne ? content[slot[t1]] : 0;
+  for (int p0 = 0; p0 < R0; p0 += 8) {
+    T v0[8], v1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int row = p0 + u;
+      const T* R = A + (int64_t)row * lda;
+      v0[u] = (row < R0 && t0 < ne) ? R[s0] : T(0);
+      v1[u] = (row < R0 && t1 < ne) ? R[s1] : T(0);
+    }
+    __syncthreads();  // every read of these rows precedes every write
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int row = p0 + u;
+      T* R = A + (int64_t)row * lda;
+      if (row < R0 && t0 < ne) R[d0] = v0[u];
+      if (row < R0 && t1 < ne) R[d1] = v1[u];
+    }
+    __syncthreads();
+  }
+}
+
+// Resident persistent kernel: workgroups take matrices from the work queue
+// (largest first); per matrix the dynamic LDS holds, in order, the panel
+// rows W ((bw + 2) x n), perm and piv (n ints each), the deferred
+// interchange list (2 n ints) and the packed upper triangle of the trailing
+// square [R0, n)^2, R0 the smallest start whose triangle fits what is left
+// (R0 = 0 for every matrix that fits whole: LdsMat, no global access until
+// the results are written back).
+template <typename T>
+__global__ __launch_bounds__(256) void k_batched_lds(BatchArgs<T> a) {
+  __shared__ T Lk[kBatchMaxWidth + 2];
+  __shared__ T Lr[kBatchMaxWidth + 2];
+  __shared__ ScaleTab st;
+  __shared__ double rv[8];  // two halves: the one-barrier reductions alternate
+  __shared__ int ri[8];
+  __shared__ PivotSlots ps;
+  __shared__ int sq;
+  __shared__ int dcount;
+  unsigned char* arena = bbk_batched_lds;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) sq = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int q = sq;
+    __syncthreads();
+    if (q >= a.count) break;
+    const int mi = a.order[q];
+    const int n = a.ns[mi];
+    if (n <= 0) continue;
+    const int c = batch_class(n);
+    const int bw = a.widths.w[c];
+    const int ts = a.widths.ts[c] == 32 ? 32 : 16;
+    T* A = a.A[mi];
+    const int64_t lda = a.lda[mi];
+    // LDS layout (16-byte aligned regions)
+    // (the W region also holds the 4 n ints of apply_deferred's scratch)
+    T* W = reinterpret_cast<T*>(arena);
+    size_t off = (std::max((size_t)(bw + 2) * n * sizeof(T), (size_t)16 * n) + 15) / 16 * 16;
+    int* lperm = reinterpret_cast<int*>(arena + off);
+    int* lpiv = lperm + n;
+    int* dlist = lpiv + n;
+    off += ((size_t)4 * n * sizeof(int) + 15) / 16 * 16;
+    T* S = reinterpret_cast<T*>(arena + off);
+    const int avail = static_cast<int>((a.lds_bytes - (int)off) / (int)sizeof(T));
+    int m = n;
+    while (m > 0 && m * (m + 1) / 2 > avail) --m;
+    const int R0 = n - m;
+    for (int i = tid; i < n; i += 256) {
+      lperm[i] = i;
+      lpiv[i] = 1;
+    }
+    if (tid == 0) dcount = 0;
+    packed_copy<T, true>(A, lda, R0, m, S);
+    __syncthreads();
+    if (R0 == 0)
+      factor_resident<T>(LdsMat<T>{S, n}, n, lperm, lpiv, bw, ts, W, Lk, Lr, st, rv, ri, ps);
+    else
+      factor_resident<T>(HybridMat<T>{A, lda, S, n, R0, dlist, &dcount}, n, lperm, lpiv, bw, ts,
+                         W, Lk, Lr, st, rv, ri, ps);
+    // results: the trailing square, perm and piv back to global memory;
+    // the deferred interchanges on the rows kept there
+    packed_copy<T, false>(A, lda, R0, m, S);
+    int* perm = a.perm[mi];
+    int* piv = a.piv[mi];
+    for (int i = tid; i < n; i += 256) {
+      perm[i] = lperm[i];
+      piv[i] = lpiv[i];
+    }
+    // W is free now: slot and content of the composed interchanges
+    int* slot = reinterpret_cast<int*>(W);
+    apply_deferred<T>(A, lda, R0, dlist, R0 > 0 ? dcount : 0, slot, slot + 2 * n);
+    __syncthreads();
   }
 }
 
@@ -439,7 +960,8 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   }
   const int s = P == 0 ? 0 : S[0];
   T* W = a.ws + a.wsoff[m];
-  const int e = factor_panel<T, 256, true>(a.A[m], a.lda[m], n, perm, piv, bw, P, s, W, Lk, Lr, stab,
+  const int e = factor_panel<T, 256, true>(GlobalMat<T>{a.A[m], a.lda[m]}, n, perm, piv, bw, P, s,
+                                           W, Lk, Lr, stab,
                                      rv, ri);
   if (threadIdx.x == 0) {
     S[0] = e;
@@ -527,6 +1049,23 @@ static int batched_grid_nt(Handle* h, int count) {
   return std::max(1, std::min(count, per_cu * cus));
 }
 
+template <typename T>
+static int resident_grid(Handle* h, int count) {
+  int per_cu = 0, cus = 0;
+  const void* kern = reinterpret_cast<const void*>(&k_batched_lds<T>);
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kBatchResidentLdsBytes);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, kBatchResidentLdsBytes) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess ||
+      cus < 1)
+    cus = 1;
+  return std::max(1, std::min(count, per_cu * cus));
+}
+
 static int batched_grid(Handle* h, int count, int elem_bytes, int nt) {
   if (elem_bytes == 8)
     return nt == 64 ? batched_grid_nt<double, 64>(h, count) : batched_grid_nt<double, 256>(h, count);
@@ -592,6 +1131,7 @@ static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes
   for (int P = 0; P < steps; ++P) p.step_tiles[P] = prefix[(size_t)P * sstride + count];
   p.grid = count;
   p.nt = 256;
+  p.resident = false;
   return BBK_OK;
 }
 
@@ -611,7 +1151,32 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   int nmax = 0;
   for (int i = 0; i < count; ++i) nmax = std::max(nmax, h_n[i]);
   if (nmax > kBatchLockstepN) return prepare_lockstep(h, count, h_n, elem_bytes, wt);
-  const int nt = wt.nt[batch_class(nmax)] == 64 ? 64 : 256;
+  if (nmax <= kBatchResidentN && wt.nt[batch_class(nmax)] == 0) {
+    int grid = elem_bytes == 8 ? resident_grid<double>(h, std::max(count, 1))
+                               : resident_grid<float>(h, std::max(count, 1));
+    if (h->batch_grid_limit > 0) grid = std::min(grid, h->batch_grid_limit);
+    const size_t ints = 2 * static_cast<size_t>(count) + 64;
+    int rc = h->reserve_buf(&p.buf, &p.buf_bytes, ints * sizeof(int), "batch schedule");
+    if (rc != BBK_OK) return rc;
+    std::vector<int> host(ints, 0);
+    std::copy(order.begin(), order.end(), host.begin());
+    std::copy(h_n, h_n + count, host.begin() + count);
+    hipError_t e = hipMemcpyAsync(p.buf, host.data(), ints * sizeof(int), hipMemcpyHostToDevice,
+                                  h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "upload batch schedule");
+    p.count = count;
+    p.elem = elem_bytes;
+    p.sizes.assign(h_n, h_n + count);
+    p.widths = wt;
+    p.ws_stride = 0;
+    p.grid = grid;
+    p.nt = 256;
+    p.lockstep = false;
+    p.resident = true;
+    return BBK_OK;
+  }
+  const int nt = wt.nt[batch_class(nmax)] == 64 ? 64 : 256;  // 0 (resident) above 256: four waves
   const int lds_elems = batch_lds_bytes(nt) / elem_bytes;
   int64_t stride = 0;
   for (int i = 0; i < count; ++i) {
@@ -646,6 +1211,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   p.grid = grid;
   p.nt = nt;
   p.lockstep = false;
+  p.resident = false;
   return BBK_OK;
 }
 
@@ -653,7 +1219,8 @@ static bool plan_matches(const BatchPlan& p, int count, const int* h_n, int elem
                          const BatchWidths& wt) {
   if (p.count != count || p.elem != elem_bytes) return false;
   for (int c = 0; c < kBatchClasses; ++c)
-    if (p.widths.w[c] != wt.w[c] || p.widths.nt[c] != wt.nt[c]) return false;
+    if (p.widths.w[c] != wt.w[c] || p.widths.nt[c] != wt.nt[c] || p.widths.ts[c] != wt.ts[c])
+      return false;
   return count == 0 || std::equal(h_n, h_n + count, p.sizes.begin());
 }
 
@@ -707,9 +1274,13 @@ int batched_launch(Handle* h, const BatchPlan& p, T* const* A, const int64_t* ld
   a.ws = reinterpret_cast<T*>(p.ws);
   a.ws_stride = p.ws_stride;
   a.lds_elems = batch_lds_bytes(p.nt) / static_cast<int>(sizeof(T));
+  a.lds_bytes = kBatchResidentLdsBytes;
   a.widths = p.widths;
   hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, h->stream, a.queue, 1, 0);
-  if (p.nt == 64)
+  if (p.resident)
+    hipLaunchKernelGGL(k_batched_lds<T>, dim3(p.grid), dim3(256), kBatchResidentLdsBytes,
+                       h->stream, a);
+  else if (p.nt == 64)
     hipLaunchKernelGGL((k_batched_sytrf<T, 64>), dim3(p.grid), dim3(64), batch_lds_bytes(64),
                        h->stream, a);
   else
@@ -831,14 +1402,29 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
     rc = batched_gen<T>(h, count, dN, 0x7475u + c, 1.0, 7, reinterpret_cast<T* const*>(dS), dL);
     float best_ms = 1e30f;
     int best_w = tab.w[c], best_nt = tab.nt[c];
-    for (int trial = 0; trial < 2 * static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++trial) {
-      const int w = cand[trial >> 1];
-      const int nt = (trial & 1) ? 256 : 64;
+    // classes n <= kBatchResidentN: panel width x mode (one-wavefront team,
+    // four-wavefront team, resident in LDS) x trailing update (SIMT 4 x 4,
+    // MFMA 16 x 16, MFMA 32 x 32; the resident mode is MFMA only); larger
+    // classes: panel width x team size, SIMT update
+    static const int rcand[] = {8, 16, 24, 32};
+    static const int modes[] = {64, 256, 0};
+    static const int tiles[] = {0, 16, 32};
+    const bool res_class = hi <= kBatchResidentN;
+    const int nw_small = static_cast<int>(sizeof(rcand) / sizeof(rcand[0]));
+    const int ntrials = res_class ? nw_small * 3 * 3
+                                  : 2 * static_cast<int>(sizeof(cand) / sizeof(cand[0]));
+    int best_ts = tab.ts[c];
+    for (int trial = 0; trial < ntrials; ++trial) {
+      const int w = res_class ? rcand[trial / 9] : cand[trial >> 1];
+      const int nt = res_class ? modes[(trial / 3) % 3] : ((trial & 1) ? 256 : 64);
+      const int ts = res_class ? tiles[trial % 3] : 0;
+      if (res_class && nt == 0 && ts == 0) continue;
       if (rc != BBK_OK) break;
       if (w > kBatchMaxWidth) continue;
       BatchWidths saved = tab;
       tab.w[c] = w;
       tab.nt[c] = nt;
+      tab.ts[c] = ts;
       float ms_min = 1e30f;
       rc = batched_prepare(h, count, ns.data(), eb);
       for (int rep = 0; rep < 2 && rc == BBK_OK; ++rep) {
@@ -859,8 +1445,10 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
         best_ms = ms_min;
         best_w = w;
         best_nt = nt;
+        best_ts = ts;
       }
     }
+    tab.ts[c] = best_ts;
     tab.w[c] = best_w;
     tab.nt[c] = best_nt;
     hipFree(mem);
@@ -1019,10 +1607,35 @@ int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes, const int* t
   if (elem_bytes != 4 && elem_bytes != 8) return -2;
   if (!team_sizes) return -3;
   for (int c = 0; c < bbk::kBatchClasses; ++c)
-    if (team_sizes[c] != 64 && team_sizes[c] != 256) return -3;
+    if (team_sizes[c] != 64 && team_sizes[c] != 256 &&
+        !(team_sizes[c] == 0 && bbk::batch_class_bound(c) <= bbk::kBatchResidentN))
+      return -3;
   Handle* h = reinterpret_cast<Handle*>(handle);
   bbk::BatchWidths& t = h->batch_widths[elem_bytes == 8 ? 0 : 1];
   for (int c = 0; c < bbk::kBatchClasses; ++c) t.nt[c] = team_sizes[c];
+  h->batch.count = -1;
+  return BBK_OK;
+}
+
+int bbk_batched_get_tiles(bbk_handle_t handle, int elem_bytes, int* tiles) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (!tiles) return -3;
+  const bbk::BatchWidths& t =
+      reinterpret_cast<Handle*>(handle)->batch_widths[elem_bytes == 8 ? 0 : 1];
+  for (int c = 0; c < bbk::kBatchClasses; ++c) tiles[c] = t.ts[c];
+  return BBK_OK;
+}
+
+int bbk_batched_set_tiles(bbk_handle_t handle, int elem_bytes, const int* tiles) {
+  if (!handle) return -1;
+  if (elem_bytes != 4 && elem_bytes != 8) return -2;
+  if (!tiles) return -3;
+  for (int c = 0; c < bbk::kBatchClasses; ++c)
+    if (tiles[c] != 0 && tiles[c] != 16 && tiles[c] != 32) return -3;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  bbk::BatchWidths& t = h->batch_widths[elem_bytes == 8 ? 0 : 1];
+  for (int c = 0; c < bbk::kBatchClasses; ++c) t.ts[c] = tiles[c];
   h->batch.count = -1;
   return BBK_OK;
 }

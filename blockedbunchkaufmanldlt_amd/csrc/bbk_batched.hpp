@@ -20,6 +20,13 @@ constexpr int kBatchLdsBytes = 76 * 1024;
 // one launch factors every matrix's panel (one workgroup each), a second
 // applies all trailing updates as MFMA tiles spread over the whole GPU.
 constexpr int kBatchLockstepN = 512;
+// Batches whose largest matrix is at most this run resident: one
+// workgroup per matrix keeps the matrix (packed upper triangle), its panel
+// rows and its permutation in LDS for the whole factorization and applies
+// the trailing updates with MFMA tiles on LDS data.
+constexpr int kBatchResidentN = 256;
+// Dynamic LDS of the resident kernel (one workgroup per CU).
+constexpr int kBatchResidentLdsBytes = 152 * 1024;
 __host__ __device__ constexpr int batch_lds_bytes(int nt) {
   return nt == 64 ? kBatchLdsBytesSmall : kBatchLdsBytes;
 }
@@ -29,18 +36,26 @@ __host__ __device__ constexpr int batch_class_bound(int c) {
   return c < kBatchClasses - 1 ? (32 << c) : 0x7fffffff;
 }
 
-// Per size class: panel width w and team size nt (threads per matrix, 64
-// or 256).  A batch runs with the team size of its largest matrix's class.
+// Per size class: panel width w, team nt and trailing-update tile ts.
+//   nt  64 or 256 threads per matrix (the global-memory kernel), or 0 for
+//       classes up to kBatchResidentN: the resident kernel (four
+//       wavefronts, the matrix in LDS).  A batch runs with the mode of its
+//       largest matrix's class.
+//   ts  0: SIMT 4 x 4 register tiles; 16 or 32: MFMA tiles per wavefront
+//       (ts x ts outputs); the resident kernel uses MFMA (0 counts as 16).
 struct BatchWidths {
   int w[kBatchClasses];
   int nt[kBatchClasses];
+  int ts[kBatchClasses];
 };
 
 // Default table (used until bbk_batched_tune or bbk_batched_set_*).
 constexpr BatchWidths kDefaultBatchWidthsF64 = {{8, 16, 16, 32, 32, 32, 32, 32},
-                                                {256, 256, 256, 256, 256, 256, 256, 256}};
+                                                {256, 256, 256, 256, 256, 256, 256, 256},
+                                                {0, 0, 0, 0, 0, 0, 0, 0}};
 constexpr BatchWidths kDefaultBatchWidthsF32 = {{8, 16, 16, 32, 32, 32, 32, 32},
-                                                {256, 256, 256, 256, 256, 256, 256, 256}};
+                                                {256, 256, 256, 256, 256, 256, 256, 256},
+                                                {0, 0, 0, 0, 0, 0, 0, 0}};
 
 // The schedule of the last prepared batch: queue order, sizes, counter.
 struct BatchPlan {
@@ -56,6 +71,7 @@ struct BatchPlan {
   int grid = 0;
   int nt = 64;          // threads per matrix team
   bool lockstep = false;
+  bool resident = false;  // matrices held in LDS (largest n <= kBatchResidentN)
   int steps = 0;        // lockstep: panel steps (largest ceil(n / width))
   int64_t step_stride = 0;  // lockstep: ints per step in the tile prefix table
   int64_t ws_elems = 0;

@@ -161,12 +161,20 @@ int bbk_batched_prepare(bbk_handle_t handle, int count, const int* h_n,
 int bbk_batched_get_widths(bbk_handle_t handle, int elem_bytes, int* widths);
 int bbk_batched_set_widths(bbk_handle_t handle, int elem_bytes,
                            const int* widths);
-/* Team size per class (threads per matrix: 64 or 256); a batch runs with
- * the team size of the class of its largest matrix. */
+/* Team per class: 64 or 256 threads per matrix (the global-memory
+ * kernel), or 0 for classes n <= 256: the resident kernel, which keeps the
+ * matrix (packed upper triangle), its panel rows and permutation in LDS
+ * with one 256-thread workgroup per matrix.  A batch runs with the mode of
+ * the class of its largest matrix. */
 int bbk_batched_get_team_sizes(bbk_handle_t handle, int elem_bytes,
                                int* team_sizes);
 int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes,
                                const int* team_sizes);
+/* Trailing-update tile per class: 0 = SIMT 4 x 4 register tiles, 16 or 32
+ * = MFMA tiles of ts x ts outputs per wavefront (the resident kernel
+ * always uses MFMA; 0 counts as 16 there). */
+int bbk_batched_get_tiles(bbk_handle_t handle, int elem_bytes, int* tiles);
+int bbk_batched_set_tiles(bbk_handle_t handle, int elem_bytes, const int* tiles);
 /* Cap on the number of persistent workgroups of the batched kernels
  * (factorization and refinement); 0 (the default) = as many as fit the
  * GPU.  Each workgroup takes matrices from the work queue until it is
@@ -174,10 +182,12 @@ int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes,
  * matrices in turn (used by the tests to exercise that path; also a way to
  * leave CUs to other work). */
 int bbk_batched_set_grid_limit(bbk_handle_t handle, int max_workgroups);
-/* Autotuner: for every size class up to max_n, times candidate (panel
- * width, team size) pairs on a synthetic batch of that class and keeps the
- * fastest in the handle's table (widths written to widths_out when not
- * NULL).  Synchronous. */
+/* Autotuner: for every size class up to max_n, times candidates on a
+ * synthetic batch of that class and keeps the fastest in the handle's
+ * table (widths written to widths_out when not NULL): for classes n <= 256
+ * every (panel width, team mode 64 / 256 / resident, update tile SIMT /
+ * MFMA 16 / MFMA 32) combination, above that (panel width, team size).
+ * Synchronous. */
 int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n,
                      int* widths_out);
 

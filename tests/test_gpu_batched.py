@@ -110,6 +110,63 @@ def test_batched_small_widths_defer_2x2(oracle, handle, bw, nt):
         handle.set_batched_team_sizes(torch.float64, [256] * 8)
 
 
+RESIDENT_NS = [1, 2, 3, 7, 16, 31, 32, 33, 47, 64, 65, 100, 127, 128, 129, 150, 176, 177, 200,
+               230, 255, 256]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("bw,ts", [(4, 16), (16, 32), (32, 16), (24, 32)])
+def test_resident_kernel_matches_oracle(oracle, handle, precision, bw, ts):
+    """Batches whose largest matrix has n <= 256 run resident: the packed
+    upper triangle in LDS (binary64 matrices above the LDS budget keep their
+    first rows in global memory and defer those rows' interchanges), MFMA
+    trailing updates with 16 x 16 or 32 x 32 tiles per wavefront."""
+    import torch
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    saved_w, saved_t = handle.batched_widths(dtype), handle.batched_tiles(dtype)
+    saved_nt = handle.batched_team_sizes(dtype)
+    handle.set_batched_widths(dtype, [bw] * 8)
+    handle.set_batched_tiles(dtype, [ts] * 8)
+    handle.set_batched_team_sizes(dtype, [0, 0, 0, 0, 256, 256, 256, 256])
+    try:
+        mats = handle.gen_synth_batched(RESIDENT_NS, seed=7000 + bw + ts, scale=1.0,
+                                        zero_diag_stride=7, dtype=dtype, pad=3)
+        hosts = [host_matrix(m) for m in mats]
+        outs = factor_batch(handle, oracle, mats, dtype)
+        for A, g in zip(hosts, outs):
+            check(oracle, A, g, bw, fp32=(precision == "fp32"))
+        assert any((g.piv == 2).any() for g in outs)
+    finally:
+        handle.set_batched_widths(dtype, saved_w)
+        handle.set_batched_tiles(dtype, saved_t)
+        handle.set_batched_team_sizes(dtype, saved_nt)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("nt,ts", [(64, 16), (64, 32), (256, 16), (256, 32)])
+def test_global_kernel_mfma_update(oracle, handle, precision, nt, ts):
+    """The global-memory kernel with the MFMA trailing update (16 x 16 or
+    32 x 32 tiles per wavefront) instead of SIMT tiles, one- and
+    four-wavefront teams, orders up to 400 (W in LDS and in global slices)."""
+    import torch
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    saved_t, saved_nt = handle.batched_tiles(dtype), handle.batched_team_sizes(dtype)
+    handle.set_batched_tiles(dtype, [ts] * 8)
+    handle.set_batched_team_sizes(dtype, [nt] * 8)
+    try:
+        ns = RESIDENT_NS + [300, 400]
+        mats = handle.gen_synth_batched(ns, seed=9100 + nt + ts, scale=1.0, zero_diag_stride=7,
+                                        dtype=dtype, pad=2)
+        hosts = [host_matrix(m) for m in mats]
+        widths = handle.batched_widths(dtype)
+        outs = factor_batch(handle, oracle, mats, dtype)
+        for A, g in zip(hosts, outs):
+            check(oracle, A, g, width_for(widths, A.shape[0]), fp32=(precision == "fp32"))
+    finally:
+        handle.set_batched_tiles(dtype, saved_t)
+        handle.set_batched_team_sizes(dtype, saved_nt)
+
+
 def test_batched_global_workspace_path(oracle, handle):
     """Matrices whose panel rows exceed the LDS budget use global slices."""
     import torch
@@ -201,9 +258,10 @@ def test_batched_autotuner(oracle, handle):
     saved_nt = handle.batched_team_sizes(torch.float64)
     try:
         widths = handle.batched_tune(torch.float64, 128)
-        assert all(w in (8, 16, 24, 32, 48, 64) for w in widths[:3])
+        assert all(w in (8, 12, 16, 24, 32, 48, 64) for w in widths[:3])
+        assert all(t in (0, 16, 32) for t in handle.batched_tiles(torch.float64))
         assert widths == handle.batched_widths(torch.float64)
-        assert all(t in (64, 256) for t in handle.batched_team_sizes(torch.float64))
+        assert all(t in (0, 64, 256) for t in handle.batched_team_sizes(torch.float64))
         ns = [30, 60, 120]
         mats = handle.gen_synth_batched(ns, seed=8, zero_diag_stride=7, dtype=torch.float64)
         hosts = [host_matrix(m) for m in mats]

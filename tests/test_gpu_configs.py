@@ -176,18 +176,19 @@ def test_c5_full_batch_reaches_1e12(handle):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("nt", [64, 256])
-def test_persistent_queue_several_matrices_per_workgroup(oracle, handle, precision, nt):
+@pytest.mark.parametrize("nt,nmax", [(64, 300), (256, 300), (0, 256)])
+def test_persistent_queue_several_matrices_per_workgroup(oracle, handle, precision, nt, nmax):
     """With the grid capped at 3 workgroups, each factors about 8 of the 24
     matrices in turn (LDS panel rows, scaling records and global W slices
-    reused across matrices): results are bitwise those of the uncapped run
-    (one matrix per workgroup) and match the oracle."""
+    reused across matrices; with n <= 256 the resident kernel's LDS matrix,
+    permutation and deferred-interchange list): results are bitwise those
+    of the uncapped run (one matrix per workgroup) and match the oracle."""
     import torch
     dtype = torch.float64 if precision == "fp64" else torch.float32
     saved = handle.batched_team_sizes(dtype)
-    handle.set_batched_team_sizes(dtype, [nt] * 8)
+    handle.set_batched_team_sizes(dtype, [nt] * 4 + [256] * 4)
     try:
-        ns = [256, 1, 200, 17, 129, 64, 300, 33, 150, 2, 255, 90, 128, 5, 220, 40,
+        ns = [256, 1, 200, 17, 129, 64, nmax, 33, 150, 2, 255, 90, 128, 5, 220, 40,
               180, 63, 100, 7, 240, 31, 160, 77]
         src = handle.gen_synth_batched(ns, seed=60606, zero_diag_stride=7, dtype=dtype, pad=3)
         ref_w, ref_p, ref_v = _run_graph(handle, torch, src, dtype)
