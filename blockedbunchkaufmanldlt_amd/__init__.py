@@ -55,6 +55,8 @@ def lib() -> ctypes.CDLL:
     L.bbk_synchronize.argtypes = [vp]
     L.bbk_last_error.argtypes = [vp]
     L.bbk_last_error.restype = ctypes.c_char_p
+    L.bbk_set_wait_limit.argtypes = [vp, i32]
+    L.bbk_get_info.argtypes = [vp, ip, ip]
     for name in ("bbk_dsytrf", "bbk_ssytrf"):
         getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp]
     L.bbk_dsytrf_host.argtypes = [vp, i32, i32, vp, i64, vp, vp]
@@ -166,6 +168,18 @@ class Handle:
 
     def synchronize(self):
         self.check(lib().bbk_synchronize(self.h), "synchronize")
+
+    def set_wait_limit(self, polls: int):
+        """Polls per cooperative-kernel wait (0 = default, < 0 = fail at once)."""
+        self.check(lib().bbk_set_wait_limit(self.h, polls), "set_wait_limit")
+
+    def info(self):
+        """(info, nonfinite) of the last factorization: info = k + 1 for the
+        first singular pivot column k (0 = none), nonfinite = 1 when a NaN or
+        infinity appeared in a pivot row."""
+        i, nf = ctypes.c_int(), ctypes.c_int()
+        self.check(lib().bbk_get_info(self.h, ctypes.byref(i), ctypes.byref(nf)), "get_info")
+        return i.value, nf.value
 
     def set_timing(self, on: bool):
         lib().bbk_set_timing(self.h, int(on))

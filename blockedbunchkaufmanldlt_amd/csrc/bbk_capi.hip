@@ -83,8 +83,9 @@ int default_panel_width(int n, int elem_bytes) {
 // count.
 template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
-                int max_groups, int* epoch, hipStream_t st, Handle* h) {
-  launch_init(n, perm, piv, fs.state, st);
+                int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
+                int poll_limit) {
+  launch_init(n, perm, piv, fs.state, status, st);
   int panels = 0;
   for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
     const int groups = (n - s_nom + kPanelThreads - 1) / kPanelThreads;
@@ -104,6 +105,8 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
     // Small panels run on one XCD: the grid is spread over the 8 XCDs and
     // only every 8th workgroup works, when all of them fit co-resident.
     pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
+    pa.status = status;
+    pa.poll_limit = poll_limit;
     if (h) h->timing_mark(Handle::kPanel, true);
     hipError_t e = launch_panel<T>(A, pa, groups * pa.xcd_stride, st);
     if (e != hipSuccess) return h ? h->hip_fail(e, "panel launch") : BBK_ERR_HIP;
@@ -124,9 +127,9 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
 }
 
 template int factor_loop<double>(int, int, double*, int64_t, int*, int*, const FactorScratch&,
-                                 int, int*, hipStream_t, Handle*);
+                                 int, int*, hipStream_t, Handle*, unsigned*, int);
 template int factor_loop<float>(int, int, float*, int64_t, int*, int*, const FactorScratch&,
-                                int, int*, hipStream_t, Handle*);
+                                int, int*, hipStream_t, Handle*, unsigned*, int);
 
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
@@ -165,11 +168,31 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     h->ws_layout = layout;
     h->epoch = 1024;
   }
-  return factor_loop<T>(n, b, A, lda, perm, piv, fs, max_groups, &h->epoch, st, h);
+  return factor_loop<T>(n, b, A, lda, perm, piv, fs, max_groups, &h->epoch, st, h, h->d_status,
+                        h->poll_limit_for_kernels());
 }
 
 template int factorize<double>(Handle*, int, int, double*, int64_t, int*, int*);
 template int factorize<float>(Handle*, int, int, float*, int64_t, int*, int*);
+
+int Handle::check_status() {
+  if (!d_status) return BBK_OK;
+  unsigned w[kStatusWords];
+  hipError_t e = hipMemcpyAsync(w, d_status, sizeof(w), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_fail(e, "read device status");
+  if (w[kStatusError] != 0) {
+    launch_zero_u32(d_status + kStatusError, 1, stream);
+    (void)hipStreamSynchronize(stream);
+    last_error = std::string("a device wait exceeded its poll limit (") +
+                 ((w[kStatusError] & kErrPanelWait) ? "panel exchange" : "") +
+                 ((w[kStatusError] & kErrPanelWait) && (w[kStatusError] & kErrSweepWait) ? ", " : "") +
+                 ((w[kStatusError] & kErrSweepWait) ? "triangular sweep" : "") +
+                 "); the results of the work queued since the last check are invalid";
+    return BBK_ERR_HIP;
+  }
+  return BBK_OK;
+}
 
 // ---- optional event timing of the kernel groups -------------------------
 void Handle::timing_reset() {
@@ -207,6 +230,7 @@ Handle::~Handle() {
   if (mixed_ws) (void)hipFree(mixed_ws);
   if (batch.buf) (void)hipFree(batch.buf);
   if (batch.ws) (void)hipFree(batch.ws);
+  if (d_status) (void)hipFree(d_status);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -237,6 +261,19 @@ int bbk_create(bbk_handle_t* handle, void* stream) {
     }
     h->own_stream = true;
   }
+  e = hipMalloc(&h->d_status, sizeof(unsigned) * bbk::kStatusWords);
+  if (e != hipSuccess) {
+    h->d_status = nullptr;
+    delete h;
+    return BBK_ERR_ALLOC;
+  }
+  const unsigned init[bbk::kStatusWords] = {0u, bbk::kInfoNone, 0u, 0u};
+  e = hipMemcpyAsync(h->d_status, init, sizeof(init), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    delete h;
+    return BBK_ERR_HIP;
+  }
   *handle = reinterpret_cast<bbk_handle_t>(h);
   return BBK_OK;
 }
@@ -261,7 +298,26 @@ int bbk_synchronize(bbk_handle_t handle) {
   Handle* h = reinterpret_cast<Handle*>(handle);
   hipError_t e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+  return h->check_status();
+}
+
+int bbk_set_wait_limit(bbk_handle_t handle, int polls) {
+  if (!handle) return -1;
+  reinterpret_cast<Handle*>(handle)->poll_limit = polls;
   return BBK_OK;
+}
+
+int bbk_get_info(bbk_handle_t handle, int* info, int* nonfinite) {
+  if (!handle) return -1;
+  Handle* h = reinterpret_cast<Handle*>(handle);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
+  unsigned w[bbk::kStatusWords];
+  e = hipMemcpy(w, h->d_status, sizeof(w), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return h->hip_fail(e, "read device status");
+  if (info) *info = w[bbk::kStatusInfo] == bbk::kInfoNone ? 0 : static_cast<int>(w[bbk::kStatusInfo]);
+  if (nonfinite) *nonfinite = static_cast<int>(w[bbk::kStatusNonFinite]);
+  return h->check_status();
 }
 
 const char* bbk_last_error(bbk_handle_t handle) {
@@ -324,6 +380,7 @@ int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
                            h->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
       if (e != hipSuccess) rc = h->hip_fail(e, "D2H copy");
+      if (rc == BBK_OK) rc = h->check_status();
     }
   }
   hipStreamSynchronize(h->stream);

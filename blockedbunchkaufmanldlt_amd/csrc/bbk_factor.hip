@@ -106,13 +106,18 @@ struct UpdateCfg {
 };
 
 __global__ void bk_init_kernel(int n, int* __restrict__ perm,
-                               int* __restrict__ piv, int* __restrict__ state) {
+                               int* __restrict__ piv, int* __restrict__ state,
+                               unsigned* __restrict__ status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     perm[i] = i;
     piv[i] = 0;
   }
   if (i < PanelState::kInts) state[i] = 0;
+  if (status && i == 0) {
+    status[kStatusInfo] = kInfoNone;
+    status[kStatusNonFinite] = 0u;
+  }
 }
 
 __global__ void bk_zero_u32_kernel(unsigned* __restrict__ p, int count) {
@@ -138,10 +143,10 @@ void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                      state, swaps, n, gemm_blocks);
 }
 
-void launch_init(int n, int* perm, int* piv, int* state, hipStream_t stream) {
+void launch_init(int n, int* perm, int* piv, int* state, unsigned* status, hipStream_t stream) {
   const int blocks = (n + 255) / 256;
   hipLaunchKernelGGL(bk_init_kernel, dim3(blocks > 0 ? blocks : 1), dim3(256),
-                     0, stream, n, perm, piv, state);
+                     0, stream, n, perm, piv, state, status);
 }
 
 void launch_zero_u32(unsigned* p, int count, hipStream_t stream) {

@@ -38,6 +38,14 @@ struct Handle {
   void* mixed_ws = nullptr;  // refinement driver: factor copy, r, d
   size_t mixed_bytes = 0;
   double* last_anorm_dev = nullptr;
+  // Device status words (kStatusWords unsigned, bbk_internal.hpp): wait
+  // errors, the last factorization's singular-pivot info and non-finite flag.
+  unsigned* d_status = nullptr;
+  int poll_limit = 0;  // exchange / sweep waits: 0 = kDefaultPollLimit, < 0 = fail at once
+  int poll_limit_for_kernels() const { return poll_limit == 0 ? kDefaultPollLimit : poll_limit; }
+  // Reads the status words after the stream has finished; a wait error is
+  // cleared and returned as BBK_ERR_HIP (with last_error set).
+  int check_status();
   int epoch = 1024;          // next panel-exchange tag range
   std::string last_error;
   bool timing = false;
@@ -73,7 +81,8 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
 // (h: timing and error text, may be NULL).
 template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
-                int max_groups, int* epoch, hipStream_t st, Handle* h);
+                int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
+                int poll_limit);
 // Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
 // refinement's solves instead of substitution, with F's strictly lower
 // triangle holding the mirror of the upper one; NULL substitutes (F upper

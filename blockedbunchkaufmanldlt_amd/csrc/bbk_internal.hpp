@@ -85,11 +85,57 @@ __device__ __forceinline__ bool xget(const XRec* p, int tag, int* aux, double* v
   return true;
 }
 
-// Poll one record until it carries `tag` (the writer is a co-resident
-// workgroup of the same cooperative launch, so the wait ends).
-__device__ __forceinline__ double xwait(const XRec* p, int tag, int* aux) {
+// Device status words of a handle (4 x 32 bits in device memory):
+//   kStatusError  bit 1: a panel exchange wait ran past its poll limit;
+//                 bit 2: a triangular-sweep wait did (results invalid).
+//                 Kept until the host reads and clears it.
+//   kStatusInfo   first singular pivot of the last factorization, as
+//                 LAPACK's info: column k + 1 of a 1x1 pivot with
+//                 |d| <= DBL_EPSILON or a 2x2 block with det = 0
+//                 (0xffffffff = none).
+//   kStatusNonFinite  1 when the last factorization met a NaN or an
+//                 infinity in a pivot row.
+constexpr int kStatusError = 0, kStatusInfo = 1, kStatusNonFinite = 2, kStatusWords = 4;
+constexpr unsigned kErrPanelWait = 1u, kErrSweepWait = 2u;
+constexpr unsigned kInfoNone = 0xffffffffu;
+// Poll limit given to the kernels when the handle's is 0 (about seconds,
+// far beyond any wait between co-resident workgroups).
+constexpr int kDefaultPollLimit = 1 << 22;
+
+// Bounded waits of a cooperative launch.  Every writer is a co-resident
+// workgroup of the same launch, so a wait ends unless a record is lost;
+// the poll limit turns such a loss into an error the host reports instead
+// of a hang.  A thread that ran out of polls marks the error once and from
+// then on returns at once (aux INT_MAX, value 0: never a valid index), so
+// the launch still terminates.  A negative limit fails every wait at once
+// (host-forced, to test the error path).
+struct WaitGuard {
+  unsigned* status;
+  int limit;
+  bool failed;
+  __device__ __forceinline__ void fail(unsigned bit) {
+    if (!failed && status) atomicOr(status + kStatusError, bit);
+    failed = true;
+  }
+};
+
+// Poll one record until it carries `tag`, within the guard's limit.
+__device__ __forceinline__ double xwait(const XRec* p, int tag, int* aux, WaitGuard& g) {
   double v;
-  while (!xget(p, tag, aux, &v)) __builtin_amdgcn_s_sleep(1);
+  if (g.failed || g.limit < 0) {
+    g.fail(kErrPanelWait);
+    *aux = INT_MAX;
+    return 0.0;
+  }
+  int polls = 0;
+  while (!xget(p, tag, aux, &v)) {
+    if (++polls > g.limit) {
+      g.fail(kErrPanelWait);
+      *aux = INT_MAX;
+      return 0.0;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
   return v;
 }
 

@@ -19,6 +19,8 @@ struct PanelArgs {
   void* Wg;         // (kPanelMaxWidth + 1) x ldw elements of T
   int64_t ldw;
   int xcd_stride;   // 8: only workgroups blockIdx % 8 == 0 work (one XCD); 1: all
+  unsigned* status; // kStatusWords device status words (may be NULL)
+  int poll_limit;   // exchange waits: polls before giving up (< 0: fail at once)
 };
 
 // Exchange records the panel kernel needs for a grid of `groups`.
@@ -35,7 +37,9 @@ template <typename T>
 void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                             const int* state, const int* swaps, int n, int m_bound,
                             int rows_bound, hipStream_t stream);
-void launch_init(int n, int* perm, int* piv, int* state, hipStream_t stream);
+// perm = identity, piv = 0, panel state cleared; with status, the info and
+// non-finite words of a new factorization reset.
+void launch_init(int n, int* perm, int* piv, int* state, unsigned* status, hipStream_t stream);
 
 // Scratch carve-up for one factorization of order n in type T: exchange
 // records, panel state, the interchange list and the flushed W rows.  A

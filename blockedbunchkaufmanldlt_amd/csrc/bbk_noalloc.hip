@@ -97,6 +97,8 @@ extern "C" int bbk_dsytrf_noalloc(int n, int nb, double* d_A, int64_t lda, int* 
     d.cleared = true;
     d.epoch = 1024;
   }
+  // no handle, so no status words: a lost exchange record ends the waits
+  // after the default poll limit without an error report
   return factor_loop<double>(n, b, d_A, lda, d_perm, d_piv, fs, max_groups, &d.epoch, st,
-                             nullptr);
+                             nullptr, nullptr, kDefaultPollLimit);
 }

@@ -430,6 +430,8 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   int lp = 0;        // Lbuf index of the current pivot
 
   auto area = [&](int tag) -> XRec* { return xbase + (tag & 1) * PS; };
+  WaitGuard wg{p.status, p.poll_limit, false};
+  bool nonfinite = false;  // a NaN or infinity seen in this thread's pivot-row values
 
   // L2-scope exchanges only when every working group reports the same
   // XCC_ID (one device-coherent exchange per launch); otherwise every store
@@ -443,7 +445,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     int same = 1;
     for (int id = tid; id < G; id += NT) {
       int aux;
-      (void)xwait(ar + id, ex, &aux);
+      (void)xwait(ar + id, ex, &aux, wg);
       if (aux != xcc) same = 0;
     }
     fast = __syncthreads_and(same) != 0;
@@ -519,6 +521,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       dotA = have_next ? pnext : panel_dot<T, WS, RR>(Lk, Wl, wr, tid, kk, base);
       w = arow - dotA;
       Wl[(kk - base) * WS + tid] = w;
+      nonfinite |= !isfinite(static_cast<double>(w));
     }
     have_next = false;
     BBK_PT(1);
@@ -569,13 +572,13 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       for (int id = tid; id < total; id += NT) {
         int aux;
         if (id < G) {
-          slot_val = xwait(ar + id, ex, &aux);
+          slot_val = xwait(ar + id, ex, &aux, wg);
           slot_aux = aux;
           argmax_combine(omega, r, fabs(slot_val), aux);
         } else {
           const int u = id - G;
           const int pid = u < 2 ? u : (has1 && u < 4 ? u : kPubWk2);
-          pubv[pid] = xwait(ar + G + pid, ex, &aux);
+          pubv[pid] = xwait(ar + G + pid, ex, &aux, wg);
         }
       }
       block_argmax1(omega, r, red_v, red_i, par);  // also publishes pubv
@@ -679,11 +682,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       for (int id = tid; id < total; id += NT) {
         int aux;
         if (id < G) {
-          const double o = xwait(ar + id, ex, &aux);
+          const double o = xwait(ar + id, ex, &aux, wg);
           sg = (sg < o) ? o : sg;
         } else {
           const int pid = kPubCr + (id - G);  // 8, 9, (10, 11), (12, 13)
-          pubv[pid] = xwait(pubs + pid, ex, &aux);
+          pubv[pid] = xwait(pubs + pid, ex, &aux, wg);
         }
       }
       const double sigma = block_max1(sg, red_v, par);  // also publishes pubv
@@ -715,6 +718,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], w);
         p.piv[k] = 1;
+        if (den == 0.0 && p.status) atomicMin(p.status + kStatusInfo, static_cast<unsigned>(k + 1));
       }
       if (has1) {
         const T lkk = static_cast<T>(div_or_zero(pubv[kPubWk1], den));
@@ -751,6 +755,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], nw);
         p.piv[k] = 1;
+        if (den == 0.0 && p.status) atomicMin(p.status + kStatusInfo, static_cast<unsigned>(k + 1));
       }
       if (gi == 0 && tid == 0) {
         swl[2 * nswap] = k;
@@ -820,6 +825,11 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], static_cast<T>(a));
         p.piv[k] = 2;
+        {
+#pragma clang fp contract(off)
+          if (a * cc - bb * bb == 0.0 && p.status)
+            atomicMin(p.status + kStatusInfo, static_cast<unsigned>(k + 1));
+        }
       }
       if (j == k1) {
         sta(&A[(int64_t)k * lda + k1], static_cast<T>(bb));
@@ -884,6 +894,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     if (t0 < ne) p.perm[swl[t0]] = v0;
     if (t1 < ne) p.perm[swl[t1]] = v1;
   }
+  if (nonfinite && p.status) atomicOr(p.status + kStatusNonFinite, 1u);
   if (gi == 0 && tid == 0) {
     p.state[PanelState::kStart] = s;
     p.state[PanelState::kWidth] = kk;

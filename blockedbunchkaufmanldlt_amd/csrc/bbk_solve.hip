@@ -81,7 +81,7 @@ __device__ __forceinline__ void xload_multi<4>(const XRec* const (&p)[4], v4i (&
 
 template <int NR, int UB>
 __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row0)[UB], int nlim,
-                                                int tag, double (&out)[UB]) {
+                                                int tag, double (&out)[UB], WaitGuard& g) {
   const int lane = threadIdx.x & 63;
   const int q = lane & 3, rr = lane >> 2;
   const int qq = q < NR ? q : NR - 1;
@@ -94,7 +94,14 @@ __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row
     p[u] = rec + (int64_t)(row < nlim ? row : nlim - 1) * kSolveMaxRhs + qq;
     out[u] = 0.0;
   }
-  for (;;) {
+  // bounded (WaitGuard): past the poll limit the missing values stay 0,
+  // the error is marked once and later polls of this thread return at once
+  if (g.failed || g.limit < 0) {
+    if ((threadIdx.x & 63) == 0) g.fail(kErrSweepWait);
+    g.failed = true;
+    return;
+  }
+  for (int polls = 0;; ++polls) {
     v4i d[UB];
     xload_multi<UB>(p, d);
     bool done = true;
@@ -108,6 +115,11 @@ __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row
         done = false;
     }
     if (__ballot(!done) == 0) break;
+    if (polls >= g.limit) {
+      if ((threadIdx.x & 63) == 0) g.fail(kErrSweepWait);
+      g.failed = true;
+      break;
+    }
     __builtin_amdgcn_s_sleep(1);
   }
 }
@@ -422,7 +434,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ B, int64_t ldb,
     T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk,
-    const T* __restrict__ Li, const int* __restrict__ stop) {
+    const T* __restrict__ Li, const int* __restrict__ stop, unsigned* __restrict__ status,
+    int poll_limit) {
   constexpr int NB = kSolveBlock, H = NB / 2;
   constexpr int NPK = INV ? NB * kInvLd : NB * (NB - 1) / 2;
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
@@ -433,6 +446,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
   // records arrive, z_J = u_J - G_J z_{J-1} is one product away
   __shared__ __attribute__((aligned(16))) T Gs[INV ? NB * kInvLd : 4];
   if (stop && *stop) return;  // every workgroup reads the same flag
+  WaitGuard wg{status, poll_limit, false};
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
@@ -496,7 +510,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
         if (edge2 && I + k / 2 == J - 1 && (k & 1)) fa[k][15] = T(0);
       }
       double zv[2 * UB];
-      poll_rows_multi<NR, 2 * UB>(rec, row0, n, tag, zv);
+      poll_rows_multi<NR, 2 * UB>(rec, row0, n, tag, zv, wg);
 #pragma unroll
       for (int k = 0; k < 2 * UB; ++k)
 #pragma unroll
@@ -544,7 +558,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
         const int rb = (J - 1) * NB + 32 * w;
         const int row0[2] = {rb, rb + 16};
         double zv[2];
-        poll_rows_multi<NR, 2>(rec, row0, n, tag, zv);
+        poll_rows_multi<NR, 2>(rec, row0, n, tag, zv, wg);
         if ((lane & 3) < NR) {
 #pragma unroll
           for (int k = 0; k < 2; ++k)
@@ -648,7 +662,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     int n, const T* __restrict__ F, int64_t lda, const int* __restrict__ piv,
     const int* __restrict__ perm, const T* __restrict__ Z, int64_t ldz,
     T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk,
-    const T* __restrict__ Li, const int* __restrict__ stop) {
+    const T* __restrict__ Li, const int* __restrict__ stop, unsigned* __restrict__ status,
+    int poll_limit) {
   constexpr int NB = kSolveBlock, H = NB / 2;
   constexpr int NPK = INV ? NB * kInvLd : NB * (NB - 1) / 2;
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
@@ -659,6 +674,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
   // v_{I+1} is one product away
   __shared__ __attribute__((aligned(16))) T Hs[INV ? NB * kInvLd : 4];
   if (stop && *stop) return;  // every workgroup reads the same flag
+  WaitGuard wg{status, poll_limit, false};
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
@@ -754,7 +770,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
         if (edge2 && Ju == I + 1 && (k & 1) == 0) fb[k][0] = T(0);
       }
       double vv[2 * UB];
-      poll_rows_multi<NR, 2 * UB>(rec, row0, n, tag, vv);
+      poll_rows_multi<NR, 2 * UB>(rec, row0, n, tag, vv, wg);
 #pragma unroll
       for (int k = 0; k < 2 * UB; ++k)
 #pragma unroll
@@ -802,7 +818,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
         const int cb = (I + 1) * NB + 32 * w;
         const int row0[2] = {cb, cb + 16};
         double vv[2];
-        poll_rows_multi<NR, 2>(rec, row0, n, tag, vv);
+        poll_rows_multi<NR, 2>(rec, row0, n, tag, vv, wg);
         if ((lane & 3) < NR) {
 #pragma unroll
           for (int k = 0; k < 2; ++k)
@@ -1247,8 +1263,10 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     XRec* r_ = rf;
     const T* Li_ = Li;
     const int* stop_ = stop;
-    void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &B_,   &ldb_,
-                    &Z_,  &ldz,  &r_,   &tag,   &nblk_, &Li_,  &stop_};
+    unsigned* st_ = h->d_status;
+    int pl_ = h->poll_limit_for_kernels();
+    void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &B_,   &ldb_, &Z_,
+                    &ldz, &r_,   &tag,  &nblk_, &Li_,   &stop_, &st_, &pl_};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR, INV>),
                                               dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
@@ -1264,8 +1282,10 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     XRec* r_ = rb;
     const T* Li_ = Li;
     const int* stop_ = stop;
-    void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &Z_,   &ldz,
-                    &X_,  &ldx_, &r_,   &tag,   &nblk_, &Li_,  &stop_};
+    unsigned* st_ = h->d_status;
+    int pl_ = h->poll_limit_for_kernels();
+    void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &Z_,   &ldz,  &X_,
+                    &ldx_, &r_,  &tag,  &nblk_, &Li_,   &stop_, &st_, &pl_};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR, INV>),
                                               dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
     if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
@@ -1468,7 +1488,7 @@ int bbk_dsysv(bbk_handle_t handle, int n, int nrhs, const double* d_A, int64_t l
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return h->hip_fail(e, "read backward error");
   if (berr_out) *berr_out = *std::max_element(he.begin(), he.end());
-  return BBK_OK;
+  return h->check_status();
 }
 
 int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int64_t lda,
@@ -1597,6 +1617,8 @@ int bbk_dsysv_mixed(bbk_handle_t handle, int n, int nrhs, const double* d_A, int
   }
   if (iters) *iters = it;
   if (berr_out) *berr_out = best;
+  rc = h->check_status();
+  if (rc != BBK_OK) return rc;
   return best <= tol ? BBK_OK : BBK_ERR_NOT_CONVERGED;
 }
 
