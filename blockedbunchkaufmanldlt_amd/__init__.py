@@ -169,9 +169,10 @@ class Handle:
     def synchronize(self):
         self.check(lib().bbk_synchronize(self.h), "synchronize")
 
-    def set_wait_limit(self, polls: int):
-        """Polls per cooperative-kernel wait (0 = default, < 0 = fail at once)."""
-        self.check(lib().bbk_set_wait_limit(self.h, polls), "set_wait_limit")
+    def set_wait_limit(self, ms: int):
+        """Time limit of a cooperative launch's waits in ms (0 = default,
+        < 0 = every wait fails at once)."""
+        self.check(lib().bbk_set_wait_limit(self.h, ms), "set_wait_limit")
 
     def info(self):
         """(info, nonfinite) of the last factorization: info = k + 1 for the

@@ -84,7 +84,7 @@ int default_panel_width(int n, int elem_bytes) {
 template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
                 int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
-                int poll_limit) {
+                long long wait_ticks) {
   launch_init(n, perm, piv, fs.state, status, st);
   int panels = 0;
   for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
@@ -106,7 +106,7 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
     // only every 8th workgroup works, when all of them fit co-resident.
     pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
     pa.status = status;
-    pa.poll_limit = poll_limit;
+    pa.wait_ticks = wait_ticks;
     if (h) h->timing_mark(Handle::kPanel, true);
     hipError_t e = launch_panel<T>(A, pa, groups * pa.xcd_stride, st);
     if (e != hipSuccess) return h ? h->hip_fail(e, "panel launch") : BBK_ERR_HIP;
@@ -121,15 +121,16 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
     if (h) h->timing_mark(Handle::kUpdate, false);
   }
   if (h && h->timing) h->panels += panels;
+  if (status) launch_factor_info<T>(n, A, lda, piv, status, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return h ? h->hip_fail(e, "kernel launch") : BBK_ERR_HIP;
   return BBK_OK;
 }
 
 template int factor_loop<double>(int, int, double*, int64_t, int*, int*, const FactorScratch&,
-                                 int, int*, hipStream_t, Handle*, unsigned*, int);
+                                 int, int*, hipStream_t, Handle*, unsigned*, long long);
 template int factor_loop<float>(int, int, float*, int64_t, int*, int*, const FactorScratch&,
-                                int, int*, hipStream_t, Handle*, unsigned*, int);
+                                int, int*, hipStream_t, Handle*, unsigned*, long long);
 
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
@@ -169,7 +170,7 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     h->epoch = 1024;
   }
   return factor_loop<T>(n, b, A, lda, perm, piv, fs, max_groups, &h->epoch, st, h, h->d_status,
-                        h->poll_limit_for_kernels());
+                        h->wait_ticks());
 }
 
 template int factorize<double>(Handle*, int, int, double*, int64_t, int*, int*);
@@ -184,7 +185,7 @@ int Handle::check_status() {
   if (w[kStatusError] != 0) {
     launch_zero_u32(d_status + kStatusError, 1, stream);
     (void)hipStreamSynchronize(stream);
-    last_error = std::string("a device wait exceeded its poll limit (") +
+    last_error = std::string("a device wait exceeded its time limit (") +
                  ((w[kStatusError] & kErrPanelWait) ? "panel exchange" : "") +
                  ((w[kStatusError] & kErrPanelWait) && (w[kStatusError] & kErrSweepWait) ? ", " : "") +
                  ((w[kStatusError] & kErrSweepWait) ? "triangular sweep" : "") +
@@ -261,6 +262,10 @@ int bbk_create(bbk_handle_t* handle, void* stream) {
     }
     h->own_stream = true;
   }
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device) == hipSuccess &&
+      khz > 0)
+    h->clock_khz = khz;
   e = hipMalloc(&h->d_status, sizeof(unsigned) * bbk::kStatusWords);
   if (e != hipSuccess) {
     h->d_status = nullptr;
@@ -301,9 +306,9 @@ int bbk_synchronize(bbk_handle_t handle) {
   return h->check_status();
 }
 
-int bbk_set_wait_limit(bbk_handle_t handle, int polls) {
+int bbk_set_wait_limit(bbk_handle_t handle, int ms) {
   if (!handle) return -1;
-  reinterpret_cast<Handle*>(handle)->poll_limit = polls;
+  reinterpret_cast<Handle*>(handle)->wait_ms = ms;
   return BBK_OK;
 }
 

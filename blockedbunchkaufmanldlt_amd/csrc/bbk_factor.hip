@@ -125,6 +125,67 @@ __global__ void bk_zero_u32_kernel(unsigned* __restrict__ p, int count) {
   if (i < count) p[i] = 0u;
 }
 
+// After a factorization: the first singular pivot (LAPACK's info: column k
+// + 1 of a 1x1 pivot with |d| <= DBL_EPSILON, whose L column the scaling
+// set to zero, or of a 2x2 block with a zero determinant, evaluated in
+// binary64 without contraction as the scaling does) and whether D holds a
+// NaN or an infinity, into the status words.  One workgroup; O(n) reads.
+template <typename T>
+__global__ __launch_bounds__(256) void bk_factor_info_kernel(int n, const T* __restrict__ A,
+                                                             int64_t lda,
+                                                             const int* __restrict__ piv,
+                                                             unsigned* __restrict__ status) {
+#pragma clang fp contract(off)
+  __shared__ unsigned smin[4];
+  __shared__ unsigned snf[4];
+  unsigned first = kInfoNone, nf = 0u;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    const int t = piv[k];
+    const double d = static_cast<double>(A[(int64_t)k * lda + k]);
+    if (!isfinite(d)) nf = 1u;
+    bool singular = false;
+    if (t == 2 && k + 1 < n) {
+      const double b = static_cast<double>(A[(int64_t)k * lda + k + 1]);
+      const double c = static_cast<double>(A[(int64_t)(k + 1) * lda + k + 1]);
+      if (!isfinite(b)) nf = 1u;
+      singular = d * c - b * b == 0.0;
+    } else if (t == 1) {
+      singular = fabs(d) <= kDblEps;
+    }
+    if (singular) first = min(first, static_cast<unsigned>(k + 1));
+  }
+  // block reduction: minimum index, or of the flags
+  for (int off = 32; off > 0; off >>= 1) {
+    first = min(first, static_cast<unsigned>(__shfl_xor(static_cast<int>(first), off, 64)));
+    nf |= static_cast<unsigned>(__shfl_xor(static_cast<int>(nf), off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    smin[threadIdx.x >> 6] = first;
+    snf[threadIdx.x >> 6] = nf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned m = smin[0], f = snf[0];
+    for (int w = 1; w < 4; ++w) {
+      m = min(m, smin[w]);
+      f |= snf[w];
+    }
+    status[kStatusInfo] = m;
+    status[kStatusNonFinite] = f;
+  }
+}
+
+template <typename T>
+void launch_factor_info(int n, const T* A, int64_t lda, const int* piv, unsigned* status,
+                        hipStream_t stream) {
+  hipLaunchKernelGGL(bk_factor_info_kernel<T>, dim3(1), dim3(256), 0, stream, n, A, lda, piv,
+                     status);
+}
+template void launch_factor_info<double>(int, const double*, int64_t, const int*, unsigned*,
+                                         hipStream_t);
+template void launch_factor_info<float>(int, const float*, int64_t, const int*, unsigned*,
+                                        hipStream_t);
+
 // ---------------------------------------------------------------------
 // Launchers.
 template <typename T>

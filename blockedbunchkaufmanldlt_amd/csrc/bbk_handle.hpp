@@ -41,8 +41,12 @@ struct Handle {
   // Device status words (kStatusWords unsigned, bbk_internal.hpp): wait
   // errors, the last factorization's singular-pivot info and non-finite flag.
   unsigned* d_status = nullptr;
-  int poll_limit = 0;  // exchange / sweep waits: 0 = kDefaultPollLimit, < 0 = fail at once
-  int poll_limit_for_kernels() const { return poll_limit == 0 ? kDefaultPollLimit : poll_limit; }
+  int wait_ms = 0;          // exchange / sweep wait limit: 0 = kDefaultWaitMs, < 0 = fail at once
+  long long clock_khz = 100000;  // device wall-clock rate (ticks per ms)
+  long long wait_ticks() const {
+    if (wait_ms < 0) return -1;
+    return static_cast<long long>(wait_ms == 0 ? kDefaultWaitMs : wait_ms) * clock_khz;
+  }
   // Reads the status words after the stream has finished; a wait error is
   // cleared and returned as BBK_ERR_HIP (with last_error set).
   int check_status();
@@ -82,7 +86,7 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
 template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
                 int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
-                int poll_limit);
+                long long wait_ticks);
 // Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
 // refinement's solves instead of substitution, with F's strictly lower
 // triangle holding the mirror of the upper one; NULL substitutes (F upper

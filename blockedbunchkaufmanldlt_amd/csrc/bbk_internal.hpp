@@ -86,56 +86,66 @@ __device__ __forceinline__ bool xget(const XRec* p, int tag, int* aux, double* v
 }
 
 // Device status words of a handle (4 x 32 bits in device memory):
-//   kStatusError  bit 1: a panel exchange wait ran past its poll limit;
+//   kStatusError  bit 1: a panel exchange wait ran past its time limit;
 //                 bit 2: a triangular-sweep wait did (results invalid).
 //                 Kept until the host reads and clears it.
 //   kStatusInfo   first singular pivot of the last factorization, as
 //                 LAPACK's info: column k + 1 of a 1x1 pivot with
 //                 |d| <= DBL_EPSILON or a 2x2 block with det = 0
 //                 (0xffffffff = none).
-//   kStatusNonFinite  1 when the last factorization met a NaN or an
-//                 infinity in a pivot row.
+//   kStatusNonFinite  1 when the last factorization's D holds a NaN or
+//                 an infinity.
 constexpr int kStatusError = 0, kStatusInfo = 1, kStatusNonFinite = 2, kStatusWords = 4;
 constexpr unsigned kErrPanelWait = 1u, kErrSweepWait = 2u;
 constexpr unsigned kInfoNone = 0xffffffffu;
-// Poll limit given to the kernels when the handle's is 0 (about seconds,
-// far beyond any wait between co-resident workgroups).
-constexpr int kDefaultPollLimit = 1 << 22;
+// Time limit of a launch's waits when the handle sets none (far beyond
+// any wait between co-resident workgroups of one launch).
+constexpr int kDefaultWaitMs = 10000;
 
 // Bounded waits of a cooperative launch.  Every writer is a co-resident
-// workgroup of the same launch, so a wait ends unless a record is lost;
-// the poll limit turns such a loss into an error the host reports instead
-// of a hang.  A thread that ran out of polls marks the error once and from
-// then on returns at once (aux INT_MAX, value 0: never a valid index), so
-// the launch still terminates.  A negative limit fails every wait at once
-// (host-forced, to test the error path).
+// workgroup of the same launch, so a wait ends unless a record is lost; a
+// deadline on the device's constant-rate clock turns such a loss into an
+// error the host reports instead of a hang.  Past the deadline every wait
+// of the launch returns after its first poll (missing values: aux INT_MAX,
+// value 0, never a valid index), so the launch still terminates.  forced:
+// every wait fails at once (host-forced, to test the error path).  All
+// fields are launch-uniform; the waits are wavefront-uniform loops.
 struct WaitGuard {
   unsigned* status;
-  int limit;
-  bool failed;
-  __device__ __forceinline__ void fail(unsigned bit) {
-    if (!failed && status) atomicOr(status + kStatusError, bit);
-    failed = true;
+  unsigned long long deadline;
+  bool forced;
+  __device__ __forceinline__ WaitGuard(unsigned* st, long long ticks)
+      : status(st), deadline(wall_clock64() + static_cast<unsigned long long>(ticks < 0 ? 0 : ticks)),
+        forced(ticks < 0) {}
+  __device__ __forceinline__ void fail(unsigned bit) const {
+    if (status) atomicOr(status + kStatusError, bit);
   }
 };
 
-// Poll one record until it carries `tag`, within the guard's limit.
-__device__ __forceinline__ double xwait(const XRec* p, int tag, int* aux, WaitGuard& g) {
+// A record no exchange ever writes, whose tag (0) no exchange uses: a wait
+// past its deadline re-targets its polls here, so the poll loop keeps a
+// single exit (the common path costs what an unbounded loop costs).
+__device__ const XRec kTimeoutRecord = {0, INT_MAX, 0.0};
+
+// Poll one record until it carries `tag`, within the guard's deadline
+// (every 64th poll reads the clock).  Past it, or at once when forced, the
+// poll is re-targeted to kTimeoutRecord (aux INT_MAX, value 0) and the
+// error is marked.
+__device__ __forceinline__ double xwait(const XRec* p, int tag, int* aux, const WaitGuard& g) {
   double v;
-  if (g.failed || g.limit < 0) {
-    g.fail(kErrPanelWait);
-    *aux = INT_MAX;
-    return 0.0;
+  if (g.forced) {
+    p = &kTimeoutRecord;
+    tag = 0;
   }
-  int polls = 0;
+  unsigned polls = 0;
   while (!xget(p, tag, aux, &v)) {
-    if (++polls > g.limit) {
-      g.fail(kErrPanelWait);
-      *aux = INT_MAX;
-      return 0.0;
-    }
     __builtin_amdgcn_s_sleep(1);
+    if ((++polls & 63u) == 0 && wall_clock64() > g.deadline) {
+      p = &kTimeoutRecord;
+      tag = 0;
+    }
   }
+  if (tag == 0) g.fail(kErrPanelWait);
   return v;
 }
 

@@ -20,7 +20,7 @@ struct PanelArgs {
   int64_t ldw;
   int xcd_stride;   // 8: only workgroups blockIdx % 8 == 0 work (one XCD); 1: all
   unsigned* status; // kStatusWords device status words (may be NULL)
-  int poll_limit;   // exchange waits: polls before giving up (< 0: fail at once)
+  long long wait_ticks;  // exchange waits: device-clock ticks before giving up (< 0: fail at once)
 };
 
 // Exchange records the panel kernel needs for a grid of `groups`.
@@ -37,6 +37,11 @@ template <typename T>
 void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                             const int* state, const int* swaps, int n, int m_bound,
                             int rows_bound, hipStream_t stream);
+// After the last panel: the first singular pivot and the non-finite flag
+// of the factorization into the status words.
+template <typename T>
+void launch_factor_info(int n, const T* A, int64_t lda, const int* piv, unsigned* status,
+                        hipStream_t stream);
 // perm = identity, piv = 0, panel state cleared; with status, the info and
 // non-finite words of a new factorization reset.
 void launch_init(int n, int* perm, int* piv, int* state, unsigned* status, hipStream_t stream);

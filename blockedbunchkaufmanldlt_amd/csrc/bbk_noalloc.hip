@@ -45,6 +45,7 @@ struct NoAllocDevice {
   void* ptr = nullptr;  // device address of `scratch` on this device
   int epoch = 1024;     // next exchange-tag range
   bool cleared = false;
+  long long clock_khz = 0;  // device wall-clock rate
 };
 
 std::mutex g_noalloc_mu;
@@ -98,7 +99,15 @@ extern "C" int bbk_dsytrf_noalloc(int n, int nb, double* d_A, int64_t lda, int* 
     d.epoch = 1024;
   }
   // no handle, so no status words: a lost exchange record ends the waits
-  // after the default poll limit without an error report
+  // at the default time limit without an error report
+  if (d.clock_khz == 0) {
+    int khz = 0;
+    d.clock_khz = (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) ==
+                       hipSuccess &&
+                   khz > 0)
+                      ? khz
+                      : 100000;
+  }
   return factor_loop<double>(n, b, d_A, lda, d_perm, d_piv, fs, max_groups, &d.epoch, st,
-                             nullptr, nullptr, kDefaultPollLimit);
+                             nullptr, nullptr, static_cast<long long>(kDefaultWaitMs) * d.clock_khz);
 }

@@ -430,8 +430,7 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
   int lp = 0;        // Lbuf index of the current pivot
 
   auto area = [&](int tag) -> XRec* { return xbase + (tag & 1) * PS; };
-  WaitGuard wg{p.status, p.poll_limit, false};
-  bool nonfinite = false;  // a NaN or infinity seen in this thread's pivot-row values
+  const WaitGuard wg(p.status, p.wait_ticks);
 
   // L2-scope exchanges only when every working group reports the same
   // XCC_ID (one device-coherent exchange per launch); otherwise every store
@@ -521,7 +520,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       dotA = have_next ? pnext : panel_dot<T, WS, RR>(Lk, Wl, wr, tid, kk, base);
       w = arow - dotA;
       Wl[(kk - base) * WS + tid] = w;
-      nonfinite |= !isfinite(static_cast<double>(w));
     }
     have_next = false;
     BBK_PT(1);
@@ -718,7 +716,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], w);
         p.piv[k] = 1;
-        if (den == 0.0 && p.status) atomicMin(p.status + kStatusInfo, static_cast<unsigned>(k + 1));
       }
       if (has1) {
         const T lkk = static_cast<T>(div_or_zero(pubv[kPubWk1], den));
@@ -755,7 +752,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], nw);
         p.piv[k] = 1;
-        if (den == 0.0 && p.status) atomicMin(p.status + kStatusInfo, static_cast<unsigned>(k + 1));
       }
       if (gi == 0 && tid == 0) {
         swl[2 * nswap] = k;
@@ -825,11 +821,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
       if (j == k) {
         sta(&A[(int64_t)k * lda + k], static_cast<T>(a));
         p.piv[k] = 2;
-        {
-#pragma clang fp contract(off)
-          if (a * cc - bb * bb == 0.0 && p.status)
-            atomicMin(p.status + kStatusInfo, static_cast<unsigned>(k + 1));
-        }
       }
       if (j == k1) {
         sta(&A[(int64_t)k * lda + k1], static_cast<T>(bb));
@@ -894,7 +885,6 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
     if (t0 < ne) p.perm[swl[t0]] = v0;
     if (t1 < ne) p.perm[swl[t1]] = v1;
   }
-  if (nonfinite && p.status) atomicOr(p.status + kStatusNonFinite, 1u);
   if (gi == 0 && tid == 0) {
     p.state[PanelState::kStart] = s;
     p.state[PanelState::kWidth] = kk;

@@ -81,7 +81,7 @@ __device__ __forceinline__ void xload_multi<4>(const XRec* const (&p)[4], v4i (&
 
 template <int NR, int UB>
 __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row0)[UB], int nlim,
-                                                int tag, double (&out)[UB], WaitGuard& g) {
+                                                int tag, double (&out)[UB], const WaitGuard& g) {
   const int lane = threadIdx.x & 63;
   const int q = lane & 3, rr = lane >> 2;
   const int qq = q < NR ? q : NR - 1;
@@ -94,14 +94,13 @@ __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row
     p[u] = rec + (int64_t)(row < nlim ? row : nlim - 1) * kSolveMaxRhs + qq;
     out[u] = 0.0;
   }
-  // bounded (WaitGuard): past the poll limit the missing values stay 0,
-  // the error is marked once and later polls of this thread return at once
-  if (g.failed || g.limit < 0) {
-    if ((threadIdx.x & 63) == 0) g.fail(kErrSweepWait);
-    g.failed = true;
+  // bounded (WaitGuard): past the deadline the missing values stay 0 and
+  // the error is marked
+  if (g.forced) {
+    g.fail(kErrSweepWait);
     return;
   }
-  for (int polls = 0;; ++polls) {
+  for (unsigned polls = 1;; ++polls) {
     v4i d[UB];
     xload_multi<UB>(p, d);
     bool done = true;
@@ -115,9 +114,8 @@ __device__ __forceinline__ void poll_rows_multi(const XRec* rec, const int (&row
         done = false;
     }
     if (__ballot(!done) == 0) break;
-    if (polls >= g.limit) {
-      if ((threadIdx.x & 63) == 0) g.fail(kErrSweepWait);
-      g.failed = true;
+    if ((polls & 63u) == 0 && wall_clock64() > g.deadline) {
+      g.fail(kErrSweepWait);
       break;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -435,7 +433,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
     const int* __restrict__ perm, const T* __restrict__ B, int64_t ldb,
     T* __restrict__ Z, int64_t ldz, XRec* __restrict__ rec, int tag, int nblk,
     const T* __restrict__ Li, const int* __restrict__ stop, unsigned* __restrict__ status,
-    int poll_limit) {
+    long long wait_ticks) {
   constexpr int NB = kSolveBlock, H = NB / 2;
   constexpr int NPK = INV ? NB * kInvLd : NB * (NB - 1) / 2;
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
@@ -446,7 +444,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_fwd(
   // records arrive, z_J = u_J - G_J z_{J-1} is one product away
   __shared__ __attribute__((aligned(16))) T Gs[INV ? NB * kInvLd : 4];
   if (stop && *stop) return;  // every workgroup reads the same flag
-  WaitGuard wg{status, poll_limit, false};
+  const WaitGuard wg(status, wait_ticks);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
   for (int J = blockIdx.x; J < nblk; J += gridDim.x) {
@@ -663,7 +661,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
     const int* __restrict__ perm, const T* __restrict__ Z, int64_t ldz,
     T* __restrict__ X, int64_t ldx, XRec* __restrict__ rec, int tag, int nblk,
     const T* __restrict__ Li, const int* __restrict__ stop, unsigned* __restrict__ status,
-    int poll_limit) {
+    long long wait_ticks) {
   constexpr int NB = kSolveBlock, H = NB / 2;
   constexpr int NPK = INV ? NB * kInvLd : NB * (NB - 1) / 2;
   __shared__ __attribute__((aligned(16))) T Lp[NPK];  // packed L, or Li (rows of kInvLd)
@@ -674,7 +672,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_trsv_bwd(
   // v_{I+1} is one product away
   __shared__ __attribute__((aligned(16))) T Hs[INV ? NB * kInvLd : 4];
   if (stop && *stop) return;  // every workgroup reads the same flag
-  WaitGuard wg{status, poll_limit, false};
+  const WaitGuard wg(status, wait_ticks);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int LAST = INV ? 1 : 0;  // blocks applied after the main loop
   for (int I = nblk - 1 - blockIdx.x; I >= 0; I -= gridDim.x) {
@@ -1264,7 +1262,7 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     const T* Li_ = Li;
     const int* stop_ = stop;
     unsigned* st_ = h->d_status;
-    int pl_ = h->poll_limit_for_kernels();
+    long long pl_ = h->wait_ticks();
     void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &B_,   &ldb_, &Z_,
                     &ldz, &r_,   &tag,  &nblk_, &Li_,   &stop_, &st_, &pl_};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR, INV>),
@@ -1283,7 +1281,7 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     const T* Li_ = Li;
     const int* stop_ = stop;
     unsigned* st_ = h->d_status;
-    int pl_ = h->poll_limit_for_kernels();
+    long long pl_ = h->wait_ticks();
     void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &Z_,   &ldz,  &X_,
                     &ldx_, &r_,  &tag,  &nblk_, &Li_,   &stop_, &st_, &pl_};
     hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR, INV>),

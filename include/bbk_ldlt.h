@@ -63,22 +63,23 @@ const char* bbk_last_error(bbk_handle_t handle);
 /* Failure reporting (no counterpart in the reference, which reports only
  * argument errors, by exception).
  * The cooperative kernels (panel exchanges, triangular sweeps) wait for
- * records written by co-resident workgroups of the same launch.  Each wait
- * is bounded by a poll limit; a wait that runs past it marks an error word
- * on the device and the kernel runs to its end with invalid values instead
- * of hanging.  bbk_synchronize and every synchronous entry point then
- * return BBK_ERR_HIP (bbk_last_error says which wait) and clear the word.
- * bbk_set_wait_limit: polls per wait; 0 = the default (about seconds of
- * polling, far beyond any real wait); a negative value makes every wait
- * fail at once (for testing the error path). */
-int bbk_set_wait_limit(bbk_handle_t handle, int polls);
+ * records written by co-resident workgroups of the same launch.  The waits
+ * of a launch are bounded by a deadline on the device clock; a wait that
+ * runs past it marks an error word on the device and the launch runs to
+ * its end with invalid values instead of hanging.  bbk_synchronize and
+ * every synchronous entry point then return BBK_ERR_HIP (bbk_last_error
+ * says which wait) and clear the word.
+ * bbk_set_wait_limit: milliseconds per launch; 0 = the default (10 s, far
+ * beyond any real wait); a negative value makes every wait fail at once
+ * (for testing the error path). */
+int bbk_set_wait_limit(bbk_handle_t handle, int ms);
 /* After the last factorization queued on the handle (waits for the
  * stream): info = k + 1 for the first column k whose 1x1 pivot had
  * |d| <= DBL_EPSILON (its L column is set to zero, as the reference's
  * SafeDenominator does, src/block_ldlt_decomposition.cc:21-26) or whose 2x2
  * block had a zero determinant, 0 when there is none (LAPACK's dsytrf
- * info convention); nonfinite = 1 when a NaN or an infinity appeared in a
- * pivot row.  Either pointer may be NULL.  Returns BBK_ERR_HIP as
+ * info convention); nonfinite = 1 when D holds a NaN or an infinity (a
+ * non-finite input reaches D).  Either pointer may be NULL.  Returns BBK_ERR_HIP as
  * bbk_synchronize does when a wait error is pending. */
 int bbk_get_info(bbk_handle_t handle, int* info, int* nonfinite);
 

@@ -1,7 +1,7 @@
 """GPU tests of failure reporting: the bounded waits of the cooperative
 kernels and the device info word.
 
-The wait-limit tests force the error from the host (a negative poll limit
+The wait-limit tests force the error from the host (a negative time limit
 makes every wait of the panel exchanges and of the triangular sweeps fail
 at once): the kernels still run to their end, the handle reports
 BBK_ERR_HIP with the wait named, and the next call with the default limit
@@ -10,7 +10,8 @@ is correct again.  No kernel faults or hangs in these tests.
 The info word follows LAPACK dsytrf's info: the first column whose 1x1
 pivot is singular (|d| <= DBL_EPSILON, where the reference's
 SafeDenominator zeroes the L column, src/block_ldlt_decomposition.cc:21-26),
-plus a flag for NaN or infinite values met in a pivot row.
+plus a flag for NaN or infinite values in D (where a non-finite input
+ends up).
 """
 import numpy as np
 import pytest
