@@ -86,6 +86,8 @@ def test_batched_small_widths_defer_2x2(oracle, handle, bw, nt):
     reference defers them (the next panel starts one column early).  Both
     team sizes (one wavefront / four per matrix)."""
     import torch
+    saved_w = handle.batched_widths(torch.float64)
+    saved_nt = handle.batched_team_sizes(torch.float64)
     handle.set_batched_widths(torch.float64, [bw] * 8)
     handle.set_batched_team_sizes(torch.float64, [nt] * 8)
     try:
@@ -106,8 +108,8 @@ def test_batched_small_widths_defer_2x2(oracle, handle, bw, nt):
         assert checked >= 3
         assert any((g.piv == 2).any() for g in outs)
     finally:
-        handle.set_batched_widths(torch.float64, [8, 16, 16, 32, 32, 32, 32, 32])
-        handle.set_batched_team_sizes(torch.float64, [256] * 8)
+        handle.set_batched_widths(torch.float64, saved_w)
+        handle.set_batched_team_sizes(torch.float64, saved_nt)
 
 
 RESIDENT_NS = [1, 2, 3, 7, 16, 31, 32, 33, 47, 64, 65, 100, 127, 128, 129, 150, 176, 177, 200,
@@ -183,15 +185,19 @@ def test_batched_global_workspace_path(oracle, handle):
 @pytest.mark.parametrize("nt", [64, 256])
 def test_batched_fp32(oracle, handle, nt):
     import torch
+    saved_nt = handle.batched_team_sizes(torch.float32)
     handle.set_batched_team_sizes(torch.float32, [nt] * 8)
     ns = [20, 64, 100, 180, 256, 400]
     mats = handle.gen_synth_batched(ns, seed=31337, scale=1.0, zero_diag_stride=7,
                                     dtype=torch.float32)
     hosts = [host_matrix(m) for m in mats]  # the binary32 values, widened
     widths = handle.batched_widths(torch.float32)
-    outs = factor_batch(handle, oracle, mats, torch.float32)
-    for A, g in zip(hosts, outs):
-        check(oracle, A, g, width_for(widths, A.shape[0]), fp32=True)
+    try:
+        outs = factor_batch(handle, oracle, mats, torch.float32)
+        for A, g in zip(hosts, outs):
+            check(oracle, A, g, width_for(widths, A.shape[0]), fp32=True)
+    finally:
+        handle.set_batched_team_sizes(torch.float32, saved_nt)
 
 
 def test_batched_equals_single_matrix_path(oracle, handle):
