@@ -9,6 +9,7 @@
 #include "../../include/bbk_ldlt.h"
 #include "bbk_handle.hpp"
 #include "bbk_kernels.hpp"
+#include "bbk_update.hpp"
 
 namespace bbk {
 
@@ -43,27 +44,92 @@ __global__ void k_scale_2x2(int count, double a, double b, double c,
 }
 
 // kernels::UpdateTrailingBlock, src/kernels.cc:67-91:
-// target(i,j) -= sum_k left(k,i) * work(k,j); 16x16 output per wave via
-// the f64 MFMA (16x16x4), one wave per 16x16 tile.
-__global__ __launch_bounds__(64) void k_update_trailing(
-    int rows, int cols, int inner, const double* left, int ls,
-    const double* work, int ws, double* target, int ts) {
-  typedef double acc_t __attribute__((ext_vector_type(4)));
-  const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
-  const int lane = threadIdx.x;
-  acc_t acc = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < inner; k0 += 4) {
-    const int k = k0 + (lane >> 4);
-    const int ii = i0 + (lane & 15), jj = j0 + (lane & 15);
-    const double a = (k < inner && ii < rows) ? left[(int64_t)k * ls + ii] : 0.0;
-    const double b = (k < inner && jj < cols) ? work[(int64_t)k * ws + jj] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+// target(i,j) -= sum_k left(k,i) * work(k,j).  One 64 x 64 target tile per
+// workgroup of four wavefronts (32 x 32 each, 2 x 2 blocks of the binary64
+// 16x16x4 MFMA).  The accumulators start from the target tile and the left
+// operand is negated while it is staged, so the MFMA chain computes
+// target - left^T work in place; K is staged through two LDS buffers in
+// chunks of 8 rows, the next chunk's global loads overlapping the current
+// chunk's MFMAs.  Every access is guarded against rows, cols and inner.
+__global__ __launch_bounds__(256) void k_update_trailing(
+    int rows, int cols, int inner, const double* __restrict__ left, int ls,
+    const double* __restrict__ work, int ws, double* __restrict__ target, int ts) {
+  constexpr int TM = 64, KC = 8, LD = TM + 4, WT = TM / 2, TI = WT / 16;
+  constexpr int PER = KC * TM / 256;  // staged elements per thread per operand
+  constexpr int RSTEP = 256 / TM;     // K rows per pass of the workgroup
+  typedef Mfma<double>::acc_t acc_t;
+  __shared__ __align__(16) double Ls[2][KC * LD];
+  __shared__ __align__(16) double Rs[2][KC * LD];
+  const int i0 = blockIdx.y * TM, j0 = blockIdx.x * TM;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wr = wave >> 1, wc = wave & 1;
+
+  acc_t acc[TI][TI];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TI; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wr * WT + ti * 16 + Mfma<double>::row(lane, v);
+        const int j = j0 + wc * WT + tj * 16 + (lane & 15);
+        acc[ti][tj][v] = (i < rows && j < cols) ? target[(int64_t)i * ts + j] : 0.0;
+      }
+
+  // wavefront w stages K rows w, w + 4 of a chunk, lane = column
+  const int qw = wave, cw = lane;
+  const bool iok = i0 + cw < rows, jok = j0 + cw < cols;
+  double ru[PER], rw[PER];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int kq = k0 + qw + RSTEP * u;
+      ru[u] = (kq < inner && iok) ? -left[(int64_t)kq * ls + i0 + cw] : 0.0;
+      rw[u] = (kq < inner && jok) ? work[(int64_t)kq * ws + j0 + cw] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      Ls[buf][(qw + RSTEP * u) * LD + cw] = ru[u];
+      Rs[buf][(qw + RSTEP * u) * LD + cw] = rw[u];
+    }
+  };
+  const int nk = (inner + KC - 1) / KC;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) fetch((kc + 1) * KC);
+#pragma unroll
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      double a[TI], b[TI];
+      const int kr = k4 + (lane >> 4);
+#pragma unroll
+      for (int t = 0; t < TI; ++t) {
+        a[t] = Ls[buf][kr * LD + wr * WT + t * 16 + (lane & 15)];
+        b[t] = Rs[buf][kr * LD + wc * WT + t * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TI; ++tj) acc[ti][tj] = Mfma<double>::mma(a[ti], b[tj], acc[ti][tj]);
+    }
+    if (kc + 1 < nk) stash(buf ^ 1);
+    __syncthreads();
   }
-  for (int v = 0; v < 4; ++v) {
-    // f64 16x16x4 output: element v of a lane is row (lane / 16) + 4 v
-    const int i = i0 + (lane >> 4) + 4 * v, j = j0 + (lane & 15);
-    if (i < rows && j < cols) target[(int64_t)i * ts + j] -= acc[v];
-  }
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TI; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + wr * WT + ti * 16 + Mfma<double>::row(lane, v);
+        const int j = j0 + wc * WT + tj * 16 + (lane & 15);
+        if (i < rows && j < cols) target[(int64_t)i * ts + j] = acc[ti][tj][v];
+      }
 }
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
@@ -132,8 +198,9 @@ int bbk_kernel_update_trailing(bbk_handle_t handle, int rows, int cols,
   if (!handle) return -1;
   if (rows <= 0 || cols <= 0 || inner <= 0) return BBK_OK;  // :72-74
   Handle* h = reinterpret_cast<Handle*>(handle);
-  dim3 grid((cols + 15) / 16, (rows + 15) / 16);
-  hipLaunchKernelGGL(bbk::k_update_trailing, grid, dim3(64), 0, h->stream,
+  if ((rows + 63) / 64 > 65535) return h->fail(BBK_ERR_UNSUPPORTED, "update_trailing: rows > 4194240");
+  dim3 grid((cols + 63) / 64, (rows + 63) / 64);
+  hipLaunchKernelGGL(bbk::k_update_trailing, grid, dim3(256), 0, h->stream,
                      rows, cols, inner, d_left, left_stride, d_work,
                      work_stride, d_target, target_stride);
   hipError_t e = hipGetLastError();

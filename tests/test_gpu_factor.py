@@ -201,7 +201,8 @@ def test_kernels_shapes(oracle, handle):
     rng = np.random.default_rng(0)
     L = oracle.lib()
     # update_trailing: rows x cols target, inner rows of operands, strides
-    for (rows, cols, inner) in [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 129)]:
+    for (rows, cols, inner) in [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 129),
+                                (200, 257, 128), (65, 300, 9)]:
         ls, ws, ts = rows + 3, cols + 1, cols + 2
         left = rng.standard_normal(inner * ls)
         work = rng.standard_normal(inner * ws)
