@@ -56,6 +56,7 @@ def lib() -> ctypes.CDLL:
     L.bbk_last_error.argtypes = [vp]
     L.bbk_last_error.restype = ctypes.c_char_p
     L.bbk_set_wait_limit.argtypes = [vp, i32]
+    L.bbk_set_panel_threads.argtypes = [vp, i32]
     L.bbk_get_info.argtypes = [vp, ip, ip]
     for name in ("bbk_dsytrf", "bbk_ssytrf"):
         getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp]
@@ -173,6 +174,11 @@ class Handle:
         """Time limit of a cooperative launch's waits in ms (0 = default,
         < 0 = every wait fails at once)."""
         self.check(lib().bbk_set_wait_limit(self.h, ms), "set_wait_limit")
+
+    def set_panel_threads(self, threads: int):
+        """Panel kernel variant: 0 = by order (256-column workgroups up to
+        n = 65536, 512-column ones up to 131072), 256 or 512 = that one."""
+        self.check(lib().bbk_set_panel_threads(self.h, threads), "set_panel_threads")
 
     def info(self):
         """(info, nonfinite) of the last factorization: info = k + 1 for the

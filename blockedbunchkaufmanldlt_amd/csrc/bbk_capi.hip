@@ -75,6 +75,34 @@ int default_panel_width(int n, int elem_bytes) {
   return 128;
 }
 
+// The panel variant for order n: workgroups of kPanelThreads columns while
+// the grid of ceil(n / 256) groups fits co-resident (n <= 65536), else of
+// kPanelThreadsWide columns (n <= 131072); force (256 or 512) allows that
+// variant only.  *max_groups: co-resident workgroups of that variant.
+// BBK_ERR_UNSUPPORTED past both.
+template <typename T>
+int panel_plan(int n, int force, int* nt, int* max_groups, hipError_t* err) {
+  *err = hipSuccess;
+  for (int cand : {kPanelThreads, kPanelThreadsWide}) {
+    if (force != 0 && cand != force) continue;
+    int g = 0;
+    hipError_t e = panel_max_groups<T>(cand, &g);
+    if (e != hipSuccess) {
+      *err = e;
+      return BBK_ERR_HIP;
+    }
+    if ((n + cand - 1) / cand <= std::min(g, kMaxPanelGroupsLds)) {
+      *nt = cand;
+      *max_groups = g;
+      return BBK_OK;
+    }
+  }
+  return BBK_ERR_UNSUPPORTED;
+}
+
+template int panel_plan<double>(int, int, int*, int*, hipError_t*);
+template int panel_plan<float>(int, int, int*, int*, hipError_t*);
+
 // The panel loop shared by every factorization entry point: queues the
 // init kernel, then per panel the cooperative panel kernel and the trailing
 // update (with the interchanges of the rows above the panel) on `st`.
@@ -83,12 +111,13 @@ int default_panel_width(int n, int elem_bytes) {
 // count.
 template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
-                int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
+                int nt, int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
                 long long wait_ticks) {
+  b = std::min(b, panel_max_width<T>(nt));
   launch_init(n, perm, piv, fs.state, status, st);
   int panels = 0;
   for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
-    const int groups = (n - s_nom + kPanelThreads - 1) / kPanelThreads;
+    const int groups = (n - s_nom + nt - 1) / nt;
     PanelArgs pa;
     pa.n = n;
     pa.b = b;
@@ -108,7 +137,7 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
     pa.status = status;
     pa.wait_ticks = wait_ticks;
     if (h) h->timing_mark(Handle::kPanel, true);
-    hipError_t e = launch_panel<T>(A, pa, groups * pa.xcd_stride, st);
+    hipError_t e = launch_panel<T>(A, pa, groups * pa.xcd_stride, nt, st);
     if (e != hipSuccess) return h ? h->hip_fail(e, "panel launch") : BBK_ERR_HIP;
     if (h) h->timing_mark(Handle::kPanel, false);
     // Rows above this panel: at most s_nom + panels (each earlier panel
@@ -128,9 +157,9 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
 }
 
 template int factor_loop<double>(int, int, double*, int64_t, int*, int*, const FactorScratch&,
-                                 int, int*, hipStream_t, Handle*, unsigned*, long long);
+                                 int, int, int*, hipStream_t, Handle*, unsigned*, long long);
 template int factor_loop<float>(int, int, float*, int64_t, int*, int*, const FactorScratch&,
-                                int, int*, hipStream_t, Handle*, unsigned*, long long);
+                                int, int, int*, hipStream_t, Handle*, unsigned*, long long);
 
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
@@ -142,14 +171,13 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
   if (!perm) return -6;
   if (!piv) return -7;
   int b = nb == 0 ? default_panel_width(n, sizeof(T)) : nb;
-  b = std::min(b, PanelCfg<T>::kMaxWidth);
-  int max_groups = 0;
-  hipError_t e = panel_max_groups<T>(&max_groups);
-  if (e != hipSuccess) return h->hip_fail(e, "occupancy query");
-  const int groups0 = (n + kPanelThreads - 1) / kPanelThreads;
-  if (groups0 > std::min(max_groups, kMaxPanelGroupsLds))
+  int nt = 0, max_groups = 0;
+  hipError_t e = hipSuccess;
+  const int pc = panel_plan<T>(n, h->panel_threads, &nt, &max_groups, &e);
+  if (pc == BBK_ERR_HIP) return h->hip_fail(e, "occupancy query");
+  if (pc != BBK_OK)
     return h->fail(BBK_ERR_UNSUPPORTED,
-                   "matrix order exceeds the co-resident panel grid");
+                   "matrix order exceeds the co-resident panel grid (n <= 131072)");
   FactorScratch fs;
   const size_t bytes = factor_scratch_bytes<T>(n, nullptr, nullptr);
   int rc = h->reserve(bytes);
@@ -169,8 +197,8 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     h->ws_layout = layout;
     h->epoch = 1024;
   }
-  return factor_loop<T>(n, b, A, lda, perm, piv, fs, max_groups, &h->epoch, st, h, h->d_status,
-                        h->wait_ticks());
+  return factor_loop<T>(n, b, A, lda, perm, piv, fs, nt, max_groups, &h->epoch, st, h,
+                        h->d_status, h->wait_ticks());
 }
 
 template int factorize<double>(Handle*, int, int, double*, int64_t, int*, int*);
@@ -304,6 +332,14 @@ int bbk_synchronize(bbk_handle_t handle) {
   hipError_t e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return h->hip_fail(e, "hipStreamSynchronize");
   return h->check_status();
+}
+
+int bbk_set_panel_threads(bbk_handle_t handle, int threads) {
+  if (!handle) return -1;
+  if (threads != 0 && threads != bbk::kPanelThreads && threads != bbk::kPanelThreadsWide)
+    return -2;
+  reinterpret_cast<Handle*>(handle)->panel_threads = threads;
+  return BBK_OK;
 }
 
 int bbk_set_wait_limit(bbk_handle_t handle, int ms) {

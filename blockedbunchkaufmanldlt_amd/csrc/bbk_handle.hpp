@@ -57,6 +57,7 @@ struct Handle {
   std::vector<hipEvent_t> marks[kGroups];
   BatchPlan batch;                 // last prepared batched schedule
   int batch_grid_limit = 0;        // > 0: cap on persistent batched workgroups
+  int panel_threads = 0;           // panel variant: 0 = by order, else 256 / 512
   BatchWidths batch_widths[2] = {kDefaultBatchWidthsF64, kDefaultBatchWidthsF32};
 
   ~Handle();
@@ -85,8 +86,14 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
 // (h: timing and error text, may be NULL).
 template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
-                int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
+                int nt, int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
                 long long wait_ticks);
+// The panel variant for order n (threads per workgroup nt, its co-resident
+// workgroups max_groups; force: 0 = the smallest variant whose grid fits,
+// else that variant only): BBK_OK, BBK_ERR_HIP (err set) or
+// BBK_ERR_UNSUPPORTED when no allowed variant's grid fits.
+template <typename T>
+int panel_plan(int n, int force, int* nt, int* max_groups, hipError_t* err);
 // Li: inverses of L's 128 x 128 diagonal blocks (binary32), used by the
 // refinement's solves instead of substitution, with F's strictly lower
 // triangle holding the mirror of the upper one; NULL substitutes (F upper

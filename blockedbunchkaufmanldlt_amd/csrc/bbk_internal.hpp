@@ -17,10 +17,15 @@ constexpr double kDblEps = 2.220446049250313080847e-16;
 
 // Panel kernel geometry: one thread owns one active column; a workgroup
 // of four wavefronts holds the panel's W rows for its 256 columns in LDS.
+// Orders whose grid of 256-column groups would not fit co-resident
+// (n > 65536) use workgroups of eight wavefronts (512 columns each).
 constexpr int kPanelThreads = 256;
+constexpr int kPanelThreadsWide = 512;
 constexpr int kPanelMaxWidth = 128;  // largest panel width of any type
 constexpr int kMaxPanelGroups = 1024;
-constexpr int kMaxPanelGroupsLds = 256;  // panel grid limit (n <= 65536)
+// panel grid limit: n <= 65536 with 256-column groups, n <= 131072 with
+// 512-column groups
+constexpr int kMaxPanelGroupsLds = 256;
 
 // 16-byte exchange record {tag, aux, value} of the panel kernel; a record
 // is valid for exchange e when tag == e.
@@ -30,25 +35,45 @@ struct __attribute__((aligned(16))) XRec {
   double v;
 };
 
-// Panel width per type: 128.  The panel's W rows live in LDS, (kLdsRows +
-// 1) x 256 elements; once the panel passes row kRegRows, rows [0, kRegRows)
-// move to registers (one column per thread) and the LDS rows are reused for
-// the rest.  The register rows' products need no LDS traffic, so half of
-// the panel's rows sit there in both types.
-template <typename T>
+// Panel width per type and workgroup size.  The panel's W rows live in
+// LDS, (kLdsRows + 1) x NT elements; once the panel passes row kRegRows,
+// rows [0, kRegRows) move to registers (one column per thread) and the LDS
+// rows are reused for the rest.  The register rows' products need no LDS
+// traffic, so half of the panel's rows sit there.  The rows [0, kRegRows]
+// sit in LDS before they move, so kLdsRows >= kRegRows.
+template <typename T, int NT = kPanelThreads>
 struct PanelCfg;
 template <>
-struct PanelCfg<double> {
+struct PanelCfg<double, 256> {
   static constexpr int kLdsRows = 64;  // 65 x 256 x 8 B = 130 KB
   static constexpr int kRegRows = 64;  // 128 VGPRs
   static constexpr int kMaxWidth = kLdsRows + kRegRows;
 };
 template <>
-struct PanelCfg<float> {
+struct PanelCfg<float, 256> {
   static constexpr int kLdsRows = 64;  // 65 x 256 x 4 B = 65 KB
   static constexpr int kRegRows = 64;  // 64 VGPRs
   static constexpr int kMaxWidth = kLdsRows + kRegRows;
 };
+template <>
+struct PanelCfg<double, 512> {
+  static constexpr int kLdsRows = 32;  // 33 x 512 x 8 B = 132 KB
+  static constexpr int kRegRows = 32;  // 64 VGPRs
+  static constexpr int kMaxWidth = kLdsRows + kRegRows;
+};
+template <>
+struct PanelCfg<float, 512> {
+  static constexpr int kLdsRows = 64;  // 65 x 512 x 4 B = 130 KB
+  static constexpr int kRegRows = 64;  // 64 VGPRs
+  static constexpr int kMaxWidth = kLdsRows + kRegRows;
+};
+
+// Largest panel width of the variant with nt threads per workgroup.
+template <typename T>
+constexpr int panel_max_width(int nt) {
+  return nt == kPanelThreadsWide ? PanelCfg<T, kPanelThreadsWide>::kMaxWidth
+                                 : PanelCfg<T, kPanelThreads>::kMaxWidth;
+}
 
 // Per-panel device state (ints).
 struct PanelState {

@@ -26,11 +26,13 @@ struct PanelArgs {
 // Exchange records the panel kernel needs for a grid of `groups`.
 size_t panel_exchange_records(int groups, int elem_bytes);
 
+// nt: threads per workgroup of the panel variant (kPanelThreads or
+// kPanelThreadsWide).
 template <typename T>
-hipError_t launch_panel(T* A, const PanelArgs& args, int groups,
+hipError_t launch_panel(T* A, const PanelArgs& args, int groups, int nt,
                         hipStream_t stream);
 template <typename T>
-hipError_t panel_max_groups(int* groups);
+hipError_t panel_max_groups(int nt, int* groups);
 // Trailing update of the panel just factored plus the interchange of the
 // rows above it (rows_bound: upper bound of the panel start; 0 = none).
 template <typename T>

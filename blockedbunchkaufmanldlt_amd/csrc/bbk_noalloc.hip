@@ -66,17 +66,15 @@ extern "C" int bbk_dsytrf_noalloc(int n, int nb, double* d_A, int64_t lda, int* 
   if (!d_piv) return -6;
   if (!stream) return -7;  // cooperative launches are refused on the null stream
   if (n > kNoAllocMaxN) return BBK_ERR_UNSUPPORTED;
-  int b = nb == 0 ? default_panel_width(n, sizeof(double)) : nb;
-  b = std::min(b, PanelCfg<double>::kMaxWidth);
+  const int b = nb == 0 ? default_panel_width(n, sizeof(double)) : nb;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return BBK_ERR_HIP;
   if (dev < 0 || dev >= kNoAllocMaxDevices) return BBK_ERR_UNSUPPORTED;
-  int max_groups = 0;
-  e = panel_max_groups<double>(&max_groups);
-  if (e != hipSuccess) return BBK_ERR_HIP;
-  if ((n + kPanelThreads - 1) / kPanelThreads > std::min(max_groups, kMaxPanelGroupsLds))
-    return BBK_ERR_UNSUPPORTED;
+  // n <= 16384: the 256-column panel variant, which the scratch is sized for
+  int nt = 0, max_groups = 0;
+  const int pc = panel_plan<double>(n, kPanelThreads, &nt, &max_groups, &e);
+  if (pc != BBK_OK) return pc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 
   std::lock_guard<std::mutex> lock(g_noalloc_mu);
@@ -108,6 +106,6 @@ extern "C" int bbk_dsytrf_noalloc(int n, int nb, double* d_A, int64_t lda, int* 
                       ? khz
                       : 100000;
   }
-  return factor_loop<double>(n, b, d_A, lda, d_perm, d_piv, fs, max_groups, &d.epoch, st,
+  return factor_loop<double>(n, b, d_A, lda, d_perm, d_piv, fs, nt, max_groups, &d.epoch, st,
                              nullptr, nullptr, static_cast<long long>(kDefaultWaitMs) * d.clock_khz);
 }

@@ -1,11 +1,12 @@
 // bbk_panel.hip -- the Bunch-Kaufman panel kernel for gfx950.
 //
 // One cooperative launch factors one panel of up to b columns starting at
-// s (read from device state).  Workgroup g starts with the active columns
-// [s + 256 g, s + 256 g + 256) of rows [s, n); thread t holds one column
+// s (read from device state).  Workgroups of NT threads (256, or 512 for
+// orders past 65536) hold NT active columns each, in 64-column chunks dealt
+// round-robin over the groups (below); thread t holds one column
 // (position) and keeps that column of the panel's unscaled rows W = D L^T
-// in LDS (binary64: the first 64 rows move to registers once the panel
-// passes them).  An interchange swaps the positions of the two threads
+// in LDS (the first kRegRows rows -- 64, or 32 in the 512-thread binary64
+// variant -- move to registers once the panel passes them).  An interchange swaps the positions of the two threads
 // holding the interchanged columns; no W entry ever moves.
 //
 // Per pivot k (local index kk) the workgroups meet at most twice:
@@ -366,12 +367,12 @@ enum PubId {
   kNumPub = 16
 };
 
-template <typename T>
-__global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__ A, PanelArgs p) {
-  constexpr int NT = kPanelThreads;
-  constexpr int BM = PanelCfg<T>::kMaxWidth;
-  constexpr int RR = PanelCfg<T>::kRegRows;
-  constexpr int LR = PanelCfg<T>::kLdsRows;
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void bk_panel_kernel(T* __restrict__ A, PanelArgs p) {
+  constexpr int BM = PanelCfg<T, NT>::kMaxWidth;
+  constexpr int RR = PanelCfg<T, NT>::kRegRows;
+  constexpr int LR = PanelCfg<T, NT>::kLdsRows;
+  static_assert(NT % 64 == 0 && NT >= BM + 2, "a thread per L-row entry of the panel");
   constexpr int NR = RR > 0 ? RR : 1;
   // rows [0, RR], RR + 1 of them, sit in Wl until they move to registers
   static_assert(LR >= RR, "the LDS rows must hold the register rows before they move");
@@ -898,19 +899,23 @@ __global__ __launch_bounds__(kPanelThreads) void bk_panel_kernel(T* __restrict__
 namespace bbk {
 
 template <typename T>
-hipError_t launch_panel(T* A, const PanelArgs& args, int groups, hipStream_t stream) {
+hipError_t launch_panel(T* A, const PanelArgs& args, int groups, int nt, hipStream_t stream) {
   T* a = A;
   PanelArgs pa = args;
   void* kargs[] = {&a, &pa};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&bk_panel_kernel<T>),
-                                    dim3(groups), dim3(kPanelThreads), kargs, 0, stream);
+  const void* fn = nt == kPanelThreadsWide
+                       ? reinterpret_cast<const void*>(&bk_panel_kernel<T, kPanelThreadsWide>)
+                       : reinterpret_cast<const void*>(&bk_panel_kernel<T, kPanelThreads>);
+  return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(nt), kargs, 0, stream);
 }
 
 template <typename T>
-hipError_t panel_max_groups(int* groups) {
+hipError_t panel_max_groups(int nt, int* groups) {
   int per_cu = 0, dev = 0, cus = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, reinterpret_cast<const void*>(&bk_panel_kernel<T>), kPanelThreads, 0);
+  const void* fn = nt == kPanelThreadsWide
+                       ? reinterpret_cast<const void*>(&bk_panel_kernel<T, kPanelThreadsWide>)
+                       : reinterpret_cast<const void*>(&bk_panel_kernel<T, kPanelThreads>);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, 0);
   if (e != hipSuccess) return e;
   e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -925,10 +930,10 @@ size_t panel_exchange_records(int groups, int elem_bytes) {
   return 2 * static_cast<size_t>(groups + kNumPub);
 }
 
-template hipError_t launch_panel<double>(double*, const PanelArgs&, int, hipStream_t);
-template hipError_t launch_panel<float>(float*, const PanelArgs&, int, hipStream_t);
-template hipError_t panel_max_groups<double>(int*);
-template hipError_t panel_max_groups<float>(int*);
+template hipError_t launch_panel<double>(double*, const PanelArgs&, int, int, hipStream_t);
+template hipError_t launch_panel<float>(float*, const PanelArgs&, int, int, hipStream_t);
+template hipError_t panel_max_groups<double>(int, int*);
+template hipError_t panel_max_groups<float>(int, int*);
 
 }  // namespace bbk
 

@@ -73,6 +73,14 @@ const char* bbk_last_error(bbk_handle_t handle);
  * beyond any real wait); a negative value makes every wait fail at once
  * (for testing the error path). */
 int bbk_set_wait_limit(bbk_handle_t handle, int ms);
+/* Panel kernel variant of the handle's factorizations (no counterpart in
+ * the reference, whose FactorPanel is serial): 0 (default) picks by order,
+ * workgroups of 256 columns up to n = 65536 and of 512 columns up to
+ * n = 131072 (binary64 panels then at most 64 wide); 256 or 512 forces that
+ * variant (orders its grid cannot hold fail with BBK_ERR_UNSUPPORTED).
+ * Both variants make the same pivot decisions and, at the same panel
+ * width, bitwise the same factors. */
+int bbk_set_panel_threads(bbk_handle_t handle, int threads);
 /* After the last factorization queued on the handle (waits for the
  * stream): info = k + 1 for the first column k whose 1x1 pivot had
  * |d| <= DBL_EPSILON (its L column is set to zero, as the reference's
