@@ -9,8 +9,9 @@ C3  n = 16384: A = Q diag(lambda) Q^T, Q from the QR of a seeded Gaussian
     (condition 1e6); b ~ N(0, 1).
 C4  4096 matrices, n uniform in [16, 256], uniform [-1, 1) entries.
 C5  1024 matrices, n uniform in [256, 2048], condition 10^U[4, 7] per
-    matrix (exact eigenvalues, Householder eigen-construction), four
-    right-hand sides ~ N(0, 1).
+    matrix: A = Q diag(lambda) Q^T with Q from the QR of a seeded Gaussian
+    matrix (a dense random orthogonal matrix), |lambda| log-spaced over
+    [1/kappa, 1] with random signs; four right-hand sides ~ N(0, 1).
 
 Used by bench.py and the GPU tests, so both see the same matrices.
 """
@@ -43,9 +44,10 @@ def c3_matrix(torch, n: int, seed: int, lda: int, kappa: float = 1e6):
 
 
 def householder_eig(torch, n: int, kappa: float, gen, reflectors: int = 4):
-    """A = U diag(lambda) U^T with U a product of random Householder
-    reflectors (as LAPACK's dlagsy builds test matrices): exact eigenvalues
-    |lambda| log-spaced over [1/kappa, 1] with random signs."""
+    """A = U diag(lambda) U^T with U a product of a few random Householder
+    reflectors: exact eigenvalues |lambda| log-spaced over [1/kappa, 1] with
+    random signs, but A is diag(lambda) plus a rank-2 * reflectors term, a
+    cheap, strongly structured test matrix (small tests only)."""
     lam = torch.pow(torch.tensor(10.0, dtype=torch.float64, device="cuda"),
                     -math.log10(kappa) * torch.arange(n, dtype=torch.float64, device="cuda")
                     / max(n - 1, 1))
@@ -57,6 +59,22 @@ def householder_eig(torch, n: int, kappa: float, gen, reflectors: int = 4):
         Av = A @ v
         A = A - 2 * torch.outer(v, Av) - 2 * torch.outer(Av, v) + \
             4 * torch.dot(v, Av) * torch.outer(v, v)
+    return 0.5 * (A + A.T)
+
+
+def orthogonal_eig(torch, n: int, kappa: float, gen):
+    """A = Q diag(lambda) Q^T with Q from the QR of a seeded N(0, 1) matrix
+    (a dense random orthogonal matrix, binary64 on the GPU): eigenvalues
+    |lambda| log-spaced over [1/kappa, 1] with random signs, so the
+    condition number is kappa up to Q's orthogonality error."""
+    G = torch.randn((n, n), generator=gen, dtype=torch.float64, device="cuda")
+    Q, _ = torch.linalg.qr(G)
+    del G
+    lam = torch.pow(torch.tensor(10.0, dtype=torch.float64, device="cuda"),
+                    -math.log10(kappa) * torch.arange(n, dtype=torch.float64, device="cuda")
+                    / max(n - 1, 1))
+    sgn = torch.randint(0, 2, (n,), generator=gen, device="cuda").to(torch.float64) * 2 - 1
+    A = (Q * (lam * sgn)) @ Q.T.contiguous()
     return 0.5 * (A + A.T)
 
 
@@ -84,7 +102,7 @@ def c5_batch(torch, padded_ld, seed: int = 1024, count: int = 1024, nrhs: int = 
     mats = []
     for n, k in zip(ns, kap):
         T = torch.empty((n, padded_ld(n)), dtype=torch.float64, device="cuda")
-        T[:, :n] = householder_eig(torch, n, k, gen)
+        T[:, :n] = orthogonal_eig(torch, n, k, gen)
         mats.append(T)
     Bs = [torch.randn((n, nrhs), generator=gen, dtype=torch.float64, device="cuda") for n in ns]
     return ns, mats, Bs
