@@ -129,30 +129,37 @@ def bench_batched_mixed(torch, B, h, reps=2, seed=1024):
     return out
 
 
-def cpu_baseline(n_sample=4096, bs=128):
+def cpu_baseline(sizes=(4096, 8192), bs=128):
     """The reference's Factorize() (oracle/_ref, OpenMP, all host cores) on a
-    bounded sample; falls back to the oracle restatement if the reference
-    checker was not built."""
-    import numpy as np
+    bounded sample: two orders, so the rate's trend with n shows (the
+    headline n = 16384 would take minutes); `value` is the larger order's
+    rate.  Falls back to the oracle restatement if the reference checker
+    was not built."""
     import oracle as O
-    A = O.gen_synth_symmetric(n_sample, 424242, 1.0, 7)
     cores = os.cpu_count() or 1
-    try:
-        if not O.ref_available():
-            raise FileNotFoundError
-        secs = O.ref_time_factorize(A, bs)
-        kind = "reference"
-        what = "reference BlockLdltDecomposition::Factorize() (kCompactOnly, OpenMP)"
-    except Exception:
-        t0 = time.perf_counter()
-        O.dsytrf(A, bs, omp=True)
-        secs = time.perf_counter() - t0
-        kind = "port"
-        what = "oracle restatement of Factorize() (OpenMP)"
-    gf = n_sample ** 3 / 3.0 / secs / 1e9
-    return {"value": round(gf, 3), "unit": "GFLOP/s", "cores": cores, "kind": kind,
-            "sample": f"{what}, n={n_sample}, bs={bs}, uniform[-1,1] entries, "
-                      f"{secs:.2f} s on {cores} host threads"}
+    rates, total = {}, 0.0
+    kind, what = "reference", "reference BlockLdltDecomposition::Factorize() (kCompactOnly, OpenMP)"
+    for n_sample in sizes:
+        A = O.gen_synth_symmetric(n_sample, 424242, 1.0, 7)
+        try:
+            if not O.ref_available():
+                raise FileNotFoundError
+            secs = O.ref_time_factorize(A, bs)
+        except Exception:
+            t0 = time.perf_counter()
+            O.dsytrf(A, bs, omp=True)
+            secs = time.perf_counter() - t0
+            kind, what = "port", "oracle restatement of Factorize() (OpenMP)"
+        rates[n_sample] = n_sample ** 3 / 3.0 / secs / 1e9
+        total += secs
+        del A
+    n_top = sizes[-1]
+    trend = ", ".join(f"n={n}: {r:.2f} GFLOP/s" for n, r in rates.items())
+    return {"value": round(rates[n_top], 3), "unit": "GFLOP/s", "cores": cores, "kind": kind,
+            "by_n": {str(n): round(r, 3) for n, r in rates.items()},
+            "sample": f"{what}, n in {list(sizes)} (value at n={n_top}), bs={bs}, uniform[-1,1] "
+                      f"entries, zero diagonal every 7th index; {trend}; {total:.1f} s on {cores} "
+                      f"host threads"}
 
 
 def main():
