@@ -137,6 +137,57 @@ __device__ __forceinline__ T panel_rows_dot(const T* l, const T* W, int n, int k
   return (a0 + a1) + (a2 + a3);
 }
 
+// The products of PB positions j0, j0 + NT, ..., j0 + (PB - 1) NT at once
+// (positions >= n give unused values): for each position the same
+// operations in the same order as panel_rows_dot, so the results are
+// bitwise those of PB separate calls, with PB times the loads of W in
+// flight (W in global memory is latency-bound one position at a time).
+template <typename T, bool ACC4, int PB, int NT>
+__device__ __forceinline__ void panel_rows_dot_multi(const T* l, const T* W, int n, int kk,
+                                                     int j0, T (&out)[PB]) {
+  int jj[PB];
+#pragma unroll
+  for (int u = 0; u < PB; ++u) jj[u] = (j0 + u * NT < n) ? j0 + u * NT : j0;  // j0 < n
+  if constexpr (!ACC4) {
+    T acc[PB];
+#pragma unroll
+    for (int u = 0; u < PB; ++u) acc[u] = T(0);
+    for (int p = 0; p < kk; ++p) {
+      const T lp = l[p];
+      const T* row = W + (int64_t)p * n;
+#pragma unroll
+      for (int u = 0; u < PB; ++u) acc[u] += lp * row[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < PB; ++u) out[u] = acc[u];
+    return;
+  }
+  T a0[PB], a1[PB], a2[PB], a3[PB];
+#pragma unroll
+  for (int u = 0; u < PB; ++u) a0[u] = a1[u] = a2[u] = a3[u] = T(0);
+  int p = 0;
+  for (; p + 4 <= kk; p += 4) {
+    const T* r0 = W + (int64_t)p * n;
+    const T* r1 = r0 + n;
+    const T* r2 = r1 + n;
+    const T* r3 = r2 + n;
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      a0[u] += l[p] * r0[jj[u]];
+      a1[u] += l[p + 1] * r1[jj[u]];
+      a2[u] += l[p + 2] * r2[jj[u]];
+      a3[u] += l[p + 3] * r3[jj[u]];
+    }
+  }
+  for (; p < kk; ++p) {
+    const T* row = W + (int64_t)p * n;
+#pragma unroll
+    for (int u = 0; u < PB; ++u) a0[u] += l[p] * row[jj[u]];
+  }
+#pragma unroll
+  for (int u = 0; u < PB; ++u) out[u] = (a0[u] + a1[u]) + (a2[u] + a3[u]);
+}
+
 // Storage views of one matrix for the batched factorization; element
 // (i, j) with i <= j (upper triangle).
 //   GlobalMat  row-major in global memory (leading dimension lda).
@@ -197,8 +248,10 @@ struct HybridMat {
 // steps with the panel rows W (row stride n; rows 0..bw = panel rows, row
 // bw + 1 = the candidate row C).  F: the matrix's storage view (GlobalMat,
 // LdsMat or HybridMat); perm/piv in global memory or LDS.  Returns the
-// panel's end e (the nominal end, or the column of a deferred 2x2).
-template <typename T, int NT, bool ACC4, typename Mat>
+// panel's end e (the nominal end, or the column of a deferred 2x2).  PB:
+// positions per thread whose row products are formed together
+// (panel_rows_dot_multi; 1 = one at a time).
+template <typename T, int NT, bool ACC4, int PB = 1, typename Mat>
 __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
                             int* __restrict__ piv, int bw, int P, int s, T* W, T* Lk, T* Lr,
                             ScaleTab& st, double* rv, int* ri) {
@@ -214,13 +267,23 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
     double best = 0.0;
     int arg = n;
     T* Wk = W + (int64_t)kk * n;
-    for (int j = k + tid; j < n; j += NT) {
-      const T w = F.at(k, j) - panel_rows_dot<T, ACC4>(Lk, W, n, kk, j);
-      Wk[j] = w;
-      const double a = fabs(static_cast<double>(w));
-      if (j > k && a > best) {
-        best = a;
-        arg = j;
+    for (int j0 = k + tid; j0 < n; j0 += PB * NT) {
+      T fk[PB], dot[PB];
+#pragma unroll
+      for (int u = 0; u < PB; ++u) fk[u] = (j0 + u * NT < n) ? F.at(k, j0 + u * NT) : T(0);
+      panel_rows_dot_multi<T, ACC4, PB, NT>(Lk, W, n, kk, j0, dot);
+#pragma unroll
+      for (int u = 0; u < PB; ++u) {
+        const int j = j0 + u * NT;
+        if (j < n) {
+          const T w = fk[u] - dot[u];
+          Wk[j] = w;
+          const double a = fabs(static_cast<double>(w));
+          if (j > k && a > best) {
+            best = a;
+            arg = j;
+          }
+        }
       }
     }
     team_argmax<NT>(best, arg, rv, ri);  // (:164-182) first maximum
@@ -235,12 +298,24 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
       if (tid < kk) Lr[tid] = ucol<T>(st, W, n, tid, r);  // L(r, s + tid)
       __syncthreads();
       double sig = 0.0;
-      for (int i = k + tid; i < n; i += NT) {
-        const T base = i < r ? F.at(i, r) : F.at(r, i);
-        const T cv = base - panel_rows_dot<T, ACC4>(Lr, W, n, kk, i);
-        C[i] = cv;
-        const double a = fabs(static_cast<double>(cv));
-        if (i != r) sig = (sig < a) ? a : sig;
+      for (int i0 = k + tid; i0 < n; i0 += PB * NT) {
+        T base[PB], dot[PB];
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+          const int i = i0 + u * NT;
+          base[u] = i < n ? (i < r ? F.at(i, r) : F.at(r, i)) : T(0);
+        }
+        panel_rows_dot_multi<T, ACC4, PB, NT>(Lr, W, n, kk, i0, dot);
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+          const int i = i0 + u * NT;
+          if (i < n) {
+            const T cv = base[u] - dot[u];
+            C[i] = cv;
+            const double a = fabs(static_cast<double>(cv));
+            if (i != r) sig = (sig < a) ? a : sig;
+          }
+        }
       }
       const double sigma = team_max<NT>(sig, rv);
       __syncthreads();
@@ -929,6 +1004,14 @@ struct StepArgs {
   BatchWidths widths;
 };
 
+// Positions per thread whose row products a lockstep panel forms together
+// (W in a global slice: more loads in flight per thread).  Measured on the
+// 1024-matrix batch (n in [256, 2048]) against one position at a time:
+// binary32 113.7 -> 106.7 ms with 8 (2: 108.4, 4: 143.5), binary64
+// 149.2 -> 144.0 ms with 4 (8: 210.2).
+template <typename T>
+constexpr int kStepPanelBatch = sizeof(T) == 4 ? 8 : 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   __shared__ T Lk[kBatchMaxWidth + 2];
@@ -960,7 +1043,8 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   }
   const int s = P == 0 ? 0 : S[0];
   T* W = a.ws + a.wsoff[m];
-  const int e = factor_panel<T, 256, true>(GlobalMat<T>{a.A[m], a.lda[m]}, n, perm, piv, bw, P, s,
+  const int e = factor_panel<T, 256, true, kStepPanelBatch<T>>(GlobalMat<T>{a.A[m], a.lda[m]}, n,
+                                                           perm, piv, bw, P, s,
                                            W, Lk, Lr, stab,
                                      rv, ri);
   if (threadIdx.x == 0) {
