@@ -250,13 +250,18 @@ struct HybridMat {
 // LdsMat or HybridMat); perm/piv in global memory or LDS.  Returns the
 // panel's end e (the nominal end, or the column of a deferred 2x2).  PB:
 // positions per thread whose row products are formed together
-// (panel_rows_dot_multi; 1 = one at a time).
-template <typename T, int NT, bool ACC4, int PB = 1, typename Mat>
+// (panel_rows_dot_multi; 1 = one at a time).  LROW: the current pivot's W
+// row and the candidate row live in LDS (rowk, rowc: n elements each,
+// indexed by column) instead of W's rows kk and bw + 1, and each finished
+// row is copied to W after its scaling; for W in global memory this takes
+// the row reads between the pivot's phases off global memory.
+template <typename T, int NT, bool ACC4, int PB = 1, bool LROW = false, typename Mat>
 __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
                             int* __restrict__ piv, int bw, int P, int s, T* W, T* Lk, T* Lr,
-                            ScaleTab& st, double* rv, int* ri) {
+                            ScaleTab& st, double* rv, int* ri, T* rowk = nullptr,
+                            T* rowc = nullptr) {
   const int tid = threadIdx.x;
-  T* C = W + (int64_t)(bw + 1) * n;
+  T* C = LROW ? rowc : W + (int64_t)(bw + 1) * n;
   int e = min((P + 1) * bw, n);  // nominal end; s may sit one column early
   int kk = 0;
   while (s + kk < e) {
@@ -266,7 +271,7 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
     // W(k, j) = F(k, j) - sum_p L(k, p) W(p, j), j >= k  (:122-162)
     double best = 0.0;
     int arg = n;
-    T* Wk = W + (int64_t)kk * n;
+    T* Wk = LROW ? rowk : W + (int64_t)kk * n;
     for (int j0 = k + tid; j0 < n; j0 += PB * NT) {
       T fk[PB], dot[PB];
 #pragma unroll
@@ -353,7 +358,8 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
       } else if (kind == 2) {
         // interchange k+1 <-> r (:500-587); row k+1 of W becomes the candidate
         const int k1 = k + 1;
-        T* Wk1 = W + (int64_t)(kk + 1) * n;
+        // row k+1 of W: with LROW the candidate row itself (C)
+        T* Wk1 = LROW ? C : W + (int64_t)(kk + 1) * n;
         if (r != k1) {
           for (int j = r + 1 + tid; j < n; j += NT) swap_elems(&F.at(k1, j), &F.at(r, j));
           for (int j = k + 2 + tid; j < r; j += NT) swap_elems(&F.at(k1, j), &F.at(j, r));
@@ -363,19 +369,25 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
           for (int p = tid; p < kk; p += NT)
             swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
         }
-        for (int j = k + 2 + tid; j < n; j += NT)
-          if (j != r) Wk1[j] = C[j];
+        if constexpr (!LROW) {
+          for (int j = k + 2 + tid; j < n; j += NT)
+            if (j != r) Wk1[j] = C[j];
+        }
         if (tid == 0) {
           if (r != k1) {
             swap_elems(&F.at(k1, k1), &F.at(r, r));
             swap_elems(&F.at(k, k1), &F.at(k, r));
             swap_elems(&Wk[k1], &Wk[r]);
-            Wk1[r] = C[k1];
+            // W(k+1, r) = C(k+1), W(k+1, k+1) = C(r): in place a swap
+            if constexpr (LROW)
+              swap_elems(&C[k1], &C[r]);
+            else
+              Wk1[r] = C[k1];
             const int t = perm[k1];
             perm[k1] = perm[r];
             perm[r] = t;
           }
-          Wk1[k1] = C[r];
+          if constexpr (!LROW) Wk1[k1] = C[r];
         }
       }
       __syncthreads();
@@ -392,11 +404,15 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
         st.ty[kk] = 1;
         st.c0[kk] = den;
       }
+      if constexpr (LROW) {
+        T* g = W + (int64_t)kk * n;
+        for (int j = k + tid; j < n; j += NT) g[j] = Wk[j];
+      }
       kk += 1;
     } else {
       // ScaleTwoByTwoPivot (:658-704), binary64 without contraction
       const int k1 = k + 1;
-      const T* Wk1 = W + (int64_t)(kk + 1) * n;
+      const T* Wk1 = LROW ? C : W + (int64_t)(kk + 1) * n;
       const double a = static_cast<double>(Wk[k]);
       const double b = static_cast<double>(Wk[k1]);
       const double c = static_cast<double>(Wk1[k1]);
@@ -421,6 +437,14 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
         st.ty[kk + 1] = -2;
         st.c0[kk + 1] = ioff;
         st.c1[kk + 1] = ibr;
+      }
+      if constexpr (LROW) {
+        T* g0 = W + (int64_t)kk * n;
+        T* g1 = g0 + n;
+        for (int j = k + tid; j < n; j += NT) {
+          g0[j] = Wk[j];
+          g1[j] = Wk1[j];
+        }
       }
       kk += 2;
     }
@@ -1004,6 +1028,10 @@ struct StepArgs {
   BatchWidths widths;
 };
 
+// Dynamic LDS limit of a lockstep panel's pivot and candidate rows (2 n
+// elements); larger orders keep both rows in the global W slice.
+constexpr size_t kStepRowLdsBytes = 60 * 1024;
+
 // Positions per thread whose row products a lockstep panel forms together
 // (W in a global slice: more loads in flight per thread).  Measured on the
 // 1024-matrix batch (n in [256, 2048]) against one position at a time:
@@ -1012,7 +1040,7 @@ struct StepArgs {
 template <typename T>
 constexpr int kStepPanelBatch = sizeof(T) == 4 ? 8 : 4;
 
-template <typename T>
+template <typename T, bool LROW>
 __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   __shared__ T Lk[kBatchMaxWidth + 2];
   __shared__ T Lr[kBatchMaxWidth + 2];
@@ -1043,10 +1071,12 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   }
   const int s = P == 0 ? 0 : S[0];
   T* W = a.ws + a.wsoff[m];
-  const int e = factor_panel<T, 256, true, kStepPanelBatch<T>>(GlobalMat<T>{a.A[m], a.lda[m]}, n,
-                                                           perm, piv, bw, P, s,
-                                           W, Lk, Lr, stab,
-                                     rv, ri);
+  // LROW: the pivot row and the candidate row in dynamic LDS (n each)
+  T* rowk = reinterpret_cast<T*>(bbk_batched_lds);
+  T* rowc = rowk + n;
+  const int e = factor_panel<T, 256, true, kStepPanelBatch<T>, LROW>(
+      GlobalMat<T>{a.A[m], a.lda[m]}, n, perm, piv, bw, P, s, W, Lk, Lr, stab, rv, ri, rowk,
+      rowc);
   if (threadIdx.x == 0) {
     S[0] = e;
     S[1] = s;
@@ -1327,10 +1357,19 @@ static int lockstep_launch(Handle* h, const BatchPlan& p, T* const* A, const int
   a.st = base + p.count;
   a.widths = p.widths;
   const int* prefix_all = base + 5 * (size_t)p.count;
+  // the pivot and candidate rows in LDS when both fit kStepRowLdsBytes
+  int nmax = 0;
+  for (int v : p.sizes) nmax = std::max(nmax, v);
+  const size_t row_lds = 2 * static_cast<size_t>(nmax) * sizeof(T);
+  const bool lrow = row_lds <= kStepRowLdsBytes;
   for (int P = 0; P < p.steps; ++P) {
     a.step = P;
     a.prefix = prefix_all + (size_t)P * p.step_stride;
-    hipLaunchKernelGGL(k_step_panel<T>, dim3(p.count), dim3(256), 0, h->stream, a);
+    if (lrow)
+      hipLaunchKernelGGL((k_step_panel<T, true>), dim3(p.count), dim3(256), row_lds, h->stream,
+                         a);
+    else
+      hipLaunchKernelGGL((k_step_panel<T, false>), dim3(p.count), dim3(256), 0, h->stream, a);
     const int tiles = p.step_tiles[P];
     if (tiles > 0)
       hipLaunchKernelGGL(k_step_update<T>, dim3(tiles), dim3(256), 0, h->stream, a);
