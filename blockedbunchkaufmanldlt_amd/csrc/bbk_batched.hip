@@ -210,6 +210,38 @@ struct GlobalMat {
   __device__ __forceinline__ int first_immediate_row(int) const { return 0; }
   __device__ __forceinline__ void defer(int, int) const {}
 };
+// LDS record of one panel's deferred interchanges (at most one per column
+// of a panel of up to kBatchMaxWidth + 1 columns) and the scratch of their
+// composition.
+struct DeferScratch {
+  static constexpr int kMax = kBatchMaxWidth + 2;
+  int list[2 * kMax];
+  int slot[2 * kMax];
+  int content[2 * kMax];
+  int src[2 * kMax];
+  int count;
+};
+
+// GlobalMat whose interchanges reach only the rows at or after the panel
+// start at once; those of the finished rows above it are recorded (defer,
+// into a DeferScratch list) and applied after the panel as one composed
+// gather per row (apply_panel_deferred), which touches each row once
+// instead of once per interchange.
+template <typename T>
+struct DeferGlobalMat {
+  T* g;
+  int64_t lda;
+  DeferScratch* ds;
+  __device__ __forceinline__ T& at(int i, int j) const { return g[(int64_t)i * lda + j]; }
+  __device__ __forceinline__ int first_immediate_row(int s) const { return s; }
+  // called by one thread
+  __device__ __forceinline__ void defer(int a, int b) const {
+    const int c = ds->count;
+    ds->list[2 * c] = a;
+    ds->list[2 * c + 1] = b;
+    ds->count = c + 1;
+  }
+};
 template <typename T>
 struct LdsMat {
   T* s;
@@ -1028,6 +1060,62 @@ struct StepArgs {
   BatchWidths widths;
 };
 
+// Rows [0, R) of A: row[dst[t]] = row[src[t]] for t < ne (one composed
+// interchange gather, ne <= 192).  Each wavefront takes whole rows, lane l
+// moving entries l, l + 64 and l + 128, four rows per pass; every load of
+// a pass completes (wait_vm) before its stores, so a row's reads precede
+// its writes; different rows are independent.
+template <typename T>
+__device__ void gather_rows(T* __restrict__ A, int64_t lda, int R, const int* dst,
+                            const int* src, int ne) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int d[3], sc[3];
+  bool h[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int t = lane + 64 * e;
+    h[e] = t < ne;
+    d[e] = h[e] ? dst[t] : 0;
+    sc[e] = h[e] ? src[t] : 0;
+  }
+  for (int i0 = w; i0 < R; i0 += 4 * nw) {
+    T v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int row = i0 + u * nw;
+      const T* Rr = A + (int64_t)(row < R ? row : 0) * lda;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) v[u][e] = (row < R && h[e]) ? Rr[sc[e]] : T(0);
+    }
+    wait_vm();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int row = i0 + u * nw;
+      if (row < R) {
+        T* Rr = A + (int64_t)row * lda;
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+          if (h[e]) Rr[d[e]] = v[u][e];
+      }
+    }
+  }
+}
+
+// A panel's deferred interchanges (ds.list, ds.count pairs in order)
+// applied to the finished rows [0, R) above it: composed into one gather
+// (compose_swaps), then gather_rows.  Called by every thread of the team.
+template <typename T, int NT>
+__device__ void apply_panel_deferred(T* A, int64_t lda, int R, DeferScratch& ds) {
+  static_assert(2 * DeferScratch::kMax <= 192, "gather_rows moves at most 192 entries");
+  const int nd = ds.count;
+  if (nd == 0 || R == 0) return;
+  compose_swaps(ds.list, nd, ds.slot, ds.content);
+  const int ne = 2 * nd;
+  for (int t = threadIdx.x; t < ne; t += NT) ds.src[t] = ds.content[ds.slot[t]];
+  __syncthreads();
+  gather_rows<T>(A, lda, R, ds.list, ds.src, ne);
+}
+
 // Dynamic LDS limit of a lockstep panel's pivot and candidate rows (2 n
 // elements); larger orders keep both rows in the global W slice.
 constexpr size_t kStepRowLdsBytes = 60 * 1024;
@@ -1039,6 +1127,37 @@ constexpr size_t kStepRowLdsBytes = 60 * 1024;
 // 149.2 -> 144.0 ms with 4 (8: 210.2).
 template <typename T>
 constexpr int kStepPanelBatch = sizeof(T) == 4 ? 8 : 4;
+
+// Interchanges of the rows above a lockstep panel deferred to one gather
+// per row after the panel (binary32: 100.3 -> 88.7 ms on that batch;
+// binary64, 16 elements per cache line instead of 32: 137.5 -> 139.4 ms,
+// so it keeps the immediate column swaps).  (The same deferral in the
+// persistent small-matrix kernel measured slower: 2.63 -> 3.28 ms.)
+template <typename T>
+constexpr bool kStepDefer = sizeof(T) == 4;
+
+// One lockstep panel of matrix m: binary32 with deferred interchanges of
+// the rows above the panel (applied before returning), binary64 with
+// immediate ones.
+template <typename T, bool LROW>
+__device__ int step_panel(T* A, int64_t lda, int n, int* perm, int* piv, int bw, int P, int s,
+                          T* W, T* Lk, T* Lr, ScaleTab& st, double* rv, int* ri, T* rowk,
+                          T* rowc) {
+  if constexpr (kStepDefer<T>) {
+    __shared__ DeferScratch ds;
+    if (threadIdx.x == 0) ds.count = 0;
+    __syncthreads();
+    const int e = factor_panel<T, 256, true, kStepPanelBatch<T>, LROW>(
+        DeferGlobalMat<T>{A, lda, &ds}, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri, rowk,
+        rowc);
+    __syncthreads();
+    apply_panel_deferred<T, 256>(A, lda, s, ds);
+    return e;
+  } else {
+    return factor_panel<T, 256, true, kStepPanelBatch<T>, LROW>(
+        GlobalMat<T>{A, lda}, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri, rowk, rowc);
+  }
+}
 
 template <typename T, bool LROW>
 __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
@@ -1074,9 +1193,8 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   // LROW: the pivot row and the candidate row in dynamic LDS (n each)
   T* rowk = reinterpret_cast<T*>(bbk_batched_lds);
   T* rowc = rowk + n;
-  const int e = factor_panel<T, 256, true, kStepPanelBatch<T>, LROW>(
-      GlobalMat<T>{a.A[m], a.lda[m]}, n, perm, piv, bw, P, s, W, Lk, Lr, stab, rv, ri, rowk,
-      rowc);
+  const int e = step_panel<T, LROW>(a.A[m], a.lda[m], n, perm, piv, bw, P, s, W, Lk, Lr, stab,
+                                    rv, ri, rowk, rowc);
   if (threadIdx.x == 0) {
     S[0] = e;
     S[1] = s;
