@@ -83,4 +83,24 @@ inline size_t factor_scratch_bytes(int n, FactorScratch* fs, char* base) {
 
 void launch_zero_u32(unsigned* p, int count, hipStream_t stream);
 
+// Launch of a kernel whose workgroups wait on each other's records (panel
+// exchanges, triangular sweeps).  Its grid never exceeds the co-resident
+// capacity (occupancy x CUs).  The cooperative launch guarantees that every
+// workgroup is resident.  The runtime refuses a cooperative launch in a
+// second process once another process on the device holds the wave-sync
+// resource; with bounded = true such a launch falls back to an ordinary one
+// (on a device not busy with other work its workgroups are all resident).
+// That is safe only because every wait of these kernels is bounded: if a
+// workgroup were not scheduled, the waits end at the time limit and the
+// call reports BBK_ERR_HIP from the status word instead of hanging.
+// Callers without a status word pass bounded = false and keep the
+// cooperative launch only.
+inline hipError_t launch_resident(const void* fn, dim3 grid, dim3 block, void** args,
+                                  hipStream_t stream, bool bounded) {
+  hipError_t e = hipLaunchCooperativeKernel(fn, grid, block, args, 0, stream);
+  if (e == hipSuccess || !bounded) return e;
+  (void)hipGetLastError();  // the refused launch's error is not sticky
+  return hipLaunchKernel(fn, grid, block, args, 0, stream);
+}
+
 }  // namespace bbk

@@ -906,7 +906,7 @@ hipError_t launch_panel(T* A, const PanelArgs& args, int groups, int nt, hipStre
   const void* fn = nt == kPanelThreadsWide
                        ? reinterpret_cast<const void*>(&bk_panel_kernel<T, kPanelThreadsWide>)
                        : reinterpret_cast<const void*>(&bk_panel_kernel<T, kPanelThreads>);
-  return hipLaunchCooperativeKernel(fn, dim3(groups), dim3(nt), kargs, 0, stream);
+  return launch_resident(fn, dim3(groups), dim3(nt), kargs, stream, args.status != nullptr);
 }
 
 template <typename T>

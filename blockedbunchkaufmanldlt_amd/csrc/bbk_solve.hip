@@ -1265,8 +1265,8 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     long long pl_ = h->wait_ticks();
     void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &B_,   &ldb_, &Z_,
                     &ldz, &r_,   &tag,  &nblk_, &Li_,   &stop_, &st_, &pl_};
-    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR, INV>),
-                                              dim3(gf), dim3(kSolveThreads), args, 0, h->stream);
+    hipError_t e = launch_resident(reinterpret_cast<const void*>(&k_trsv_fwd<T, NR, INV>),
+                                   dim3(gf), dim3(kSolveThreads), args, h->stream, true);
     if (e != hipSuccess) return h->hip_fail(e, "forward solve launch");
   }
   {
@@ -1284,8 +1284,8 @@ static int solve_launch(Handle* h, int n, int nblk, const T* F, int64_t lda, con
     long long pl_ = h->wait_ticks();
     void* args[] = {&n_,  &F_,   &lda_, &piv_,  &perm_, &Z_,   &ldz,  &X_,
                     &ldx_, &r_,  &tag,  &nblk_, &Li_,   &stop_, &st_, &pl_};
-    hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR, INV>),
-                                              dim3(gb), dim3(kSolveThreads), args, 0, h->stream);
+    hipError_t e = launch_resident(reinterpret_cast<const void*>(&k_trsv_bwd<T, NR, INV>),
+                                   dim3(gb), dim3(kSolveThreads), args, h->stream, true);
     if (e != hipSuccess) return h->hip_fail(e, "backward solve launch");
   }
   return BBK_OK;

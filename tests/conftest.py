@@ -26,10 +26,12 @@ def pytest_collection_modifyitems(config, items):
 
     The panel and triangular-solve kernels are cooperative launches, which
     use the device's global wave sync resource; only one process at a time
-    can hold it.  Once this pytest process has made a cooperative launch,
-    a child process's cooperative launch is refused, so the drop-in tests
-    (tests/test_gpu_dropin.py, which run compiled programs) go before every
-    test that launches in-process."""
+    can hold it.  Once this pytest process has made a cooperative launch, a
+    child process's cooperative launch is refused and the library falls
+    back to an ordinary launch (tests/test_gpu_second_process.py covers
+    that path), so the drop-in tests (tests/test_gpu_dropin.py, which run
+    compiled programs) go before every test that launches in-process to
+    exercise the cooperative path there too."""
     first = [it for it in items if it.module.__name__.endswith("test_gpu_dropin")]
     rest = [it for it in items if not it.module.__name__.endswith("test_gpu_dropin")]
     items[:] = first + rest
