@@ -420,18 +420,34 @@ __global__ __launch_bounds__(kRT) void k_batched_refine(RefineArgs a) {
   }
 }
 
-// Upper triangle binary64 -> binary32, one workgroup per matrix.
-__global__ void k_batched_cast(int count, const int* __restrict__ ns,
-                               const double* const* A, const int64_t* __restrict__ lda,
-                               float* const* S, const int64_t* __restrict__ lds) {
-  for (int m = blockIdx.x; m < count; m += gridDim.x) {
-    const int n = ns[m];
-    const double* Am = A[m];
-    float* Sm = S[m];
-    const int64_t la = lda[m], ls = lds[m];
-    for (int i = 0; i < n; ++i)
-      for (int j = i + threadIdx.x; j < n; j += blockDim.x)
-        Sm[(int64_t)i * ls + j] = (float)Am[(int64_t)i * la + j];
+// Upper triangle binary64 -> binary32.  Workgroup g of the grid takes
+// matrix g % count and every (gridDim.x / count)-th group of 4 rows of it
+// (a grid of a multiple of count), one wavefront per row, four 64-column
+// chunks of loads in flight per lane.
+__global__ __launch_bounds__(256) void k_batched_cast(int count, const int* __restrict__ ns,
+                                                      const double* const* A,
+                                                      const int64_t* __restrict__ lda,
+                                                      float* const* S,
+                                                      const int64_t* __restrict__ lds) {
+  const int per = gridDim.x / count;  // workgroups per matrix
+  const int m = blockIdx.x % count, part = blockIdx.x / count;
+  const int n = ns[m];
+  const double* Am = A[m];
+  float* Sm = S[m];
+  const int64_t la = lda[m], ls = lds[m];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = 4 * part + w; i < n; i += 4 * per) {
+    const double* Ar = Am + (int64_t)i * la;
+    float* Sr = Sm + (int64_t)i * ls;
+    int j = i + lane;
+    for (; j + 192 < n; j += 256) {
+      const double a0 = Ar[j], a1 = Ar[j + 64], a2 = Ar[j + 128], a3 = Ar[j + 192];
+      Sr[j] = (float)a0;
+      Sr[j + 64] = (float)a1;
+      Sr[j + 128] = (float)a2;
+      Sr[j + 192] = (float)a3;
+    }
+    for (; j < n; j += 64) Sr[j] = (float)Ar[j];
   }
 }
 
@@ -515,8 +531,10 @@ int batched_sysv_mixed(Handle* h, int count, const int* h_n, int nrhs,
   const int* dOrder = reinterpret_cast<const int*>(dptr(hOrder));
   const int* dN = reinterpret_cast<const int*>(dptr(hN));
   int* dQueue = reinterpret_cast<int*>(dptr(hQueue));
-  hipLaunchKernelGGL(k_batched_cast, dim3(std::min(count, 4096)), dim3(256), 0, h->stream, count,
-                     dN, A, lda, dF, dL);
+  // 8 workgroups per matrix (4 rows each per pass), at least 2048 in all
+  const int cast_per = count >= (1 << 27) ? 1 : std::max(8, 2048 / count);
+  hipLaunchKernelGGL(k_batched_cast, dim3(count * cast_per), dim3(256), 0, h->stream, count, dN,
+                     A, lda, dF, dL);
   rc = batched_sytrf<float>(h, count, h_n, dF, dL, dP, dV);
   if (rc != BBK_OK) return rc;
   RefineArgs ra;
