@@ -734,10 +734,9 @@ __device__ int factor_panel_lds(T* S, int n, int* perm, int* piv, int bw, int P,
     int kind = 1;
     double ck = 0.0, ck1 = 0.0, crr = 0.0;
     T c = T(0);
-    T wk_k1 = T(0), wk_r = T(0);
+    T wk_k1 = T(0);
     if (!(akk >= kAlpha * omega || omega == 0.0)) {
       // ---- B: candidate row r (:184-367) ---------------------------------
-      wk_r = Wk[r];
       if (k + 1 < n) wk_k1 = Wk[k + 1];
       double sv = 0.0;
       if (j >= k && j < n) {
@@ -809,9 +808,11 @@ __device__ int factor_panel_lds(T* S, int n, int* perm, int* piv, int bw, int P,
       const int k1 = k + 1;
       T* Wk1 = W + (kk + 1) * n;
       const bool sw = r != k1;
-      // W(k, k+1) after the interchange, and the D block
+      // W(k, k+1) after the interchange is the candidate's value at k (the
+      // candidate row overwrote W(k, r), which the interchange moves to
+      // k+1: :184-247, :510-514), and the D block
       const double a = wkk;
-      const double b = static_cast<double>(sw ? wk_r : wk_k1);
+      const double b = ck;
       const double cc = crr;
       double itl, ibr, ioff;
       inverse_2x2_nc(a, b, cc, &itl, &ibr, &ioff);
@@ -1039,6 +1040,539 @@ __global__ __launch_bounds__(256) void k_batched_lds(BatchArgs<T> a) {
     apply_deferred<T>(A, lda, R0, dlist, R0 > 0 ? dcount : 0, slot, slot + 2 * n);
     __syncthreads();
   }
+}
+
+// ---- symmetric-workspace mode (one wavefront per matrix, n <= 256) ---------
+// Each workgroup is one wavefront and owns a slice of the batch workspace
+// holding a full symmetric copy S of the matrix it factors (both triangles,
+// row stride n).  An interchange moves no data: the logical -> physical map
+// pi is updated instead, and logical element (i, j) is S(pi[i], pi[j]).
+//  * Every row the panel reads -- the pivot row and the candidate row, whose
+//    column part (A(i, r), i < r) is a strided walk in upper storage -- is one
+//    physical row of S, read with a gather that is contiguous where pi is.
+//  * The reference's column swaps of the finished rows
+//    (src/block_ldlt_decomposition.cc:414-587) cost nothing: a finished
+//    row's values stay at the physical columns they were written to, and
+//    its final column j is S(pi[k], pi[j]) with the final map.
+//  * The pivot loop stores nothing to global memory.  Rows k + 1 and k + 2
+//    are loaded one pivot ahead and, when the interchange touches them,
+//    patched in registers (S is symmetric, so the new entry of row x at
+//    column r is row k's raw value at column x); the candidate row is the
+//    one load a pivot waits for.  The scaled rows U and the D blocks are
+//    written once per panel, rebuilt from W and the per-row scaling record
+//    (bit for bit the values the scaling forms), and the trailing update
+//    (MFMA tiles over the logical upper triangle, each result stored at its
+//    place and its mirror) takes its U operands from the same rebuild.
+// Positions: lane l holds columns l + 64 u (u < kSymNP) of every row; pi of
+// those columns sits in registers (pj), and in LDS for the update.  The
+// arithmetic is factor_panel's (same row products, decisions, W updates and
+// scaling, src/block_ldlt_decomposition.cc:80-120,122-704) and
+// mfma_update's, element for element, so the factors equal the
+// global-memory team's with MFMA update tiles bit for bit.  Loads are issued
+// from clamped (always valid) addresses and their values masked after, so
+// no load sits in a branch of its own.
+constexpr int kSymNP = 4;         // positions per lane: orders up to 256
+constexpr int kSymMaxWidth = 32;  // widest panel of this mode (register budget)
+
+// The matrix pointers come from a device array, so the compiler cannot
+// tell they are global; this view makes their accesses global loads and
+// stores (flat ones would also wait on the LDS counter).
+template <typename T>
+using gmem_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ gmem_t<T>* to_global(T* p) {
+  return (gmem_t<T>*)p;
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane) {
+  if constexpr (sizeof(T) == 8) {
+    return readlane_f64(v, lane);
+  } else {
+    static_assert(sizeof(T) == 4, "readlane of a 4-byte value");
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  }
+}
+
+// The value at position j (0 <= j < 64 kSymNP, wave-uniform) of a per-lane
+// position array whose lane l, slot u holds position l + 64 u.  pos_get
+// reads another lane's register, so it is called where every lane is
+// active.  (Constant subscripts only, so the arrays stay in registers.)
+static_assert(kSymNP == 4, "pos_get / pos_set select among four slots");
+template <typename T>
+__device__ __forceinline__ T pos_get(const T (&v)[kSymNP], int j) {
+  const int u = j >> 6;
+  const T x = u == 0 ? v[0] : u == 1 ? v[1] : u == 2 ? v[2] : v[3];
+  return readlane_t(x, j & 63);
+}
+template <typename T>
+__device__ __forceinline__ void pos_set(T (&v)[kSymNP], int j, T val) {
+  const bool mine = (threadIdx.x & 63) == (j & 63);
+  const int u = j >> 6;
+  v[0] = (mine && u == 0) ? val : v[0];
+  v[1] = (mine && u == 1) ? val : v[1];
+  v[2] = (mine && u == 2) ? val : v[2];
+  v[3] = (mine && u == 3) ? val : v[3];
+}
+
+// Logical row x at every position (values at positions past n unused).
+template <typename T>
+__device__ __forceinline__ void sym_row(const gmem_t<T>* S, int n, const int (&pj)[kSymNP], int x,
+                                        T (&v)[kSymNP]) {
+  const gmem_t<T>* row = S + static_cast<unsigned>(pos_get(pj, x) * n);
+#pragma unroll
+  for (int u = 0; u < kSymNP; ++u) v[u] = row[static_cast<unsigned>(pj[u])];
+}
+
+// Row x, loaded before this pivot's interchange of ps and r, made current:
+// x == r: the row now lives where row ps lived, so it is row ps (src) with
+// its column r taken from src's diagonal; otherwise only column r changed,
+// to S(pi_old[ps], pi[x]), which by symmetry is src at column x.
+template <typename T>
+__device__ __forceinline__ void sym_patch(T (&rx)[kSymNP], int x, const T (&src)[kSymNP], int ps,
+                                          int r) {
+  if (x == r) {
+    const T d = pos_get(src, ps);
+#pragma unroll
+    for (int u = 0; u < kSymNP; ++u) rx[u] = src[u];
+    pos_set(rx, r, d);
+  } else if (r > x) {
+    pos_set(rx, r, pos_get(src, x));
+  }
+}
+
+// Panel P of one matrix held in S (see above): the pivot loop; returns the
+// panel's end e.  pj: pi of the lane's positions (updated by the
+// interchanges); spiv: the pivot types (LDS).
+template <typename T, bool ACC4>
+__device__ int sym_panel(const gmem_t<T>* S, int n, int (&pj)[kSymNP], int* spiv, int bw, int P,
+                         int s, T* W, T* Lk, T* Lr, ScaleTab& st) {
+  const int lane = threadIdx.x & 63;
+  const int j0 = min(lane, n - 1);
+  int e = min((P + 1) * bw, n);
+  T rk0[kSymNP], rk1[kSymNP], rk2[kSymNP];
+  sym_row<T>(S, n, pj, s, rk0);
+#pragma unroll
+  for (int u = 0; u < kSymNP; ++u) rk1[u] = rk2[u] = T(0);
+  // nothing in flight when the pivot loop starts, so the compiler's waits
+  // inside it count only the loop's own loads (vmcnt(0); gfx9 encoding)
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  bool have1 = false;
+  int kk = 0;
+  while (s + kk < e) {
+    const int k = s + kk;
+    if (!have1 && k + 1 < n) sym_row<T>(S, n, pj, k + 1, rk1);
+    if (k + 2 < n) sym_row<T>(S, n, pj, k + 2, rk2);
+    if (lane < kk) Lk[lane] = ucol<T>(st, W, n, lane, k);  // L(k, s + lane)
+    __syncthreads();
+    // W(k, j) = A(k, j) - sum_p L(k, p) W(p, j), j >= k  (:122-162)
+    T w[kSymNP], dot[kSymNP];
+    panel_rows_dot_multi<T, ACC4, kSymNP, 64>(Lk, W, n, kk, j0, dot);
+    T* Wk = W + (int64_t)kk * n;
+    double best = 0.0;
+    int arg = n;
+#pragma unroll
+    for (int u = 0; u < kSymNP; ++u) {
+      const int j = lane + 64 * u;
+      w[u] = rk0[u] - dot[u];
+      if (j >= k && j < n) {
+        Wk[j] = w[u];
+        const double a = fabs(static_cast<double>(w[u]));
+        if (j > k && a > best) {
+          best = a;
+          arg = j;
+        }
+      }
+    }
+    wave_argmax(best, arg);  // (:164-182) first maximum
+    const double omega = best;
+    const int r = arg;
+    const double wkk = static_cast<double>(pos_get(w, k));
+    const double akk = fabs(wkk);
+    int kind = 1;
+    T c[kSymNP];
+    double ck = 0.0, ck1 = 0.0, crr = 0.0;
+    if (!(akk >= kAlpha * omega || omega == 0.0)) {
+      // candidate row r: C(i) = A(r, i) - sum_p L(r, p) W(p, i), i >= k
+      // (:184-352), sigma over i != r (:354-367); A(r, i) = S(pi[r], pi[i])
+      T cr[kSymNP];
+      sym_row<T>(S, n, pj, r, cr);
+      if (lane < kk) Lr[lane] = ucol<T>(st, W, n, lane, r);  // L(r, s + lane)
+      __syncthreads();
+      panel_rows_dot_multi<T, ACC4, kSymNP, 64>(Lr, W, n, kk, j0, dot);
+      double sig = 0.0;
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) {
+        const int i = lane + 64 * u;
+        c[u] = cr[u] - dot[u];
+        const double a = fabs(static_cast<double>(c[u]));
+        if (i >= k && i < n && i != r) sig = (sig < a) ? a : sig;
+      }
+      const double sigma = wave_max_nan_ignoring(sig);
+      crr = static_cast<double>(pos_get(c, r));
+      ck = static_cast<double>(pos_get(c, k));
+      ck1 = k + 1 < n ? static_cast<double>(pos_get(c, k + 1)) : 0.0;
+      kind = select_pivot_nc(akk, omega, sigma, fabs(crr));
+      if (kind == 2 && kk == e - s - 1 && e < n) {
+        // 2x2 across the panel edge: this panel ends before k (:589-631)
+        e = k;
+        __syncthreads();
+        break;
+      }
+      if (kind == -1) {
+        // interchange k <-> r (:414-498): the finished rows' W columns, pi;
+        // row k of W becomes the candidate row with W(k, r) = C(k)
+        // (the candidate's value at k, :184-247) and W(k, k) = C(r)
+        if (lane < kk) swap_elems(&W[(int64_t)lane * n + k], &W[(int64_t)lane * n + r]);
+#pragma unroll
+        for (int u = 0; u < kSymNP; ++u) {
+          const int j = lane + 64 * u;
+          if (j > k && j < n && j != r) Wk[j] = c[u];
+        }
+        if (lane == 0) {
+          Wk[k] = static_cast<T>(crr);
+          Wk[r] = static_cast<T>(ck);
+        }
+        const int pk = pos_get(pj, k), pr = pos_get(pj, r);
+        pos_set(pj, k, pr);
+        pos_set(pj, r, pk);
+        sym_patch<T>(rk1, k + 1, rk0, k, r);
+        sym_patch<T>(rk2, k + 2, rk0, k, r);
+      } else if (kind == 2) {
+        // interchange k+1 <-> r (:500-587); row k+1 of W becomes the
+        // candidate row; W(k, k+1) = C(k), W(k, r) = the old W(k, k+1)
+        const int k1 = k + 1;
+        T* Wk1 = W + (int64_t)(kk + 1) * n;
+        const T wk_k1 = pos_get(w, k1);
+        if (r != k1 && lane < kk)
+          swap_elems(&W[(int64_t)lane * n + k1], &W[(int64_t)lane * n + r]);
+#pragma unroll
+        for (int u = 0; u < kSymNP; ++u) {
+          const int j = lane + 64 * u;
+          if (j >= k + 2 && j < n && j != r) Wk1[j] = c[u];
+        }
+        if (lane == 0) {
+          Wk[k1] = static_cast<T>(ck);
+          if (r != k1) {
+            Wk[r] = wk_k1;
+            Wk1[r] = static_cast<T>(ck1);
+          }
+          Wk1[k1] = static_cast<T>(crr);
+        }
+        if (r != k1) {
+          const int pa = pos_get(pj, k1), pb = pos_get(pj, r);
+          pos_set(pj, k1, pb);
+          pos_set(pj, r, pa);
+          sym_patch<T>(rk2, k + 2, rk1, k1, r);
+        }
+      } else if (lane == 0) {
+        Wk[r] = static_cast<T>(ck);  // (:184-247)
+      }
+    }
+    if (kind != 2) {
+      // ScaleOneByOnePivot (:633-656): the record; U and D are written at
+      // the panel's end
+      if (lane == 0) {
+        spiv[k] = 1;
+        st.ty[kk] = 1;
+        st.c0[kk] = safe_den(kind == -1 ? crr : wkk);
+      }
+      kk += 1;
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) {
+        rk0[u] = rk1[u];
+        rk1[u] = rk2[u];
+      }
+      have1 = true;
+    } else {
+      // ScaleTwoByTwoPivot (:658-704), binary64 without contraction:
+      // D = [W(k,k) C(k); C(k) C(r)]
+      if (lane == 0) {
+        double itl, ibr, ioff;
+        inverse_2x2_nc(wkk, ck, crr, &itl, &ibr, &ioff);
+        spiv[k] = 2;
+        spiv[k + 1] = -2;
+        st.ty[kk] = 2;
+        st.c0[kk] = itl;
+        st.c1[kk] = ioff;
+        st.ty[kk + 1] = -2;
+        st.c0[kk + 1] = ioff;
+        st.c1[kk + 1] = ibr;
+      }
+      kk += 2;
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) rk0[u] = rk2[u];
+      have1 = false;
+    }
+    __syncthreads();
+  }
+  return e;
+}
+
+// The panel's finished rows s..e-1 into S: row s+p at physical row
+// pi[s+p]: D(p, p) = W(p, s+p), D(p, p+1) = W(p, s+p+1) for the first row
+// of a 2x2, and U(p, j) = ucol(p, j) at every later column (the values
+// ScaleOneByOnePivot / ScaleTwoByTwoPivot form, :633-704).
+template <typename T>
+__device__ void sym_write_rows(gmem_t<T>* S, int n, const int (&pj)[kSymNP], int s, int e,
+                               const T* W, const ScaleTab& st) {
+  const int lane = threadIdx.x & 63;
+  for (int p = 0; p < e - s; ++p) {
+    const int k = s + p;
+    gmem_t<T>* row = S + static_cast<unsigned>(pos_get(pj, k) * n);
+    const int dlast = st.ty[p] == 2 ? k + 1 : k;  // last column holding D
+#pragma unroll
+    for (int u = 0; u < kSymNP; ++u) {
+      const int j = lane + 64 * u;
+      if (j >= k && j < n) {
+        const int jc = j;
+        row[static_cast<unsigned>(pj[u])] = j <= dlast ? W[p * n + jc] : ucol<T>(st, W, n, p, jc);
+      }
+    }
+  }
+}
+
+// Trailing update of the logical upper triangle of [e, n)^2,
+// C(i, j) -= sum_{p < kp} U(p, i) W(p, j), as mfma_update does it (same K
+// steps of 4, U negated, accumulators started from C), each result stored
+// at (i, j) and at its mirror.  U comes from ucol (LDS), C from S; a
+// wavefront walks the tile rows, loading the next tile's C values (two
+// register sets, used in turn) while the current tile's MFMAs run.
+// Operands of rows i >= n or columns j >= n only reach outputs that are
+// not stored, so only the K rows past kp (in the last, partial K step) are
+// masked.  pi: LDS copy of the map.
+template <typename T, int TS>
+__device__ void sym_update(gmem_t<T>* S, int n, const int* pi, int s, int e, const T* W,
+                           const ScaleTab& st) {
+  constexpr int TB = TS / 16;
+  constexpr int KS = (kSymMaxWidth + 1 + 3) / 4;
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int lane = threadIdx.x & 63;
+  const int m = n - e, kp = e - s;
+  if (m <= 0 || kp <= 0) return;
+  const int q = lane >> 4, cl = lane & 15;
+  const int nkf = kp >> 2;            // K steps whose four rows are all < kp
+  const int nk = (kp + 3) >> 2;       // with the partial one
+  const int nt = (m + TS - 1) / TS;
+  for (int I = 0; I < nt; ++I) {
+    const int i0 = e + I * TS;
+    // physical rows of the lane's outputs (and those times n): every S
+    // offset fits 32 bits (n <= 256)
+    unsigned prow[TB][4], prn[TB][4];
+#pragma unroll
+    for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        prow[ti][v] = pi[min(i0 + 16 * ti + Mfma<T>::row(lane, v), n - 1)];
+        prn[ti][v] = prow[ti][v] * n;
+      }
+    // -U(p, i) for the tile row's K steps
+    T a[TB][KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int p = 4 * ks + q;
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb) {
+        const int ia = min(i0 + 16 * tb + cl, n - 1);
+        if (ks < nkf) {
+          a[tb][ks] = -ucol<T>(st, W, n, p, ia);
+        } else if (ks < nk) {
+          const T u = ucol<T>(st, W, n, min(p, kp - 1), ia);
+          a[tb][ks] = p < kp ? -u : T(0);
+        } else {
+          a[tb][ks] = T(0);
+        }
+      }
+    }
+    auto load_c = [&](int J, acc_t (&acc)[TB][TB], unsigned (&pc)[TB]) {
+      const int jj = e + J * TS;
+#pragma unroll
+      for (int tj = 0; tj < TB; ++tj) pc[tj] = pi[min(jj + 16 * tj + cl, n - 1)];
+#pragma unroll
+      for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TB; ++tj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[ti][tj][v] = S[prn[ti][v] + pc[tj]];
+    };
+    auto tile = [&](int J, acc_t (&acc)[TB][TB], const unsigned (&pc)[TB]) {
+      const int jj = e + J * TS;
+      // W(p, j) for every K step of the tile, read together
+      T b[KS][TB];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int p = 4 * ks + q;
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int jb = min(jj + 16 * tb + cl, n - 1);
+          if (ks < nkf) {
+            b[ks][tb] = W[p * n + jb];
+          } else if (ks < nk) {
+            const T wv = W[min(p, kp - 1) * n + jb];
+            b[ks][tb] = p < kp ? wv : T(0);
+          } else {
+            b[ks][tb] = T(0);
+          }
+        }
+      }
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        if (ks < nk) {
+#pragma unroll
+          for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < TB; ++tj)
+              acc[ti][tj] = Mfma<T>::mma(a[ti][ks], b[ks][tj], acc[ti][tj]);
+        }
+      }
+#pragma unroll
+      for (int tj = 0; tj < TB; ++tj) {
+        const unsigned pcn = pc[tj] * n;
+#pragma unroll
+        for (int ti = 0; ti < TB; ++ti)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = i0 + 16 * ti + Mfma<T>::row(lane, v), j = jj + 16 * tj + cl;
+            if (i <= j && j < n) {
+              S[prn[ti][v] + pc[tj]] = acc[ti][tj][v];
+              if (i < j) S[pcn + prow[ti][v]] = acc[ti][tj][v];
+            }
+          }
+      }
+    };
+    acc_t c0[TB][TB], c1[TB][TB];
+    unsigned pc0[TB], pc1[TB];
+    load_c(I, c0, pc0);
+    for (int J = I; J < nt; J += 2) {
+      const bool two = J + 1 < nt;
+      if (two) load_c(J + 1, c1, pc1);
+      tile(J, c0, pc0);
+      if (two) {
+        if (J + 2 < nt) load_c(J + 2, c0, pc0);
+        tile(J + 1, c1, pc1);
+      }
+    }
+  }
+}
+
+// S := the symmetric matrix whose upper triangle is A's (four rows in
+// flight per pass).
+template <typename T>
+__device__ void sym_load(const gmem_t<T>* A, int64_t lda, int n, gmem_t<T>* S) {
+  const int lane = threadIdx.x & 63;
+  for (int i0 = 0; i0 < n; i0 += 4) {
+    T v[4][kSymNP];
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) {
+        const int i = min(i0 + d, n - 1), j = min(i + lane + 64 * u, n - 1);
+        v[d][u] = A[i * lda + static_cast<unsigned>(j)];
+      }
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) {
+        const int i = i0 + d, j = i + lane + 64 * u;
+        if (i < n && j < n) {
+          S[static_cast<unsigned>(i * n + j)] = v[d][u];
+          S[static_cast<unsigned>(j * n + i)] = v[d][u];
+        }
+      }
+  }
+}
+
+// A(k, j) := S(pi[k], pi[j]) for k <= j (the factors in the final logical
+// order), perm := pi, piv from LDS; four rows in flight per pass.
+template <typename T>
+__device__ void sym_store(const gmem_t<T>* S, int n, const int* pi, const int* spiv,
+                          gmem_t<T>* A, int64_t lda, gmem_t<int>* perm, gmem_t<int>* piv) {
+  const int lane = threadIdx.x & 63;
+  int pj[kSymNP];
+#pragma unroll
+  for (int u = 0; u < kSymNP; ++u) pj[u] = pi[min(lane + 64 * u, n - 1)];
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    T v[4][kSymNP];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const gmem_t<T>* row = S + static_cast<unsigned>(pi[min(k0 + d, n - 1)] * n);
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) v[d][u] = row[static_cast<unsigned>(pj[u])];
+    }
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u) {
+        const int k = k0 + d, j = lane + 64 * u;
+        if (k < n && j >= k && j < n) A[k * lda + static_cast<unsigned>(j)] = v[d][u];
+      }
+  }
+  for (int i = lane; i < n; i += 64) {
+    perm[i] = pi[i];
+    piv[i] = spiv[i];
+  }
+}
+
+// Persistent kernel of the symmetric-workspace mode: workgroups (one
+// wavefront each) take matrices from the work queue (largest first).
+// Dynamic LDS per matrix of order n and panel width bw: W ((bw + 1) x n),
+// pi and the pivot types (n ints each).  Workspace slice of the workgroup:
+// n x n elements of S.
+// (binary64: LDS admits one wavefront per SIMD, so it may take all 512
+// registers; binary32 keeps two per SIMD)
+template <typename T>
+__global__ __launch_bounds__(64, sizeof(T) == 8 ? 1 : 2) void k_batched_sym(BatchArgs<T> a) {
+  __shared__ T Lk[kSymMaxWidth + 2];
+  __shared__ T Lr[kSymMaxWidth + 2];
+  __shared__ ScaleTab st;
+  __shared__ int sq;
+  T* lds = reinterpret_cast<T*>(bbk_batched_lds);
+  gmem_t<T>* S = to_global(a.ws) + (int64_t)blockIdx.x * a.ws_stride;
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    if (lane == 0) sq = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int q = sq;
+    __syncthreads();
+    if (q >= a.count) break;
+    const int mi = a.order[q];
+    const int n = a.ns[mi];
+    if (n <= 0) continue;
+    const int c = batch_class(n);
+    const int bw = a.widths.w[c];
+    T* W = lds;
+    int* pi = reinterpret_cast<int*>(W + (int64_t)(bw + 1) * n);
+    int* spiv = pi + n;
+    gmem_t<T>* A = to_global(a.A[mi]);
+    const int64_t lda = a.lda[mi];
+    int pj[kSymNP];
+#pragma unroll
+    for (int u = 0; u < kSymNP; ++u) pj[u] = min(lane + 64 * u, n - 1);
+    sym_load<T>(A, lda, n, S);
+    const int ts = a.widths.ts[c] == 32 ? 32 : 16;
+    int s = 0;
+    for (int P = 0; s < n; ++P) {
+      const int e = sym_panel<T, sizeof(T) == 8>(S, n, pj, spiv, bw, P, s, W, Lk, Lr, st);
+#pragma unroll
+      for (int u = 0; u < kSymNP; ++u)
+        if (lane + 64 * u < n) pi[lane + 64 * u] = pj[u];
+      __syncthreads();
+      sym_write_rows<T>(S, n, pj, s, e, W, st);
+      if (ts == 32)
+        sym_update<T, 32>(S, n, pi, s, e, W, st);
+      else
+        sym_update<T, 16>(S, n, pi, s, e, W, st);
+      __syncthreads();
+      s = e;
+    }
+    sym_store<T>(S, n, pi, spiv, A, lda, to_global(a.perm[mi]), to_global(a.piv[mi]));
+    __syncthreads();
+  }
+}
+
+// Dynamic LDS bytes of k_batched_sym for orders up to nmax, widths up to wmax.
+__host__ __device__ constexpr int sym_lds_bytes(int nmax, int wmax, int elem_bytes) {
+  return ((wmax + 1) * nmax * elem_bytes + 2 * nmax * 4 + 15) / 16 * 16;
 }
 
 // ---- lockstep mode (batches of large matrices) -----------------------------
@@ -1298,6 +1832,21 @@ static int resident_grid(Handle* h, int count) {
   return std::max(1, std::min(count, per_cu * cus));
 }
 
+template <typename T>
+static int sym_grid(Handle* h, int count, int lds) {
+  int per_cu = 0, cus = 0;
+  const void* kern = reinterpret_cast<const void*>(&k_batched_sym<T>);
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess ||
+      cus < 1)
+    cus = 1;
+  return std::max(1, std::min(count, per_cu * cus));
+}
+
 static int batched_grid(Handle* h, int count, int elem_bytes, int nt) {
   if (elem_bytes == 8)
     return nt == 64 ? batched_grid_nt<double, 64>(h, count) : batched_grid_nt<double, 256>(h, count);
@@ -1364,6 +1913,7 @@ static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes
   p.grid = count;
   p.nt = 256;
   p.resident = false;
+  p.sym = false;
   return BBK_OK;
 }
 
@@ -1383,6 +1933,45 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   int nmax = 0;
   for (int i = 0; i < count; ++i) nmax = std::max(nmax, h_n[i]);
   if (nmax > kBatchLockstepN) return prepare_lockstep(h, count, h_n, elem_bytes, wt);
+  if (nmax <= kBatchResidentN && wt.nt[batch_class(nmax)] == kBatchTeamSym) {
+    // symmetric-workspace kernel: LDS for the widest panel of the batch's
+    // classes, one n x n slice of S per workgroup
+    int wmax = 1;
+    for (int c = 0; c <= batch_class(std::max(nmax, 1)); ++c) wmax = std::max(wmax, wt.w[c]);
+    if (wmax > kSymMaxWidth)
+      return h->fail(BBK_ERR_UNSUPPORTED,
+                     "batched: the symmetric-workspace mode takes panel widths up to 32");
+    const int lds = sym_lds_bytes(std::max(nmax, 1), wmax, elem_bytes);
+    int grid = elem_bytes == 8 ? sym_grid<double>(h, std::max(count, 1), lds)
+                               : sym_grid<float>(h, std::max(count, 1), lds);
+    if (h->batch_grid_limit > 0) grid = std::min(grid, h->batch_grid_limit);
+    const int64_t stride = ((int64_t)std::max(nmax, 1) * std::max(nmax, 1) + 63) / 64 * 64;
+    const size_t ints = 2 * static_cast<size_t>(count) + 64;
+    int rc = h->reserve_buf(&p.buf, &p.buf_bytes, ints * sizeof(int), "batch schedule");
+    if (rc != BBK_OK) return rc;
+    rc = h->reserve_buf(&p.ws, &p.ws_bytes, static_cast<size_t>(stride) * grid * elem_bytes,
+                        "batch workspace");
+    if (rc != BBK_OK) return rc;
+    std::vector<int> host(ints, 0);
+    std::copy(order.begin(), order.end(), host.begin());
+    std::copy(h_n, h_n + count, host.begin() + count);
+    hipError_t e = hipMemcpyAsync(p.buf, host.data(), ints * sizeof(int), hipMemcpyHostToDevice,
+                                  h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return h->hip_fail(e, "upload batch schedule");
+    p.count = count;
+    p.elem = elem_bytes;
+    p.sizes.assign(h_n, h_n + count);
+    p.widths = wt;
+    p.ws_stride = stride;
+    p.grid = grid;
+    p.nt = 64;
+    p.lockstep = false;
+    p.resident = false;
+    p.sym = true;
+    p.lds_dyn = lds;
+    return BBK_OK;
+  }
   if (nmax <= kBatchResidentN && wt.nt[batch_class(nmax)] == 0) {
     int grid = elem_bytes == 8 ? resident_grid<double>(h, std::max(count, 1))
                                : resident_grid<float>(h, std::max(count, 1));
@@ -1406,6 +1995,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
     p.nt = 256;
     p.lockstep = false;
     p.resident = true;
+    p.sym = false;
     return BBK_OK;
   }
   const int nt = wt.nt[batch_class(nmax)] == 64 ? 64 : 256;  // 0 (resident) above 256: four waves
@@ -1444,6 +2034,7 @@ int batched_prepare(Handle* h, int count, const int* h_n, int elem_bytes) {
   p.nt = nt;
   p.lockstep = false;
   p.resident = false;
+  p.sym = false;
   return BBK_OK;
 }
 
@@ -1518,7 +2109,9 @@ int batched_launch(Handle* h, const BatchPlan& p, T* const* A, const int64_t* ld
   a.lds_bytes = kBatchResidentLdsBytes;
   a.widths = p.widths;
   hipLaunchKernelGGL(k_batch_fill, dim3(1), dim3(64), 0, h->stream, a.queue, 1, 0);
-  if (p.resident)
+  if (p.sym)
+    hipLaunchKernelGGL(k_batched_sym<T>, dim3(p.grid), dim3(64), p.lds_dyn, h->stream, a);
+  else if (p.resident)
     hipLaunchKernelGGL(k_batched_lds<T>, dim3(p.grid), dim3(256), kBatchResidentLdsBytes,
                        h->stream, a);
   else if (p.nt == 64)
@@ -1644,22 +2237,24 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
     float best_ms = 1e30f;
     int best_w = tab.w[c], best_nt = tab.nt[c];
     // classes n <= kBatchResidentN: panel width x mode (one-wavefront team,
-    // four-wavefront team, resident in LDS) x trailing update (SIMT 4 x 4,
-    // MFMA 16 x 16, MFMA 32 x 32; the resident mode is MFMA only); larger
-    // classes: panel width x team size, SIMT update
+    // four-wavefront team, resident in LDS, symmetric workspace) x trailing
+    // update (SIMT 4 x 4, MFMA 16 x 16, MFMA 32 x 32; the resident and
+    // symmetric-workspace modes are MFMA only); larger classes: panel width
+    // x team size, SIMT update
     static const int rcand[] = {8, 16, 24, 32};
-    static const int modes[] = {64, 256, 0};
+    static const int modes[] = {64, 256, 0, kBatchTeamSym};
     static const int tiles[] = {0, 16, 32};
     const bool res_class = hi <= kBatchResidentN;
     const int nw_small = static_cast<int>(sizeof(rcand) / sizeof(rcand[0]));
-    const int ntrials = res_class ? nw_small * 3 * 3
+    const int ntrials = res_class ? nw_small * 4 * 3
                                   : 2 * static_cast<int>(sizeof(cand) / sizeof(cand[0]));
     int best_ts = tab.ts[c];
     for (int trial = 0; trial < ntrials; ++trial) {
-      const int w = res_class ? rcand[trial / 9] : cand[trial >> 1];
-      const int nt = res_class ? modes[(trial / 3) % 3] : ((trial & 1) ? 256 : 64);
+      const int w = res_class ? rcand[trial / 12] : cand[trial >> 1];
+      const int nt = res_class ? modes[(trial / 3) % 4] : ((trial & 1) ? 256 : 64);
       const int ts = res_class ? tiles[trial % 3] : 0;
-      if (res_class && nt == 0 && ts == 0) continue;
+      // the resident and symmetric-workspace kernels have MFMA updates only
+      if (res_class && (nt == 0 || nt == kBatchTeamSym) && ts == 0) continue;
       if (rc != BBK_OK) break;
       if (w > kBatchMaxWidth) continue;
       BatchWidths saved = tab;
@@ -1849,7 +2444,8 @@ int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes, const int* t
   if (!team_sizes) return -3;
   for (int c = 0; c < bbk::kBatchClasses; ++c)
     if (team_sizes[c] != 64 && team_sizes[c] != 256 &&
-        !(team_sizes[c] == 0 && bbk::batch_class_bound(c) <= bbk::kBatchResidentN))
+        !((team_sizes[c] == 0 || team_sizes[c] == bbk::kBatchTeamSym) &&
+          bbk::batch_class_bound(c) <= bbk::kBatchResidentN))
       return -3;
   Handle* h = reinterpret_cast<Handle*>(handle);
   bbk::BatchWidths& t = h->batch_widths[elem_bytes == 8 ? 0 : 1];

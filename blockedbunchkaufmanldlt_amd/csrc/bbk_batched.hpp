@@ -27,6 +27,11 @@ constexpr int kBatchLockstepN = 512;
 constexpr int kBatchResidentN = 256;
 // Dynamic LDS of the resident kernel (one workgroup per CU).
 constexpr int kBatchResidentLdsBytes = 152 * 1024;
+// Team-mode code of the symmetric-workspace kernel (classes up to
+// kBatchResidentN): one wavefront per matrix, a full symmetric copy of the
+// matrix in a per-workgroup workspace slice, interchanges as index-map
+// updates.
+constexpr int kBatchTeamSym = 1;
 __host__ __device__ constexpr int batch_lds_bytes(int nt) {
   return nt == 64 ? kBatchLdsBytesSmall : kBatchLdsBytes;
 }
@@ -37,12 +42,14 @@ __host__ __device__ constexpr int batch_class_bound(int c) {
 }
 
 // Per size class: panel width w, team nt and trailing-update tile ts.
-//   nt  64 or 256 threads per matrix (the global-memory kernel), or 0 for
-//       classes up to kBatchResidentN: the resident kernel (four
-//       wavefronts, the matrix in LDS).  A batch runs with the mode of its
+//   nt  64 or 256 threads per matrix (the global-memory kernel), or for
+//       classes up to kBatchResidentN 0: the resident kernel (four
+//       wavefronts, the matrix in LDS) or kBatchTeamSym: the
+//       symmetric-workspace kernel.  A batch runs with the mode of its
 //       largest matrix's class.
 //   ts  0: SIMT 4 x 4 register tiles; 16 or 32: MFMA tiles per wavefront
-//       (ts x ts outputs); the resident kernel uses MFMA (0 counts as 16).
+//       (ts x ts outputs); the resident and symmetric-workspace kernels use
+//       MFMA (0 counts as 16).
 struct BatchWidths {
   int w[kBatchClasses];
   int nt[kBatchClasses];
@@ -76,6 +83,8 @@ struct BatchPlan {
   int nt = 64;          // threads per matrix team
   bool lockstep = false;
   bool resident = false;  // matrices held in LDS (largest n <= kBatchResidentN)
+  bool sym = false;       // symmetric-workspace kernel (largest n <= kBatchResidentN)
+  int lds_dyn = 0;        // sym: dynamic LDS bytes per workgroup
   int steps = 0;        // lockstep: panel steps (largest ceil(n / width))
   int64_t step_stride = 0;  // lockstep: ints per step in the tile prefix table
   int64_t ws_elems = 0;

@@ -267,7 +267,7 @@ def test_batched_autotuner(oracle, handle):
         assert all(w in (8, 12, 16, 24, 32, 48, 64) for w in widths[:3])
         assert all(t in (0, 16, 32) for t in handle.batched_tiles(torch.float64))
         assert widths == handle.batched_widths(torch.float64)
-        assert all(t in (0, 64, 256) for t in handle.batched_team_sizes(torch.float64))
+        assert all(t in (0, 1, 64, 256) for t in handle.batched_team_sizes(torch.float64))
         ns = [30, 60, 120]
         mats = handle.gen_synth_batched(ns, seed=8, zero_diag_stride=7, dtype=torch.float64)
         hosts = [host_matrix(m) for m in mats]
