@@ -193,10 +193,14 @@ int bbk_batched_get_widths(bbk_handle_t handle, int elem_bytes, int* widths);
 int bbk_batched_set_widths(bbk_handle_t handle, int elem_bytes,
                            const int* widths);
 /* Team per class: 64 or 256 threads per matrix (the global-memory
- * kernel), or 0 for classes n <= 256: the resident kernel, which keeps the
+ * kernel), or for classes n <= 256 0: the resident kernel, which keeps the
  * matrix (packed upper triangle), its panel rows and permutation in LDS
- * with one 256-thread workgroup per matrix.  A batch runs with the mode of
- * the class of its largest matrix. */
+ * with one 256-thread workgroup per matrix, or 1: the symmetric-workspace
+ * kernel (one wavefront per matrix, a full symmetric copy of the matrix in
+ * the handle's workspace, interchanges as index-map updates; panel widths
+ * up to 32, wider ones fail with BBK_ERR_UNSUPPORTED).  A batch runs with
+ * the mode of the class of its largest matrix.  All modes give the same
+ * factors (bit for bit with MFMA update tiles). */
 int bbk_batched_get_team_sizes(bbk_handle_t handle, int elem_bytes,
                                int* team_sizes);
 int bbk_batched_set_team_sizes(bbk_handle_t handle, int elem_bytes,
@@ -216,7 +220,8 @@ int bbk_batched_set_grid_limit(bbk_handle_t handle, int max_workgroups);
 /* Autotuner: for every size class up to max_n, times candidates on a
  * synthetic batch of that class and keeps the fastest in the handle's
  * table (widths written to widths_out when not NULL): for classes n <= 256
- * every (panel width, team mode 64 / 256 / resident, update tile SIMT /
+ * every (panel width, team mode 64 / 256 / resident / symmetric
+ * workspace, update tile SIMT /
  * MFMA 16 / MFMA 32) combination, above that (panel width, team size).
  * Synchronous. */
 int bbk_batched_tune(bbk_handle_t handle, int elem_bytes, int max_n,
