@@ -61,6 +61,9 @@ def lib() -> ctypes.CDLL:
     for name in ("bbk_dsytrf", "bbk_ssytrf"):
         getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp]
     L.bbk_dsytrf_host.argtypes = [vp, i32, i32, vp, i64, vp, vp]
+    L.bbk_dsytrf_ooc.argtypes = [vp, i32, i32, vp, i64, vp, vp, ctypes.c_size_t]
+    L.bbk_host_alloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    L.bbk_host_free.argtypes = [vp]
     for name in ("bbk_dsytrs", "bbk_ssytrs"):
         getattr(L, name).argtypes = [vp, i32, i32, vp, i64, vp, vp, vp, i64]
     L.bbk_dsyberr.argtypes = [vp, i32, i32, vp, i64, vp, i64, vp, i64, vp]
@@ -219,6 +222,20 @@ class Handle:
         else:
             raise TypeError("A must be float64 or float32")
         self.check(rc, "sytrf")
+        return perm, piv
+
+    def sytrf_ooc(self, M: "PinnedMatrix", nb: int = 0, device_bytes: int = 0):
+        """Factor the float64 matrix held in pinned host memory M in place
+        (bbk_dsytrf_ooc): HBM use for the matrix's working copies bounded by
+        device_bytes (0 = the free HBM less 1 GiB).  Synchronous.  Returns
+        (perm, piv) as int32 numpy arrays."""
+        import numpy as np
+        perm = np.empty(M.n, dtype=np.int32)
+        piv = np.empty(M.n, dtype=np.int32)
+        with self.ordered():
+            rc = lib().bbk_dsytrf_ooc(self.h, M.n, nb, M.ptr, M.lda,
+                                      perm.ctypes.data, piv.ctypes.data, device_bytes)
+        self.check(rc, "sytrf_ooc")
         return perm, piv
 
     def sytrs(self, F, perm, piv, B):
@@ -482,6 +499,37 @@ class Handle:
             self.check(lib().bbk_kernel_update_trailing(self.h, rows, cols, inner, _ptr(left), ls,
                                                         _ptr(work), ws, _ptr(target), ts),
                        "update_trailing")
+
+
+class PinnedMatrix:
+    """An (n, lda) row-major float64 matrix in pinned host memory
+    (bbk_host_alloc), the storage bbk_dsytrf_ooc factors in place.
+    ``array`` is a numpy view of it, valid until close()."""
+
+    def __init__(self, n: int, lda: Optional[int] = None):
+        import numpy as np
+        self.n = n
+        self.lda = n if lda is None else lda
+        assert self.lda >= n > 0
+        p = ctypes.c_void_p()
+        rc = lib().bbk_host_alloc(ctypes.byref(p), 8 * n * self.lda)
+        if rc != 0:
+            raise BbkError(rc, "bbk_host_alloc failed")
+        self.ptr = p.value
+        buf = (ctypes.c_double * (n * self.lda)).from_address(self.ptr)
+        self.array = np.ctypeslib.as_array(buf).reshape(n, self.lda)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            lib().bbk_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BatchGraph:

@@ -110,6 +110,39 @@ template int panel_plan<float>(int, int, int*, int*, hipError_t*);
 // 1024 per launch).  h (may be NULL) receives timing marks and the panel
 // count.
 template <typename T>
+hipError_t queue_panel(int n, int b, int s_nom, T* A, int64_t lda, int* perm, int* piv,
+                       const FactorScratch& fs, int nt, int max_groups, int* epoch,
+                       hipStream_t st, unsigned* status, long long wait_ticks) {
+  const int groups = (n - s_nom + nt - 1) / nt;
+  PanelArgs pa;
+  pa.n = n;
+  pa.b = b;
+  pa.lda = lda;
+  pa.state = fs.state;
+  pa.swaps = fs.swaps;
+  pa.perm = perm;
+  pa.piv = piv;
+  pa.xrec = fs.xrec;
+  pa.epoch = *epoch;
+  *epoch += 1024;
+  pa.Wg = fs.Wg;
+  pa.ldw = fs.ldw;
+  // Small panels run on one XCD: the grid is spread over the 8 XCDs and
+  // only every 8th workgroup works, when all of them fit co-resident.
+  pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
+  pa.status = status;
+  pa.wait_ticks = wait_ticks;
+  return launch_panel<T>(A, pa, groups * pa.xcd_stride, nt, st);
+}
+
+template hipError_t queue_panel<double>(int, int, int, double*, int64_t, int*, int*,
+                                        const FactorScratch&, int, int, int*, hipStream_t,
+                                        unsigned*, long long);
+template hipError_t queue_panel<float>(int, int, int, float*, int64_t, int*, int*,
+                                       const FactorScratch&, int, int, int*, hipStream_t,
+                                       unsigned*, long long);
+
+template <typename T>
 int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const FactorScratch& fs,
                 int nt, int max_groups, int* epoch, hipStream_t st, Handle* h, unsigned* status,
                 long long wait_ticks) {
@@ -117,27 +150,9 @@ int factor_loop(int n, int b, T* A, int64_t lda, int* perm, int* piv, const Fact
   launch_init(n, perm, piv, fs.state, status, st);
   int panels = 0;
   for (int s_nom = 0; s_nom < n; s_nom += b, ++panels) {
-    const int groups = (n - s_nom + nt - 1) / nt;
-    PanelArgs pa;
-    pa.n = n;
-    pa.b = b;
-    pa.lda = lda;
-    pa.state = fs.state;
-    pa.swaps = fs.swaps;
-    pa.perm = perm;
-    pa.piv = piv;
-    pa.xrec = fs.xrec;
-    pa.epoch = *epoch;
-    *epoch += 1024;
-    pa.Wg = fs.Wg;
-    pa.ldw = fs.ldw;
-    // Small panels run on one XCD: the grid is spread over the 8 XCDs and
-    // only every 8th workgroup works, when all of them fit co-resident.
-    pa.xcd_stride = 8 * groups <= max_groups ? 8 : 1;
-    pa.status = status;
-    pa.wait_ticks = wait_ticks;
     if (h) h->timing_mark(Handle::kPanel, true);
-    hipError_t e = launch_panel<T>(A, pa, groups * pa.xcd_stride, nt, st);
+    hipError_t e = queue_panel<T>(n, b, s_nom, A, lda, perm, piv, fs, nt, max_groups, epoch, st,
+                                  status, wait_ticks);
     if (e != hipSuccess) return h ? h->hip_fail(e, "panel launch") : BBK_ERR_HIP;
     if (h) h->timing_mark(Handle::kPanel, false);
     // Rows above this panel: at most s_nom + panels (each earlier panel
@@ -179,27 +194,35 @@ int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
     return h->fail(BBK_ERR_UNSUPPORTED,
                    "matrix order exceeds the co-resident panel grid (n <= 131072)");
   FactorScratch fs;
+  int rc = factor_setup<T>(h, n, &fs);
+  if (rc != BBK_OK) return rc;
+  return factor_loop<T>(n, b, A, lda, perm, piv, fs, nt, max_groups, &h->epoch, h->stream, h,
+                        h->d_status, h->wait_ticks());
+}
+
+template <typename T>
+int factor_setup(Handle* h, int n, FactorScratch* fs) {
   const size_t bytes = factor_scratch_bytes<T>(n, nullptr, nullptr);
   int rc = h->reserve(bytes);
   if (rc != BBK_OK) return rc;
-  factor_scratch_bytes<T>(n, &fs, reinterpret_cast<char*>(h->ws));
-
-  hipStream_t st = h->stream;
+  factor_scratch_bytes<T>(n, fs, reinterpret_cast<char*>(h->ws));
   // Exchange-record tags must never repeat within the live record area:
   // each launch takes a fresh range of 1024 tags; a (re)allocated area, an
   // area laid out for another order or type, or a wrapped tag range is
   // cleared first (fill kernel).
   const long layout = static_cast<long>(n) * 16 + static_cast<long>(sizeof(T));
   if (h->ws_gen != h->ws_clear_gen || h->ws_layout != layout || h->epoch > (1 << 30)) {
-    launch_zero_u32(reinterpret_cast<unsigned*>(fs.xrec),
-                    static_cast<int>(fs.xrec_bytes / sizeof(unsigned)), st);
+    launch_zero_u32(reinterpret_cast<unsigned*>(fs->xrec),
+                    static_cast<int>(fs->xrec_bytes / sizeof(unsigned)), h->stream);
     h->ws_clear_gen = h->ws_gen;
     h->ws_layout = layout;
     h->epoch = 1024;
   }
-  return factor_loop<T>(n, b, A, lda, perm, piv, fs, nt, max_groups, &h->epoch, st, h,
-                        h->d_status, h->wait_ticks());
+  return BBK_OK;
 }
+
+template int factor_setup<double>(Handle*, int, FactorScratch*);
+template int factor_setup<float>(Handle*, int, FactorScratch*);
 
 template int factorize<double>(Handle*, int, int, double*, int64_t, int*, int*);
 template int factorize<float>(Handle*, int, int, float*, int64_t, int*, int*);

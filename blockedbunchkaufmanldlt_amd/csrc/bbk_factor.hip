@@ -54,12 +54,14 @@ __device__ __forceinline__ void swap_rows(T* __restrict__ A, int64_t lda,
 
 // One launch after each panel: workgroups [0, gemm_blocks) update trailing
 // tiles (XCD-contiguous order), the rest apply the panel's interchanges to
-// RPG rows each of the finished rows above the panel.  The two regions are
-// disjoint (rows >= e against rows < s), so both run concurrently.
+// RPG rows each of the finished rows [rows_lo, s) above the panel.  The two
+// regions are disjoint (rows >= e against rows < s), so both run
+// concurrently.
 template <typename T, int TM, int MINB, int KC, int RPG, bool SUPER = true>
 __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
     T* __restrict__ A, int64_t lda, const T* __restrict__ Wg, int64_t ldw,
-    const int* __restrict__ state, const int* __restrict__ swaps, int n, int gemm_blocks) {
+    const int* __restrict__ state, const int* __restrict__ swaps, int n, int gemm_blocks,
+    int rows_lo, int rows_hi) {
   constexpr int NE = 2 * (kPanelMaxWidth + 1);
   constexpr int kStage = 2 * KC * (TM + 4);  // elements of one operand's two buffers
   static_assert(3 * NE * sizeof(int) <= 2 * kStage * sizeof(T),
@@ -84,10 +86,37 @@ __global__ __launch_bounds__(256, MINB) void bk_trailing_update_kernel(
     update_tile<T, TM, KC, SUPER>(A, lda, Wg, ldw, s, bw, n, idx, Us, Ws);
   } else {
     const int ns = state[PanelState::kNumSwaps];
-    const int p0 = (idx - gemm_blocks) * RPG;
-    if (ns == 0 || p0 >= s) return;
-    swap_rows<T, RPG>(A, lda, swaps, ns, p0, min(s, p0 + RPG), cols, slot, content);
+    const int p0 = rows_lo + (idx - gemm_blocks) * RPG;
+    const int p1 = min(min(s, rows_hi), p0 + RPG);
+    if (ns == 0 || p0 >= p1) return;
+    swap_rows<T, RPG>(A, lda, swaps, ns, p0, p1, cols, slot, content);
   }
+}
+
+// The trailing update restricted to tile columns [ja, jb) of the trailing
+// triangle (every tile I <= J of those columns), on a copy of that part of
+// the matrix: C and U are bases shifted by their copies' origins, so both
+// are indexed with the matrix's own row and column numbers.  Tiles in
+// column-major triangle order, XCD-contiguous runs.
+template <typename T, int TM, int MINB, int KC>
+__global__ __launch_bounds__(256, MINB) void bk_slab_update_kernel(
+    T* __restrict__ C, int64_t ldc, const T* __restrict__ U, int64_t ldu,
+    const T* __restrict__ Wg, int64_t ldw, const int* __restrict__ state, int n, int ja,
+    int count) {
+  constexpr int kStage = 2 * KC * (TM + 4);
+  __shared__ __align__(16) T stage[2 * kStage];
+  T(*Us)[KC * (TM + 4)] = reinterpret_cast<T(*)[KC * (TM + 4)]>(stage);
+  T(*Ws)[KC * (TM + 4)] = reinterpret_cast<T(*)[KC * (TM + 4)]>(stage + kStage);
+  const int s = state[PanelState::kStart];
+  const int bw = state[PanelState::kWidth];
+  if (bw <= 0 || s + bw >= n) return;
+  const int nb = count, per = (nb + 7) / 8, x = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int full = nb - 8 * (per - 1);
+  const int idx = x < full ? x * per + r : full * per + (x - full) * (per - 1) + r;
+  if (idx >= count) return;
+  int I, J;
+  tri_tile(ja * (ja + 1) / 2 + idx, &I, &J);
+  update_tile_at<T, TM, KC>(C, ldc, U, ldu, Wg, ldw, s, bw, n, I, J, Us, Ws);
 }
 
 // Tile shape (measured on MI355X at n = 16384, K = 128, inside the
@@ -191,17 +220,33 @@ template void launch_factor_info<float>(int, const float*, int64_t, const int*, 
 template <typename T>
 void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                             const int* state, const int* swaps, int n, int m_bound,
-                            int rows_bound, hipStream_t stream) {
+                            int rows_bound, hipStream_t stream, int rows_lo) {
   constexpr int TM = UpdateCfg<T>::kTM;
   constexpr int RPG = 8;
   const int tiles = m_bound > 0 ? (m_bound + TM - 1) / TM : 0;
   const int gemm_blocks = tiles * (tiles + 1) / 2;
-  const int swap_blocks = rows_bound > 0 ? (rows_bound + RPG - 1) / RPG : 0;
+  const int swap_blocks = rows_bound > rows_lo ? (rows_bound - rows_lo + RPG - 1) / RPG : 0;
   if (gemm_blocks + swap_blocks == 0) return;
   hipLaunchKernelGGL((bk_trailing_update_kernel<T, TM, UpdateCfg<T>::kMinBlocks,
                                                  UpdateCfg<T>::kKC, RPG>),
                      dim3(gemm_blocks + swap_blocks), dim3(256), 0, stream, A, lda, Wg, ldw,
-                     state, swaps, n, gemm_blocks);
+                     state, swaps, n, gemm_blocks, rows_lo, rows_bound);
+}
+
+template <typename T>
+int slab_tile_width() {
+  return UpdateCfg<T>::kTM;
+}
+
+template <typename T>
+void launch_slab_update(T* C, int64_t ldc, const T* U, int64_t ldu, const T* Wg, int64_t ldw,
+                        const int* state, int n, int ja, int jb, hipStream_t stream) {
+  constexpr int TM = UpdateCfg<T>::kTM;
+  const int count = jb * (jb + 1) / 2 - ja * (ja + 1) / 2;
+  if (count <= 0) return;
+  hipLaunchKernelGGL((bk_slab_update_kernel<T, TM, UpdateCfg<T>::kMinBlocks, UpdateCfg<T>::kKC>),
+                     dim3(count), dim3(256), 0, stream, C, ldc, U, ldu, Wg, ldw, state, n, ja,
+                     count);
 }
 
 void launch_init(int n, int* perm, int* piv, int* state, unsigned* status, hipStream_t stream) {
@@ -217,9 +262,12 @@ void launch_zero_u32(unsigned* p, int count, hipStream_t stream) {
 
 template void launch_trailing_update<double>(double*, int64_t, const double*, int64_t,
                                              const int*, const int*, int, int, int,
-                                             hipStream_t);
+                                             hipStream_t, int);
 template void launch_trailing_update<float>(float*, int64_t, const float*, int64_t,
                                             const int*, const int*, int, int, int,
-                                            hipStream_t);
+                                            hipStream_t, int);
+template int slab_tile_width<double>();
+template void launch_slab_update<double>(double*, int64_t, const double*, int64_t, const double*,
+                                         int64_t, const int*, int, int, int, hipStream_t);
 
 }  // namespace bbk

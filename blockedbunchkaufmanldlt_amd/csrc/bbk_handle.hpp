@@ -82,6 +82,16 @@ int default_panel_width(int n, int elem_bytes);
 template <typename T>
 int factorize(Handle* h, int n, int nb, T* A, int64_t lda, int* perm,
               int* piv);
+// Scratch of a factorization of order n (exchange records cleared when
+// their area or layout changed), queued on h->stream.
+template <typename T>
+int factor_setup(Handle* h, int n, FactorScratch* fs);
+// One panel launch at nominal start s_nom (the actual start is in the
+// panel state), taking the next exchange-tag range from *epoch.
+template <typename T>
+hipError_t queue_panel(int n, int b, int s_nom, T* A, int64_t lda, int* perm, int* piv,
+                       const FactorScratch& fs, int nt, int max_groups, int* epoch,
+                       hipStream_t st, unsigned* status, long long wait_ticks);
 // The factorization's launch sequence on stream st with carved scratch fs
 // (h: timing and error text, may be NULL).
 template <typename T>

@@ -34,11 +34,20 @@ hipError_t launch_panel(T* A, const PanelArgs& args, int groups, int nt,
 template <typename T>
 hipError_t panel_max_groups(int nt, int* groups);
 // Trailing update of the panel just factored plus the interchange of the
-// rows above it (rows_bound: upper bound of the panel start; 0 = none).
+// rows [rows_lo, min(rows_bound, panel start)) above it (rows_bound: upper
+// bound of the panel start; rows_bound <= rows_lo: none).
 template <typename T>
 void launch_trailing_update(T* A, int64_t lda, const T* Wg, int64_t ldw,
                             const int* state, const int* swaps, int n, int m_bound,
-                            int rows_bound, hipStream_t stream);
+                            int rows_bound, hipStream_t stream, int rows_lo = 0);
+// The trailing update's tiles in tile columns [ja, jb) (all tiles I <= J of
+// those columns), on copies: C(i, j) = C[i ldc + j] and U(s + q, i) =
+// U[(s + q) ldu + i] with bases shifted by the copies' origins.
+template <typename T>
+int slab_tile_width();
+template <typename T>
+void launch_slab_update(T* C, int64_t ldc, const T* U, int64_t ldu, const T* Wg, int64_t ldw,
+                        const int* state, int n, int ja, int jb, hipStream_t stream);
 // After the last panel: the first singular pivot and the non-finite flag
 // of the factorization into the status words.
 template <typename T>

@@ -122,28 +122,24 @@ __device__ __forceinline__ void c_tile_io(T* __restrict__ A, int64_t lda, int n,
       }
 }
 
-template <typename T, int TM, int KC, bool SUPER = true>
-__device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
-                                            const T* __restrict__ Wg, int64_t ldw, int s,
-                                            int bw, int n, int idx, T (*Us)[KC * (TM + 4)],
-                                            T (*Ws)[KC * (TM + 4)]) {
+// Tile (I, J) of the trailing triangle: C(i, j) = C[i ldc + j] and
+// U(s + q, i) = U[(s + q) ldu + i] in the matrix's own indices (the
+// factorization passes A for both; a copy of part of the matrix passes a
+// base shifted by the copy's origin), W(q, j) = Wg[q ldw + j].  Every
+// element takes the same MFMA chain whichever tile or copy holds it, so any
+// tiling of the update gives bitwise the same result.
+template <typename T, int TM, int KC>
+__device__ __forceinline__ void update_tile_at(T* __restrict__ A, int64_t lda,
+                                               const T* __restrict__ U, int64_t ldu,
+                                               const T* __restrict__ Wg, int64_t ldw, int s,
+                                               int bw, int n, int I, int J,
+                                               T (*Us)[KC * (TM + 4)], T (*Ws)[KC * (TM + 4)]) {
   constexpr int LDS_LD = TM + 4;
   constexpr int PER = KC * TM / 256;  // staged elements per thread per operand
   constexpr int WT = TM / 2;          // wavefront tile
   constexpr int TI = WT / 16;         // 16 x 16 MFMA blocks per wavefront side
   typedef typename Mfma<T>::acc_t acc_t;
   const int e = s + bw;
-  const int m = n - e;
-  if (m <= 0 || bw <= 0) return;
-  const int tiles = (m + TM - 1) / TM;
-  int I, J;
-  if (SUPER) {
-    if (idx >= tiles * (tiles + 1) / 2) return;  // the grid covers a size bound
-    supertile_of(idx, tiles, &I, &J);
-  } else {
-    tri_tile(idx, &I, &J);
-    if (J >= tiles) return;
-  }
   const int i0 = e + I * TM, j0 = e + J * TM;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -167,7 +163,7 @@ __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int kq = k0 + qw + RSTEP * u;
-      ru[u] = (kq < bw && iok) ? -A[(int64_t)(s + kq) * lda + i0 + cw] : T(0);
+      ru[u] = (kq < bw && iok) ? -U[(int64_t)(s + kq) * ldu + i0 + cw] : T(0);
       rw[u] = (kq < bw && jok) ? Wg[(int64_t)kq * ldw + j0 + cw] : T(0);
     }
   };
@@ -208,6 +204,25 @@ __device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
     c_tile_io<T, TM, true>(A, lda, n, i0, j0, wr, wc, lane, acc, true);
   else
     c_tile_io<T, TM, false>(A, lda, n, i0, j0, wr, wc, lane, acc, true);
+}
+
+template <typename T, int TM, int KC, bool SUPER = true>
+__device__ __forceinline__ void update_tile(T* __restrict__ A, int64_t lda,
+                                            const T* __restrict__ Wg, int64_t ldw, int s,
+                                            int bw, int n, int idx, T (*Us)[KC * (TM + 4)],
+                                            T (*Ws)[KC * (TM + 4)]) {
+  const int m = n - (s + bw);
+  if (m <= 0 || bw <= 0) return;
+  const int tiles = (m + TM - 1) / TM;
+  int I, J;
+  if (SUPER) {
+    if (idx >= tiles * (tiles + 1) / 2) return;  // the grid covers a size bound
+    supertile_of(idx, tiles, &I, &J);
+  } else {
+    tri_tile(idx, &I, &J);
+    if (J >= tiles) return;
+  }
+  update_tile_at<T, TM, KC>(A, lda, A, lda, Wg, ldw, s, bw, n, I, J, Us, Ws);
 }
 
 }  // namespace bbk

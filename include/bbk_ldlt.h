@@ -112,6 +112,30 @@ int bbk_ssytrf(bbk_handle_t handle, int n, int nb, float* d_A, int64_t lda,
 int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
                     int64_t lda, int* perm, int* piv);
 
+/* ---- out-of-core (non-GPU-resident) factorization ----------------------
+ * The reference's Factorize() (src/block_ldlt_decomposition.cc:50-59) for
+ * a matrix that stays in host memory; the reference cites this setting
+ * (README.md:24-25) but is in-core only (README.md:155-157).  A: row-major,
+ * lda, upper triangle, inside ONE pinned allocation from bbk_host_alloc
+ * that covers (n - 1) lda + n elements (other memory returns -4); it is
+ * factored in place, perm and piv are host arrays of n ints.  device_bytes
+ * bounds the HBM the call allocates for the matrix's working copies (0:
+ * the free HBM less 1 GiB); the handle's own scratch, O(nb n), is extra.
+ * While the trailing matrix is larger than that, the panel runs on the
+ * host copy across PCIe and the trailing update streams column slabs
+ * through HBM; once it fits, it moves to HBM for the remaining panels.
+ * The factors, perm and piv are bitwise those of bbk_dsytrf; the strictly
+ * lower triangle is left as it was.  Synchronous.  Returns BBK_OK, -i for
+ * an invalid argument i, BBK_ERR_UNSUPPORTED (order beyond the panel grid,
+ * or device_bytes below (nb + 1) n + 64 n elements, rounded up),
+ * BBK_ERR_ALLOC or BBK_ERR_HIP. */
+int bbk_dsytrf_ooc(bbk_handle_t handle, int n, int nb, double* A, int64_t lda, int* perm,
+                   int* piv, size_t device_bytes);
+/* Pinned host memory for bbk_dsytrf_ooc (hipHostMalloc: mapped into every
+ * GPU's address space); bbk_host_free releases it. */
+int bbk_host_alloc(void** p, size_t bytes);
+int bbk_host_free(void* p);
+
 /* ---- factorization that allocates nothing -----------------------------
  * No counterpart in the reference.  For callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
