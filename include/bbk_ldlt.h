@@ -123,12 +123,16 @@ int bbk_dsytrf_host(bbk_handle_t handle, int n, int nb, double* A,
  * the free HBM less 1 GiB); the handle's own scratch, O(nb n), is extra.
  * While the trailing matrix is larger than that, the panel runs on the
  * host copy across PCIe and the trailing update streams column slabs
- * through HBM; once it fits, it moves to HBM for the remaining panels.
- * The factors, perm and piv are bitwise those of bbk_dsytrf; the strictly
- * lower triangle is left as it was.  Synchronous.  Returns BBK_OK, -i for
- * an invalid argument i, BBK_ERR_UNSUPPORTED (order beyond the panel grid,
- * or device_bytes below (nb + 1) n + 64 n elements, rounded up),
- * BBK_ERR_ALLOC or BBK_ERR_HIP. */
+ * through HBM (copies in and out on two streams of the call's own,
+ * overlapped with the update); once it fits, it moves to HBM for the
+ * remaining panels.  The interchanges of the rows factored on the host
+ * copy are applied at the end, one gather per row.  The factors, perm and
+ * piv are bitwise those of bbk_dsytrf; the strictly lower triangle is left
+ * as it was.  Synchronous.  Returns BBK_OK, -i for an invalid argument i,
+ * BBK_ERR_UNSUPPORTED (order beyond the panel grid, or device_bytes below
+ * (nb + 1) ldd + 128 n elements, ldd = n rounded up to a multiple of 64
+ * (plus 64 when that is a multiple of 1024), each part rounded up to 256
+ * bytes), BBK_ERR_ALLOC or BBK_ERR_HIP. */
 int bbk_dsytrf_ooc(bbk_handle_t handle, int n, int nb, double* A, int64_t lda, int* perm,
                    int* piv, size_t device_bytes);
 /* Pinned host memory for bbk_dsytrf_ooc (hipHostMalloc: mapped into every

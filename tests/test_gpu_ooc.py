@@ -25,7 +25,7 @@ def _min_budget(n, nb):
 
     def r256(v):
         return (v + 255) // 256 * 256
-    return r256(8 * (nb + 1) * pitch(n)) + r256(8 * n * 64)
+    return r256(8 * (nb + 1) * pitch(n)) + 2 * r256(8 * n * 64)
 
 
 def _compare(handle, n, nb, budget, seed, zero_stride=7, extra=3):
