@@ -129,6 +129,44 @@ def bench_batched_mixed(torch, B, h, reps=2, seed=1024):
     return out
 
 
+def bench_ooc(torch, B, h, A0, n, nb=128, frac=4):
+    """The same matrix factored out of core (bbk_dsytrf_ooc): A in pinned
+    host memory, HBM for its copies bounded by 1/frac of the matrix's
+    device bytes.  Checked bitwise against an in-core factorization.
+    Algorithmic PCIe bytes: for every panel run on the host copy, the
+    trailing upper triangle in and out plus the U rows; the trailing matrix
+    in and out once it fits; the deferred gathers of the host-resident rows."""
+    import numpy as np
+    budget = 8 * n * B.padded_ld(n) // frac
+    T = A0.clone()
+    perm, piv = h.sytrf(T, nb)
+    h.synchronize()
+    ref = np.triu(T[:, :n].cpu().numpy())
+    del T
+    torch.cuda.empty_cache()
+    M = B.PinnedMatrix(n)
+    try:
+        M.array[:, :] = A0[:, :n].cpu().numpy()
+        t0 = time.perf_counter()
+        p2, v2 = h.sytrf_ooc(M, nb, budget)
+        dt = time.perf_counter() - t0
+        same = (np.array_equal(np.triu(M.array), ref) and np.array_equal(p2, perm.cpu().numpy())
+                and np.array_equal(v2, piv.cpu().numpy()))
+    finally:
+        M.close()
+    w0 = 0
+    while w0 < n and 8 * (n - w0) * B.padded_ld(n - w0) > budget:
+        w0 += nb
+    alg = sum((n - e) * (n - e + 1) / 2.0 * 16 + nb * (n - e) * 8.0
+              for e in range(nb, w0 + 1, nb))
+    alg += (n - w0) * (n - w0 + 1) / 2.0 * 16 + sum(nb * (n - e) * 16.0
+                                                    for e in range(nb, w0 + 1, nb))
+    return {"workload": f"n={n} FP64 in pinned host memory, HBM budget 1/{frac} of the matrix",
+            "device_budget_bytes": budget, "time_ms": round(dt * 1e3, 1),
+            "panels_on_host": w0 // nb, "pcie_alg_bytes": alg,
+            "pcie_alg_GB/s": round(alg / dt / 1e9, 1), "bitwise_equal_in_core": bool(same)}
+
+
 def cpu_baseline(sizes=(4096, 8192), bs=128):
     """The reference's Factorize() (oracle/_ref, OpenMP, all host cores) on a
     bounded sample: two orders, so the rate's trend with n shows (the
@@ -172,6 +210,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-ooc", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="factorizations only (for rocprofv3 runs)")
     args = ap.parse_args()
@@ -293,6 +332,11 @@ def main():
         batched = {"small_fp64_fp32": bench_batched_small(torch, B, h),
                    "mixed_refined": bench_batched_mixed(torch, B, h)}
 
+    ooc = None
+    if not args.no_ooc and not args.profile:
+        log("out-of-core factorization")
+        ooc = bench_ooc(torch, B, h, A0, n)
+
     cpu = None
     if not args.no_cpu and not args.profile:
         log("cpu baseline")
@@ -322,6 +366,7 @@ def main():
         "time_to_1e-12": solve,
         "verification": verification,
         "batched": batched,
+        "out_of_core": ooc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
