@@ -2165,7 +2165,8 @@ int batched_gen(Handle* h, int count, const int* d_n, uint64_t seed, double scal
 
 // ---- autotuner -----------------------------------------------------------
 // For every size class up to max_n: a synthetic batch with sizes spread
-// over the class (uniform [-1, 1] entries, zero diagonal on every 7th
+// over the class (in the resident range, from 16 up to the class bound;
+// uniform [-1, 1] entries, zero diagonal on every 7th
 // index to force 2x2 pivots), each candidate (panel width, team size) timed
 // with HIP events (best of two), the fastest kept in the handle's table.
 template <typename T>
@@ -2181,13 +2182,19 @@ int batched_tune(Handle* h, int max_n, int* widths_out) {
   for (int c = 0; c < kBatchClasses && rc == BBK_OK; ++c) {
     const int hi = std::min(batch_class_bound(c), std::max(max_n, 1));
     if (lo > hi) break;
-    // enough matrices to keep every CU's teams busy for small classes
-    // (concurrency decides the team size there), about 2.5e10 element
-    // operations per trial batch for the large ones
-    const double mean_n = 0.5 * (lo + hi);
+    // A batch runs with the mode of its largest matrix's class, so a class
+    // of the resident range (n <= kBatchResidentN) is tuned on batches whose
+    // largest class it is: sizes from 16 (or its own lower bound, if
+    // smaller) up to its bound, the smaller ones under their classes' tuned
+    // widths and tiles.  Enough matrices to keep every CU's teams busy for
+    // small classes (concurrency decides the team size there), about
+    // 2.5e10 element operations per trial batch for the large ones.
+    const int lo_t = hi <= kBatchResidentN ? std::min(lo, 16) : lo;
+    const double mean_n = 0.5 * (lo_t + hi);
     int count = static_cast<int>(std::min(3072.0, std::max(64.0, 2.5e10 / (mean_n * mean_n * mean_n))));
     std::vector<int> ns(count);
-    for (int i = 0; i < count; ++i) ns[i] = lo + static_cast<int>((static_cast<int64_t>(i) * 7919) % (hi - lo + 1));
+    for (int i = 0; i < count; ++i)
+      ns[i] = lo_t + static_cast<int>((static_cast<int64_t>(i) * 7919) % (hi - lo_t + 1));
     std::vector<int64_t> lds(count);
     size_t elems = 0;
     std::vector<size_t> offs(count);

@@ -57,16 +57,16 @@ struct BatchWidths {
 };
 
 // Default table (used until bbk_batched_tune or bbk_batched_set_*): what
-// the autotuner chose on MI355X for batches of n in [16, 256] (one-wavefront
-// teams below 129, MFMA 32 x 32 update tiles from 33 up in binary64, MFMA
-// tiles everywhere in binary32); the global-memory kernel with SIMT updates
-// above 256.
-constexpr BatchWidths kDefaultBatchWidthsF64 = {{32, 16, 16, 16, 32, 32, 32, 32},
-                                                {64, 64, 64, 256, 256, 256, 256, 256},
-                                                {0, 32, 32, 32, 0, 0, 0, 0}};
-constexpr BatchWidths kDefaultBatchWidthsF32 = {{16, 8, 16, 16, 32, 32, 32, 32},
-                                                {64, 64, 64, 64, 256, 256, 256, 256},
-                                                {16, 16, 32, 32, 0, 0, 0, 0}};
+// the autotuner chose on MI355X up to n = 256 (one-wavefront teams below
+// 129; batches whose largest order is above 128 in the symmetric-workspace
+// mode; MFMA 32 x 32 update tiles); the global-memory kernel with SIMT
+// updates above 256.
+constexpr BatchWidths kDefaultBatchWidthsF64 = {{16, 8, 16, 16, 32, 32, 32, 32},
+                                                {64, 64, 64, kBatchTeamSym, 256, 256, 256, 256},
+                                                {32, 32, 32, 32, 0, 0, 0, 0}};
+constexpr BatchWidths kDefaultBatchWidthsF32 = {{8, 8, 16, 24, 32, 32, 32, 32},
+                                                {64, 64, 64, kBatchTeamSym, 256, 256, 256, 256},
+                                                {32, 32, 32, 32, 0, 0, 0, 0}};
 
 // The schedule of the last prepared batch: queue order, sizes, counter.
 struct BatchPlan {
