@@ -11,6 +11,11 @@ Printed JSON line:
   roofline   dominant kernel vs its MI355X roofline (HIP events, live)
   fp64 / mixed  time to backward error <= 1e-12 for A x = b:
              FP64 factor + solve, and FP32 factor + FP64 refinement
+  batched    the 4096-matrix (n in [16, 256]) batch in both precisions and
+             the 1024-matrix (n in [256, 2048]) batched mixed solve
+  out_of_core  the same matrix factored from pinned host memory with a
+             quarter of its bytes as the HBM budget, checked bitwise
+             against the in-core factors
   cpu_baseline  the reference's own Factorize() (built from its sources,
              OpenMP on all host cores) on a bounded sample of the same
              matrix family
