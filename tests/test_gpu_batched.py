@@ -325,3 +325,44 @@ def test_batched_lockstep_fp32_and_graph(oracle, handle):
                 assert np.array_equal(v.cpu().numpy(), e.piv)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("ns", [[17, 64, 130, 200, 256, 255, 1], [600, 1031, 17, 300, 513]])
+def test_batched_factors_and_residuals_match_single_matrix_path(oracle, handle, ns):
+    """Per matrix, the batched factorization (the symmetric-workspace kernel
+    of the default tables for the first batch, lockstep launches for the
+    second) against the single-matrix
+    path bbk_dsytrf on the same input: identical pivot types and
+    permutation whenever every decision is well separated (the two differ
+    only in panel geometry and summation grouping), compact factors to 1e-9
+    relative, and device residuals within 10x of each other."""
+    import torch
+    import blockedbunchkaufmanldlt_amd as B
+    mats = handle.gen_synth_batched(ns, seed=5151 + len(ns), scale=1.0, zero_diag_stride=7,
+                                    dtype=torch.float64, pad=3)
+    srcs = [m.clone() for m in mats]
+    perms = [torch.empty(n, dtype=torch.int32, device="cuda") for n in ns]
+    pivs = [torch.empty(n, dtype=torch.int32, device="cuda") for n in ns]
+    handle.sytrf_batched(mats, perms, pivs)
+    handle.synchronize()
+    separated = 0
+    for m, src, p, v in zip(mats, srcs, perms, pivs):
+        n = m.shape[0]
+        T = torch.zeros((n, B.padded_ld(n)), dtype=torch.float64, device="cuda")
+        T[:, :n] = src[:, :n]
+        ps, vs = handle.sytrf(T)
+        handle.synchronize()
+        r_b = handle.residual(src, m, p, v)
+        r_s = handle.residual(src, T, ps, vs)
+        assert r_b < 1e-10 and r_s < 1e-10, (n, r_b, r_s)
+        assert r_b <= 10 * max(r_s, 1e-15) and r_s <= 10 * max(r_b, 1e-15), (n, r_b, r_s)
+        o = oracle.dsytrf(host_matrix(src), 64, trace=True)
+        if oracle.min_margin(o.trace) > MARGIN:
+            separated += 1
+            assert torch.equal(p, ps), n
+            assert torch.equal(v, vs), n
+            Fb = np.triu(m[:, :n].cpu().numpy())
+            Fs = np.triu(T[:, :n].cpu().numpy())
+            scale = max(1.0, np.abs(Fs).max())
+            assert np.abs(Fb - Fs).max() / scale < 1e-9, n
+    assert separated >= len(ns) // 2
