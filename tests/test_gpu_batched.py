@@ -331,11 +331,11 @@ def test_batched_lockstep_fp32_and_graph(oracle, handle):
 def test_batched_factors_and_residuals_match_single_matrix_path(oracle, handle, ns):
     """Per matrix, the batched factorization (the symmetric-workspace kernel
     of the default tables for the first batch, lockstep launches for the
-    second) against the single-matrix
-    path bbk_dsytrf on the same input: identical pivot types and
-    permutation whenever every decision is well separated (the two differ
-    only in panel geometry and summation grouping), compact factors to 1e-9
-    relative, and device residuals within 10x of each other."""
+    second) against the single-matrix path bbk_dsytrf on the same input:
+    identical pivot types and permutation whenever every decision is well
+    separated (the two differ only in panel geometry and summation
+    grouping), compact factors to 1e-9 relative, and device residuals within
+    10x of each other."""
     import torch
     import blockedbunchkaufmanldlt_amd as B
     mats = handle.gen_synth_batched(ns, seed=5151 + len(ns), scale=1.0, zero_diag_stride=7,
