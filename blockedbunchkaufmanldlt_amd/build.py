@@ -27,6 +27,7 @@ LIB = os.path.join(PKG, "libbbk_ldlt.so")
 CXXLIB = os.path.join(PKG, "libnaive_block_ldlt.so")
 DUMP = os.path.join(BUILD, "drop_in_dump")
 PROFDRV = os.path.join(BUILD, "bbk_profile_driver")
+PEAKS = os.path.join(BUILD, "bbk_peaks")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -81,6 +82,8 @@ def build(verbose: bool = False) -> str:
         _run([HIPCC, "--offload-arch=gfx950", "-O2", "-I" + os.path.join(ROOT, "include"),
               "-o", PROFDRV, os.path.join(PKG, "tools", "bbk_profile_driver.cc"),
               "-L" + PKG, "-lbbk_ldlt", "-Wl,-rpath,$ORIGIN/.."])
+    # achievable peaks of the device for the benchmark (tools/bbk_peaks.hip)
+    _run([HIPCC] + HIP_FLAGS + ["-o", PEAKS, os.path.join(PKG, "tools", "bbk_peaks.hip")])
     return LIB
 
 

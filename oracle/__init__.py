@@ -99,6 +99,9 @@ def ref():
         R.ref_time_factorize.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                          _dp, _ip, _ip]
         R.ref_time_factorize.restype = ctypes.c_int
+        R.ref_time_batch.argtypes = [ctypes.c_int, _ip, _dp, ctypes.POINTER(ctypes.c_longlong),
+                                     ctypes.c_int, _dp]
+        R.ref_time_batch.restype = ctypes.c_int
         R.ref_kernel_scale_1x1.argtypes = [ctypes.c_int, ctypes.c_double, _dp,
                                            ctypes.c_int, _dp, ctypes.c_int]
         R.ref_kernel_scale_2x2.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + \
@@ -206,6 +209,28 @@ def ref_time_factorize(A: np.ndarray, bs: int, want_factors: bool = False):
         raise RuntimeError("reference threw")
     if want_factors:
         return float(sec[0]), Factorization(n, bs, compact, perm, piv)
+    return float(sec[0])
+
+
+def ref_time_batch(mats, bs: int) -> float:
+    """Seconds of the reference's Factorize() over a list of symmetric
+    matrices, an outer OpenMP loop over the matrices with each factorization
+    on one thread (batched CPU baseline)."""
+    ns = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    offs = np.zeros(len(mats), dtype=np.int64)
+    total = 0
+    for i, m in enumerate(mats):
+        offs[i] = total
+        total += m.shape[0] * m.shape[0]
+    data = np.empty(max(total, 1), dtype=np.float64)
+    for m, o in zip(mats, offs):
+        data[o:o + m.size] = np.ascontiguousarray(m, dtype=np.float64).ravel()
+    sec = np.zeros(1)
+    fails = ref().ref_time_batch(len(mats), _i(ns), _d(data),
+                                 offs.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), bs,
+                                 _d(sec))
+    if fails:
+        raise RuntimeError(f"reference threw on {fails} matrices")
     return float(sec[0])
 
 

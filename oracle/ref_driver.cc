@@ -4,9 +4,12 @@
 // oracle against the real reference and lets the CPU baseline time the
 // reference's Factorize().  It calls only the reference's public API
 // (include/*.h under the reference tree).
+#include <omp.h>
+
 #include <chrono>
 #include <cstddef>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <vector>
 
@@ -36,6 +39,22 @@ void ExportGeometry(const BlockMatrix& m, int* geom) {
       g[2] = b.rows();
       g[3] = b.cols();
       g[4] = b.leading_dimension();
+    }
+}
+
+// The block-fill ingest of tests/block_ldlt_large_stress_test.cc:116-136:
+// A (n*n row-major, upper triangle read) straight into the blocks, skipping
+// the O(n^2 nb^2) element-wise constructor.
+void FillBlocks(BlockMatrix& bm, const double* A, int n) {
+  for (int J = 0; J < bm.num_blocks(); ++J)
+    for (int I = 0; I <= J; ++I) {
+      Block& b = bm.GetBlock(I, J);
+      for (int r = 0; r < b.rows(); ++r)
+        for (int c = 0; c < b.cols(); ++c) {
+          const int gr = b.row0() + r, gc = b.col0() + c;
+          if (gr > gc) continue;
+          b(r, c) = A[static_cast<std::size_t>(gr) * n + gc];
+        }
     }
 }
 
@@ -108,16 +127,7 @@ int ref_time_factorize(int n, int bs, const double* A, double* seconds,
                        double* compact, int* perm, int* piv) {
   try {
     BlockMatrix bm(n, bs);
-    for (int J = 0; J < bm.num_blocks(); ++J)
-      for (int I = 0; I <= J; ++I) {
-        Block& b = bm.GetBlock(I, J);
-        for (int r = 0; r < b.rows(); ++r)
-          for (int c = 0; c < b.cols(); ++c) {
-            const int gr = b.row0() + r, gc = b.col0() + c;
-            if (gr > gc) continue;
-            b(r, c) = A[static_cast<std::size_t>(gr) * n + gc];
-          }
-      }
+    FillBlocks(bm, A, n);
     BlockLdltDecomposition dec(
         bm, BlockLdltDecomposition::FactorizationMode::kCompactOnly);
     const auto t0 = std::chrono::steady_clock::now();
@@ -137,6 +147,39 @@ int ref_time_factorize(int n, int bs, const double* A, double* seconds,
   } catch (const std::exception&) {
     return 1;
   }
+}
+
+// The batched CPU baseline: every matrix's block storage is built first
+// (block-fill ingest), then Factorize() (kCompactOnly) runs under an outer
+// `omp parallel for schedule(dynamic)` over the matrices with nested
+// parallel regions serialized, so each factorization runs on one thread;
+// only that loop is timed.  Matrix i: ns[i] x ns[i] row-major at
+// data + offs[i].  Returns the number of factorizations that threw.
+int ref_time_batch(int count, const int* ns, const double* data, const long long* offs, int bs,
+                   double* seconds) {
+  std::vector<std::unique_ptr<BlockLdltDecomposition>> decs(count);
+  for (int i = 0; i < count; ++i) {
+    BlockMatrix bm(ns[i], bs);
+    FillBlocks(bm, data + offs[i], ns[i]);
+    decs[i] = std::make_unique<BlockLdltDecomposition>(
+        bm, BlockLdltDecomposition::FactorizationMode::kCompactOnly);
+  }
+  const int levels = omp_get_max_active_levels();
+  omp_set_max_active_levels(1);
+  int failures = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic) reduction(+ : failures)
+  for (int i = 0; i < count; ++i) {
+    try {
+      decs[i]->Factorize();
+    } catch (const std::exception&) {
+      failures += 1;
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  omp_set_max_active_levels(levels);
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return failures;
 }
 
 void ref_kernel_scale_1x1(int count, double pivot, const double* s, int ss,
