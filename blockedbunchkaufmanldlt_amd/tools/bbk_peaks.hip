@@ -25,20 +25,44 @@ namespace {
     }                                                                 \
   } while (0)
 
-// U float4 loads of a thread in flight before its stores.
-template <int U>
-__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a,
-                                              float4* __restrict__ b, size_t count) {
+// U float4 loads of a thread in flight before its stores; NT: the stores
+// non-temporal.
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_copy(const v4f_t* __restrict__ a,
+                                              v4f_t* __restrict__ b, size_t count) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i + (U - 1) * stride < count; i += U * stride) {
-    float4 v[U];
+    v4f_t v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) v[u] = a[i + u * stride];
 #pragma unroll
-    for (int u = 0; u < U; ++u) b[i + u * stride] = v[u];
+    for (int u = 0; u < U; ++u) {
+      if (NT)
+        __builtin_nontemporal_store(v[u], &b[i + u * stride]);
+      else
+        b[i + u * stride] = v[u];
+    }
   }
   for (; i < count; i += stride) b[i] = a[i];
+}
+
+// A workgroup copies U * 256 consecutive elements per step (its U loads
+// per thread adjacent in memory), steps strided over the grid.
+template <int U>
+__global__ __launch_bounds__(256) void k_copy_blocked(const v4f_t* __restrict__ a,
+                                                      v4f_t* __restrict__ b, size_t count) {
+  const size_t step = (size_t)U * 256;
+  for (size_t base = (size_t)blockIdx.x * step; base + step <= count;
+       base += (size_t)gridDim.x * step) {
+    v4f_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = a[base + (size_t)u * 256 + threadIdx.x];
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[base + (size_t)u * 256 + threadIdx.x] = v[u];
+  }
 }
 
 // CH independent accumulator chains per wavefront.
@@ -132,27 +156,36 @@ int main() {
   CHECK(hipGetDevice(&dev));
   CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 
-  // HBM: copy between two 2 GiB buffers; 1 or 4 loads in flight per
-  // thread, 16 or 64 workgroups per CU
+  // HBM: copy between two 2 GiB buffers; 1, 4 or 8 loads in flight per
+  // thread, grid-strided or workgroup-contiguous, plain or non-temporal
+  // stores, 8, 16 or 64 workgroups per CU (count is a multiple of every
+  // workgroup step, so the blocked form copies everything)
   const size_t bytes = size_t(2) << 30;
-  const size_t count = bytes / sizeof(float4);
-  float4 *a = nullptr, *b = nullptr;
+  const size_t count = bytes / sizeof(v4f_t);
+  v4f_t *a = nullptr, *b = nullptr;
   CHECK(hipMalloc(&a, bytes));
   CHECK(hipMalloc(&b, bytes));
   CHECK(hipMemset(a, 0, bytes));
   float copy_ms = 1e30f;
-  for (int per_cu : {16, 64}) {
+  for (int per_cu : {8, 16, 64}) {
     const int grid = cus * per_cu;
-    float t1 = 0.f, t4 = 0.f;
-    if (best_ms([&] { hipLaunchKernelGGL(k_copy<1>, dim3(grid), dim3(256), 0, nullptr, a, b,
-                                         count); },
-                &t1))
+    float t[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (best_ms([&] { hipLaunchKernelGGL((k_copy<1, false>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[0]) ||
+        best_ms([&] { hipLaunchKernelGGL((k_copy<4, false>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[1]) ||
+        best_ms([&] { hipLaunchKernelGGL((k_copy<8, false>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[2]) ||
+        best_ms([&] { hipLaunchKernelGGL((k_copy<4, true>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[3]) ||
+        best_ms([&] { hipLaunchKernelGGL((k_copy<8, true>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[4]) ||
+        best_ms([&] { hipLaunchKernelGGL((k_copy_blocked<4>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[5]) ||
+        best_ms([&] { hipLaunchKernelGGL((k_copy_blocked<8>), dim3(grid), dim3(256), 0, nullptr,
+                                         a, b, count); }, &t[6]))
       return 1;
-    if (best_ms([&] { hipLaunchKernelGGL(k_copy<4>, dim3(grid), dim3(256), 0, nullptr, a, b,
-                                         count); },
-                &t4))
-      return 1;
-    copy_ms = std::min(copy_ms, std::min(t1, t4));
+    for (float v : t) copy_ms = std::min(copy_ms, v);
   }
   CHECK(hipFree(a));
   CHECK(hipFree(b));
