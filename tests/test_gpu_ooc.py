@@ -117,3 +117,15 @@ def test_ooc_refuses_unpinned_memory_and_small_budgets(handle):
     finally:
         M.close()
     del ctypes
+
+
+def test_ooc_with_the_wide_panel_variant(handle):
+    """The 512-column panel workgroups (the variant orders past 65536 use),
+    forced on both paths: the out-of-core factors stay bitwise the in-core
+    ones."""
+    handle.set_panel_threads(512)
+    try:
+        twos = _compare(handle, 3000, 64, 8 * 3000 * 3000 // 3, 6464)
+    finally:
+        handle.set_panel_threads(0)
+    assert twos > 0
