@@ -5,7 +5,9 @@
 //   fp64_mfma  v_mfma_f64_16x16x4f64, independent accumulator chains per
 //              wavefront;
 //   fp32_mfma  the faster of v_mfma_f32_16x16x4f32 and
-//              v_mfma_f32_32x32x2f32.
+//              v_mfma_f32_32x32x2f32;
+//   fp64_valu  v_fma_f64, independent chains per lane (the vector unit's
+//              binary64 rate, which the panel's row products use).
 // Each over a few launch shapes (loads in flight, workgroups per CU,
 // chains per wavefront), the best of five timed launches of each (HIP
 // events, after one untimed launch); the best shape is reported.
@@ -81,6 +83,23 @@ __global__ __launch_bounds__(256) void k_mfma_f64(int iters, double* out) {
   double s = 0.0;
 #pragma unroll
   for (int c = 0; c < CH; ++c) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// VF independent v_fma_f64 chains per lane.
+template <int VF>
+__global__ __launch_bounds__(256) void k_valu_f64(int iters, double* out) {
+  double x[VF];
+  const double a = 1.0 + 1e-9 * threadIdx.x, b = 1e-9 * blockIdx.x;
+#pragma unroll
+  for (int c = 0; c < VF; ++c) x[c] = 1e-3 * c;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int c = 0; c < VF; ++c) x[c] = __builtin_fma(x[c], a, b);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < VF; ++c) s += x[c];
   out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
@@ -194,7 +213,7 @@ int main() {
   // chains per wavefront; the best rate of each instruction
   void* out = nullptr;
   CHECK(hipMalloc(&out, (size_t)cus * 8 * 256 * sizeof(double)));
-  double f64 = 0.0, f32a = 0.0, f32b = 0.0;
+  double f64 = 0.0, f32a = 0.0, f32b = 0.0, v64 = 0.0;
   for (int per_cu : {4, 8}) {
     const int grid = cus * per_cu, threads = 256;
     const double waves = (double)grid * threads / 64.0;
@@ -217,13 +236,19 @@ int main() {
                                          it32, static_cast<float*>(out)); }, &t))
       return 1;
     f32b = std::max(f32b, waves * it32 * 4 * 4096.0 / (t * 1e-3) / 1e12);
+    // about 1.7e11 flop per launch
+    const int itv = 40000 / per_cu;
+    if (best_ms([&] { hipLaunchKernelGGL(k_valu_f64<8>, dim3(grid), dim3(threads), 0, nullptr,
+                                         itv, static_cast<double*>(out)); }, &t))
+      return 1;
+    v64 = std::max(v64, (double)grid * threads * itv * 8 * 2.0 / (t * 1e-3) / 1e12);
   }
   CHECK(hipFree(out));
 
   const double hbm = 2.0 * (double)bytes / (copy_ms * 1e-3) / 1e9;
   std::printf("{\"hbm_copy_GB/s\": %.1f, \"fp64_mfma_TFLOP/s\": %.2f, "
               "\"fp32_mfma_16x16x4_TFLOP/s\": %.2f, \"fp32_mfma_32x32x2_TFLOP/s\": %.2f, "
-              "\"fp32_mfma_TFLOP/s\": %.2f, \"cus\": %d}\n",
-              hbm, f64, f32a, f32b, std::max(f32a, f32b), cus);
+              "\"fp32_mfma_TFLOP/s\": %.2f, \"fp64_valu_TFLOP/s\": %.2f, \"cus\": %d}\n",
+              hbm, f64, f32a, f32b, std::max(f32a, f32b), v64, cus);
   return 0;
 }
