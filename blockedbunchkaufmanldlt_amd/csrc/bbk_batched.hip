@@ -1455,31 +1455,66 @@ __device__ void sym_update(gmem_t<T>* S, int n, const int* pi, int s, int e, con
   }
 }
 
-// S := the symmetric matrix whose upper triangle is A's (four rows in
-// flight per pass).
+// S := the symmetric matrix whose upper triangle is A's, R rows at a time
+// (R in {4, 8, 16}, four rows in flight per pass).  Each row goes to S as
+// it is and to a copy in LDS (stage, R x (n + 1) elements, a stride that
+// keeps the transposed reads off one bank); the strictly lower half is
+// then written from the copy, R consecutive elements of an S row per
+// lane group, instead of one element per row.  R = 0: no LDS, each
+// element's mirror stored on its own.
 template <typename T>
-__device__ void sym_load(const gmem_t<T>* A, int64_t lda, int n, gmem_t<T>* S) {
+__device__ void sym_load(const gmem_t<T>* A, int64_t lda, int n, gmem_t<T>* S, T* stage,
+                         int R) {
   const int lane = threadIdx.x & 63;
-  for (int i0 = 0; i0 < n; i0 += 4) {
-    T v[4][kSymNP];
+  const int ld = n + 1;
+  const int rows = R > 0 ? R : 4;
+  for (int i0 = 0; i0 < n; i0 += rows) {
+    for (int d0 = 0; d0 < rows; d0 += 4) {
+      T v[4][kSymNP];
 #pragma unroll
-    for (int d = 0; d < 4; ++d)
+      for (int d = 0; d < 4; ++d)
 #pragma unroll
-      for (int u = 0; u < kSymNP; ++u) {
-        const int i = min(i0 + d, n - 1), j = min(i + lane + 64 * u, n - 1);
-        v[d][u] = A[i * lda + static_cast<unsigned>(j)];
-      }
+        for (int u = 0; u < kSymNP; ++u) {
+          const int i = min(i0 + d0 + d, n - 1), j = min(i + lane + 64 * u, n - 1);
+          v[d][u] = A[i * lda + static_cast<unsigned>(j)];
+        }
 #pragma unroll
-    for (int d = 0; d < 4; ++d)
+      for (int d = 0; d < 4; ++d)
 #pragma unroll
-      for (int u = 0; u < kSymNP; ++u) {
-        const int i = i0 + d, j = i + lane + 64 * u;
-        if (i < n && j < n) {
-          S[static_cast<unsigned>(i * n + j)] = v[d][u];
-          S[static_cast<unsigned>(j * n + i)] = v[d][u];
+        for (int u = 0; u < kSymNP; ++u) {
+          const int i = i0 + d0 + d, j = i + lane + 64 * u;
+          if (i < n && j < n) {
+            S[static_cast<unsigned>(i * n + j)] = v[d][u];
+            if (R > 0)
+              stage[(d0 + d) * ld + j] = v[d][u];
+            else
+              S[static_cast<unsigned>(j * n + i)] = v[d][u];
+          }
+        }
+    }
+    if (R > 0) {
+      __syncthreads();
+      // S(j, i) for i in [i0, i0 + R), i < j < n: lane group jr takes row
+      // j, lane ic column i0 + ic
+      const int per = 64 / R, ic = lane & (R - 1), jr = lane / R;
+      for (int j0 = i0 + 1; j0 < n; j0 += 4 * per) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int j = j0 + t * per + jr, i = i0 + ic;
+          if (j < n && i < j) S[static_cast<unsigned>(j * n + i)] = stage[ic * ld + j];
         }
       }
+      __syncthreads();
+    }
   }
+}
+
+// Rows per pass of sym_load for order n when `elems` elements of LDS are
+// free: the largest of 16, 8, 4 whose R x (n + 1) copy fits, else 0.
+__device__ __forceinline__ int sym_load_rows(int n, int elems) {
+  for (int R = 16; R >= 4; R >>= 1)
+    if (R * (n + 1) <= elems) return R;
+  return 0;
 }
 
 // A(k, j) := S(pi[k], pi[j]) for k <= j (the factors in the final logical
@@ -1548,7 +1583,8 @@ __global__ __launch_bounds__(64, sizeof(T) == 8 ? 1 : 2) void k_batched_sym(Batc
     int pj[kSymNP];
 #pragma unroll
     for (int u = 0; u < kSymNP; ++u) pj[u] = min(lane + 64 * u, n - 1);
-    sym_load<T>(A, lda, n, S);
+    // the W rows' LDS ((bw + 1) x n elements) is free until the first panel
+    sym_load<T>(A, lda, n, S, W, sym_load_rows(n, (bw + 1) * n));
     const int ts = a.widths.ts[c] == 32 ? 32 : 16;
     int s = 0;
     for (int P = 0; s < n; ++P) {
