@@ -144,7 +144,8 @@ __device__ __forceinline__ T panel_rows_dot(const T* l, const T* W, int n, int k
 // flight (W in global memory is latency-bound one position at a time).
 template <typename T, bool ACC4, int PB, int NT>
 __device__ __forceinline__ void panel_rows_dot_multi(const T* l, const T* W, int n, int kk,
-                                                     int j0, T (&out)[PB]) {
+                                                     int j0, T (&out)[PB],
+                                                     const T* Wl = nullptr, int RL = 0) {
   int jj[PB];
 #pragma unroll
   for (int u = 0; u < PB; ++u) jj[u] = (j0 + u * NT < n) ? j0 + u * NT : j0;  // j0 < n
@@ -166,6 +167,22 @@ __device__ __forceinline__ void panel_rows_dot_multi(const T* l, const T* W, int
 #pragma unroll
   for (int u = 0; u < PB; ++u) a0[u] = a1[u] = a2[u] = a3[u] = T(0);
   int p = 0;
+  // rows [0, RL) (RL a multiple of 4) from the copy in LDS: the same
+  // values, summed in the same order
+  const int kl = min(kk, RL);
+  for (; p + 4 <= kl; p += 4) {
+    const T* r0 = Wl + p * n;
+    const T* r1 = r0 + n;
+    const T* r2 = r1 + n;
+    const T* r3 = r2 + n;
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      a0[u] += l[p] * r0[jj[u]];
+      a1[u] += l[p + 1] * r1[jj[u]];
+      a2[u] += l[p + 2] * r2[jj[u]];
+      a3[u] += l[p + 3] * r3[jj[u]];
+    }
+  }
   for (; p + 4 <= kk; p += 4) {
     const T* r0 = W + (int64_t)p * n;
     const T* r1 = r0 + n;
@@ -286,12 +303,16 @@ struct HybridMat {
 // row and the candidate row live in LDS (rowk, rowc: n elements each,
 // indexed by column) instead of W's rows kk and bw + 1, and each finished
 // row is copied to W after its scaling; for W in global memory this takes
-// the row reads between the pivot's phases off global memory.
+// the row reads between the pivot's phases off global memory.  With LROW,
+// Wl (LDS, RL rows of n, RL a multiple of 4; ACC4 only) also holds W's rows
+// [0, RL) once finished, interchanges applied to both copies, and the row
+// products read those rows from it.
 template <typename T, int NT, bool ACC4, int PB = 1, bool LROW = false, typename Mat>
 __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
                             int* __restrict__ piv, int bw, int P, int s, T* W, T* Lk, T* Lr,
                             ScaleTab& st, double* rv, int* ri, T* rowk = nullptr,
-                            T* rowc = nullptr) {
+                            T* rowc = nullptr, T* Wl = nullptr, int RL = 0) {
+  static_assert(ACC4 || !LROW, "the LDS copy of W's rows is read by the four-sum products");
   const int tid = threadIdx.x;
   T* C = LROW ? rowc : W + (int64_t)(bw + 1) * n;
   int e = min((P + 1) * bw, n);  // nominal end; s may sit one column early
@@ -308,7 +329,7 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
       T fk[PB], dot[PB];
 #pragma unroll
       for (int u = 0; u < PB; ++u) fk[u] = (j0 + u * NT < n) ? F.at(k, j0 + u * NT) : T(0);
-      panel_rows_dot_multi<T, ACC4, PB, NT>(Lk, W, n, kk, j0, dot);
+      panel_rows_dot_multi<T, ACC4, PB, NT>(Lk, W, n, kk, j0, dot, Wl, RL);
 #pragma unroll
       for (int u = 0; u < PB; ++u) {
         const int j = j0 + u * NT;
@@ -342,7 +363,7 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
           const int i = i0 + u * NT;
           base[u] = i < n ? (i < r ? F.at(i, r) : F.at(r, i)) : T(0);
         }
-        panel_rows_dot_multi<T, ACC4, PB, NT>(Lr, W, n, kk, i0, dot);
+        panel_rows_dot_multi<T, ACC4, PB, NT>(Lr, W, n, kk, i0, dot, Wl, RL);
 #pragma unroll
         for (int u = 0; u < PB; ++u) {
           const int i = i0 + u * NT;
@@ -378,6 +399,7 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
         if (i0 > 0 && tid == 0) F.defer(k, r);
         for (int p = tid; p < kk; p += NT)
           swap_elems(&W[(int64_t)p * n + k], &W[(int64_t)p * n + r]);
+        for (int p = tid; p < min(kk, RL); p += NT) swap_elems(&Wl[p * n + k], &Wl[p * n + r]);
         for (int j = k + 1 + tid; j < n; j += NT)
           if (j != r) Wk[j] = C[j];
         if (tid == 0) {
@@ -400,6 +422,8 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
           if (i0 > 0 && tid == 0) F.defer(k1, r);
           for (int p = tid; p < kk; p += NT)
             swap_elems(&W[(int64_t)p * n + k1], &W[(int64_t)p * n + r]);
+          for (int p = tid; p < min(kk, RL); p += NT)
+            swap_elems(&Wl[p * n + k1], &Wl[p * n + r]);
         }
         if constexpr (!LROW) {
           for (int j = k + 2 + tid; j < n; j += NT)
@@ -439,6 +463,8 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
       if constexpr (LROW) {
         T* g = W + (int64_t)kk * n;
         for (int j = k + tid; j < n; j += NT) g[j] = Wk[j];
+        if (kk < RL)
+          for (int j = k + tid; j < n; j += NT) Wl[kk * n + j] = Wk[j];
       }
       kk += 1;
     } else {
@@ -477,6 +503,10 @@ __device__ int factor_panel(const Mat& F, int n, int* __restrict__ perm,
           g0[j] = Wk[j];
           g1[j] = Wk1[j];
         }
+        if (kk < RL)
+          for (int j = k + tid; j < n; j += NT) Wl[kk * n + j] = Wk[j];
+        if (kk + 1 < RL)
+          for (int j = k + tid; j < n; j += NT) Wl[(kk + 1) * n + j] = Wk1[j];
       }
       kk += 2;
     }
@@ -1628,6 +1658,7 @@ struct StepArgs {
   int* st;
   const int* prefix;     // this step's first tile per matrix (count + 1 entries)
   BatchWidths widths;
+  int lds_dyn;           // dynamic LDS bytes of k_step_panel<T, true>
 };
 
 // Rows [0, R) of A: row[dst[t]] = row[src[t]] for t < ne (one composed
@@ -1712,20 +1743,21 @@ constexpr bool kStepDefer = sizeof(T) == 4;
 template <typename T, bool LROW>
 __device__ int step_panel(T* A, int64_t lda, int n, int* perm, int* piv, int bw, int P, int s,
                           T* W, T* Lk, T* Lr, ScaleTab& st, double* rv, int* ri, T* rowk,
-                          T* rowc) {
+                          T* rowc, T* Wl, int RL) {
   if constexpr (kStepDefer<T>) {
     __shared__ DeferScratch ds;
     if (threadIdx.x == 0) ds.count = 0;
     __syncthreads();
     const int e = factor_panel<T, 256, true, kStepPanelBatch<T>, LROW>(
         DeferGlobalMat<T>{A, lda, &ds}, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri, rowk,
-        rowc);
+        rowc, Wl, RL);
     __syncthreads();
     apply_panel_deferred<T, 256>(A, lda, s, ds);
     return e;
   } else {
     return factor_panel<T, 256, true, kStepPanelBatch<T>, LROW>(
-        GlobalMat<T>{A, lda}, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri, rowk, rowc);
+        GlobalMat<T>{A, lda}, n, perm, piv, bw, P, s, W, Lk, Lr, st, rv, ri, rowk, rowc, Wl,
+        RL);
   }
 }
 
@@ -1760,11 +1792,18 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   }
   const int s = P == 0 ? 0 : S[0];
   T* W = a.ws + a.wsoff[m];
-  // LROW: the pivot row and the candidate row in dynamic LDS (n each)
+  // LROW: the pivot row and the candidate row in dynamic LDS (n each),
+  // then W's first RL rows (a multiple of 4, at most bw) in what is left
   T* rowk = reinterpret_cast<T*>(bbk_batched_lds);
   T* rowc = rowk + n;
+  T* Wl = rowc + n;
+  int RL = 0;
+  if (LROW) {
+    const int spare = a.lds_dyn / static_cast<int>(sizeof(T)) - 2 * n;
+    RL = spare > 0 ? min(bw, spare / n) & ~3 : 0;
+  }
   const int e = step_panel<T, LROW>(a.A[m], a.lda[m], n, perm, piv, bw, P, s, W, Lk, Lr, stab,
-                                    rv, ri, rowk, rowc);
+                                    rv, ri, rowk, rowc, Wl, RL);
   if (threadIdx.x == 0) {
     S[0] = e;
     S[1] = s;
@@ -1892,6 +1931,39 @@ static int batched_grid(Handle* h, int count, int elem_bytes, int nt) {
 // Lockstep schedule: per matrix a W slice and 4 state ints; per panel step
 // the prefix sums of the matrices' trailing-tile counts (a bound from the
 // nominal panel end: a deferred 2x2 ends a panel at most one column early).
+// Dynamic LDS of k_step_panel<T, true> that keeps as many workgroups on a
+// CU as its registers allow (160 KB of LDS per CU shared among them, less
+// the kernel's static LDS); the kernel's limit is raised to it once.
+static int step_panel_lds_cap(int elem_bytes) {
+  auto cap_of = [](const void* k) {
+    hipFuncAttributes fa;
+    size_t stat = 16 * 1024;
+    if (hipFuncGetAttributes(&fa, k) == hipSuccess)
+      stat = fa.sharedSizeBytes;
+    else
+      (void)hipGetLastError();
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess ||
+        per_cu < 1) {
+      (void)hipGetLastError();
+      per_cu = 4;
+    }
+    const size_t share = 160 * 1024 / static_cast<size_t>(per_cu);
+    const int cap =
+        share > stat + 1024 ? static_cast<int>((share - stat - 1024) / 1024 * 1024) : 0;
+    // past 64 KB a launch needs the raised limit
+    if (cap > 64 * 1024 &&
+        hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, cap) != hipSuccess) {
+      (void)hipGetLastError();
+      return 64 * 1024;
+    }
+    return cap;
+  };
+  static const int cap64 = cap_of(reinterpret_cast<const void*>(&k_step_panel<double, true>));
+  static const int cap32 = cap_of(reinterpret_cast<const void*>(&k_step_panel<float, true>));
+  return elem_bytes == 8 ? cap64 : cap32;
+}
+
 static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes,
                             const BatchWidths& wt) {
   BatchPlan& p = h->batch;
@@ -1950,6 +2022,18 @@ static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes
   p.nt = 256;
   p.resident = false;
   p.sym = false;
+  // the panel's pivot and candidate rows in LDS when both fit
+  // kStepRowLdsBytes; beyond them, as far as the workgroups its registers
+  // allow per CU still fit, copies of the first W rows
+  int nmax = 0;
+  for (int i = 0; i < count; ++i) nmax = std::max(nmax, h_n[i]);
+  const size_t row_lds = 2 * static_cast<size_t>(nmax) * elem_bytes;
+  p.lds_dyn = 0;
+  if (nmax > 0 && row_lds <= kStepRowLdsBytes) {
+    const size_t want = row_lds + static_cast<size_t>(kBatchMaxWidth) * nmax * elem_bytes;
+    const size_t cap = static_cast<size_t>(step_panel_lds_cap(elem_bytes));
+    p.lds_dyn = static_cast<int>(std::max(row_lds, std::min(want, cap)));
+  }
   return BBK_OK;
 }
 
@@ -2102,17 +2186,14 @@ static int lockstep_launch(Handle* h, const BatchPlan& p, T* const* A, const int
   a.st = base + p.count;
   a.widths = p.widths;
   const int* prefix_all = base + 5 * (size_t)p.count;
-  // the pivot and candidate rows in LDS when both fit kStepRowLdsBytes
-  int nmax = 0;
-  for (int v : p.sizes) nmax = std::max(nmax, v);
-  const size_t row_lds = 2 * static_cast<size_t>(nmax) * sizeof(T);
-  const bool lrow = row_lds <= kStepRowLdsBytes;
+  // the pivot and candidate rows (and W row copies) in LDS: p.lds_dyn > 0
+  a.lds_dyn = p.lds_dyn;
   for (int P = 0; P < p.steps; ++P) {
     a.step = P;
     a.prefix = prefix_all + (size_t)P * p.step_stride;
-    if (lrow)
-      hipLaunchKernelGGL((k_step_panel<T, true>), dim3(p.count), dim3(256), row_lds, h->stream,
-                         a);
+    if (p.lds_dyn > 0)
+      hipLaunchKernelGGL((k_step_panel<T, true>), dim3(p.count), dim3(256), p.lds_dyn,
+                         h->stream, a);
     else
       hipLaunchKernelGGL((k_step_panel<T, false>), dim3(p.count), dim3(256), 0, h->stream, a);
     const int tiles = p.step_tiles[P];

@@ -84,7 +84,7 @@ struct BatchPlan {
   bool lockstep = false;
   bool resident = false;  // matrices held in LDS (largest n <= kBatchResidentN)
   bool sym = false;       // symmetric-workspace kernel (largest n <= kBatchResidentN)
-  int lds_dyn = 0;        // sym: dynamic LDS bytes per workgroup
+  int lds_dyn = 0;        // dynamic LDS bytes per workgroup: sym kernel or lockstep panel
   int steps = 0;        // lockstep: panel steps (largest ceil(n / width))
   int64_t step_stride = 0;  // lockstep: ints per step in the tile prefix table
   int64_t ws_elems = 0;
