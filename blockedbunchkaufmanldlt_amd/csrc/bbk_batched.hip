@@ -1659,6 +1659,7 @@ struct StepArgs {
   const int* prefix;     // this step's first tile per matrix (count + 1 entries)
   BatchWidths widths;
   int lds_dyn;           // dynamic LDS bytes of k_step_panel<T, true>
+  const int* order;      // panel workgroup -> matrix (largest first)
 };
 
 // Rows [0, R) of A: row[dst[t]] = row[src[t]] for t < ne (one composed
@@ -1768,7 +1769,7 @@ __global__ __launch_bounds__(256) void k_step_panel(StepArgs<T> a) {
   __shared__ ScaleTab stab;
   __shared__ double rv[4];
   __shared__ int ri[4];
-  const int m = blockIdx.x;
+  const int m = a.order[blockIdx.x];
   const int n = a.ns[m];
   int* S = a.st + 4 * m;
   if (n <= 0) {
@@ -1992,8 +1993,9 @@ static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes
     }
     pr[count] = acc;
   }
-  // buf: n[count], state[4 count], prefix[steps * (count + 1)], wsoff (int64)
-  const size_t ints = count + 4 * (size_t)count + prefix.size() + 64;
+  // buf: n[count], state[4 count], prefix[steps * (count + 1)],
+  // order[count], wsoff (int64)
+  const size_t ints = count + 4 * (size_t)count + prefix.size() + count + 64;
   const size_t bytes = ints * sizeof(int) + count * sizeof(int64_t) + 256;
   int rc = h->reserve_buf(&p.buf, &p.buf_bytes, bytes, "batch schedule");
   if (rc != BBK_OK) return rc;
@@ -2003,6 +2005,12 @@ static int prepare_lockstep(Handle* h, int count, const int* h_n, int elem_bytes
   int* hi = reinterpret_cast<int*>(host.data());
   std::copy(h_n, h_n + count, hi);
   std::copy(prefix.begin(), prefix.end(), hi + 5 * (size_t)count);
+  // panel workgroups take the matrices largest first: a step lasts until
+  // its slowest resident workgroups finish, and a batch larger than the
+  // resident grid balances best when the long panels start first
+  int* ord = hi + 5 * (size_t)count + prefix.size();
+  std::iota(ord, ord + count, 0);
+  std::stable_sort(ord, ord + count, [&](int x, int y) { return h_n[x] > h_n[y]; });
   const size_t off64 = ((ints * sizeof(int)) + 255) / 256 * 256;
   std::memcpy(host.data() + off64, wsoff.data(), count * sizeof(int64_t));
   hipError_t e = hipMemcpyAsync(p.buf, host.data(), bytes, hipMemcpyHostToDevice, h->stream);
@@ -2172,7 +2180,8 @@ template <typename T>
 static int lockstep_launch(Handle* h, const BatchPlan& p, T* const* A, const int64_t* lda,
                            int* const* perm, int* const* piv) {
   int* base = reinterpret_cast<int*>(p.buf);
-  const size_t ints = p.count + 4 * (size_t)p.count + (size_t)p.steps * p.step_stride + 64;
+  const size_t ints =
+      p.count + 4 * (size_t)p.count + (size_t)p.steps * p.step_stride + p.count + 64;
   StepArgs<T> a;
   a.count = p.count;
   a.ns = base;
@@ -2186,6 +2195,7 @@ static int lockstep_launch(Handle* h, const BatchPlan& p, T* const* A, const int
   a.st = base + p.count;
   a.widths = p.widths;
   const int* prefix_all = base + 5 * (size_t)p.count;
+  a.order = prefix_all + (size_t)p.steps * p.step_stride;
   // the pivot and candidate rows (and W row copies) in LDS: p.lds_dyn > 0
   a.lds_dyn = p.lds_dyn;
   for (int P = 0; P < p.steps; ++P) {
