@@ -1817,7 +1817,12 @@ __global__ __launch_bounds__(256, 4) void k_step_update(StepArgs<T> a) {
   constexpr int TM = 64, KC = 8;
   __shared__ T Us[2][KC * (TM + 4)];
   __shared__ T Ws[2][KC * (TM + 4)];
-  const int b = blockIdx.x;
+  // workgroups are dealt to the 8 XCDs in turn (blockIdx mod 8); each XCD
+  // takes one contiguous range of tiles instead, so the tiles of a matrix
+  // share its U and W rows in one L2 (a bijection onto [0, gridDim.x))
+  const int G = gridDim.x, q = G >> 3, r = G & 7;
+  const int xcd = blockIdx.x & 7;
+  const int b = xcd * q + min(xcd, r) + (blockIdx.x >> 3);
   // matrix m with prefix[m] <= b < prefix[m + 1]
   int lo = 0, hi = a.count;
   while (hi - lo > 1) {
